@@ -1,0 +1,63 @@
+/*
+ * ORACLE - test infrastructure, not product code.
+ *
+ * A sequential FP64 CPU restatement, in plain C, of ECHO-QGP's per-timestep
+ * grid update (reference: gabriele-inghirami/echo-qgp-mhd-devel).  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it (oracle/liboracle.so), and only as the checker or the timed CPU
+ * baseline; the HIP product never links it.
+ *
+ * Every function cites the reference file:line it restates.  The oracle is
+ * pinned against the reference Fortran itself (oracle/_ref, built from the
+ * reference sources by oracle/build_ref.sh) and against closed forms; the
+ * pins live in tests/test_oracle_*.py.
+ *
+ * Layout: arrays are Fortran order (x fastest, variable slowest), 0-based:
+ * a[((k*nz + iz)*ny + iy)*nx + ix].
+ */
+#ifndef ECHO_ORACLE_H
+#define ECHO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Configuration, same meaning as the reference harness arrays
+ * (oracle/ref_harness.f90):
+ *  ic[0..2] nx,ny,nz   ic[3] coordinates 1/2   ic[4] viscosity
+ *  ic[5] bulk          ic[6] mhd 0/1/2         ic[7] divclean
+ *  ic[8] eqstate 1/2/3 ic[9] reconstruction    ic[10] nrk 2/3/4
+ *  ic[11] boundary_cond 0/2  ic[12] algo_solver  ic[13] obtained zz
+ *  ic[14] init_type    ic[15] derivatives_out
+ *  rc[0..5] x1,x2,y1,y2,z1,z2  rc[6] tmin  rc[7] cfl  rc[8] maxdt
+ *  rc[9] maxspeed  rc[10] eta_over_s  rc[11] tau_pi_coeff  rc[12] enezero
+ *  rc[13] glm_alpha rc[14] sigma_el  rc[15] sigma_chi  rc[16] dtout
+ *  rc[17] smooth_temp  rc[18] tmax
+ */
+int  orc_setup(const int *ic, const double *rc, const char *table_path);
+void orc_free(void);
+int  orc_nv(void);
+int  orc_init_from_prim(const double *vin);
+void orc_set_state(const double *v, const double *u);
+void orc_get_state(double *v, double *u);
+void orc_get_deriv(double *d);
+void orc_get_scalars(double *scal);
+int  orc_steps(int nsteps);
+
+/* pointwise operators at the current metric */
+void orc_set_time(double t);
+void orc_rec(int code, int n, int ngc_in, const double *f, double *gl, double *gr);
+int  orc_prim2cons(double *v, double *u);
+int  orc_cons2prim(double *u, double *v);
+int  orc_flux(double *v, int dir, double *u, double *f, double *vf);
+int  orc_eos_pressure(double rh, double en, double *out3);
+int  orc_eos_energy(double rh, double pr, double *en);
+int  orc_eos_sound(double rh, double en, double pr, double *cs2);
+int  orc_eos_derived(double rh, double pr, double *out3);
+int  orc_eos_table(int n, double *t, double *e, double *p, double *c);
+double orc_przero(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,569 @@
+/* ORACLE - test infrastructure, not product code (see oracle.h).
+ * Grid, boundaries, directional sweeps, sources, RK stage and time step:
+ * restates evolve.f90, work.f90 (HLL), common.f90:441-481 (grid),
+ * viscous.f90:56-97 (dts) and echo.f90:185-241 (step bookkeeping).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+#include "oracle_int.h"
+
+orc_state G;
+
+static void free_fields(void) {
+  free(G.v); free(G.u); free(G.u0); free(G.du); free(G.du_stiff);
+  free(G.deriv); free(G.vnewstep); free(G.vold); free(G.ax); free(G.ay); free(G.az);
+  free(G.xc); free(G.yc); free(G.zc);
+  G.v = G.u = G.u0 = G.du = G.du_stiff = G.deriv = G.vnewstep = G.vold = NULL;
+  G.ax = G.ay = G.az = G.xc = G.yc = G.zc = NULL;
+}
+
+void orc_free(void) {
+  free_fields();
+  free(G.tab_t); free(G.tab_e); free(G.tab_p); free(G.tab_c);
+  G.tab_t = G.tab_e = G.tab_p = G.tab_c = NULL;
+}
+
+int orc_nv(void) { return G.nv; }
+
+/* system_setup, system.f90:67-128 (HD / viscous) */
+static void system_setup(void) {
+  G.nov = 5; G.nou = 5;
+  G.nv = G.viscosity ? 12 : 5;
+  G.nu = G.nv;
+}
+
+/* common_grid, common.f90:441-481 */
+static void common_grid(void) {
+  G.dx = (G.x2 - G.x1) / G.nx;
+  G.dy = (G.y2 - G.y1) / G.ny;
+  G.dz = (G.z2 - G.z1) / G.nz;
+  G.ddx = G.nx > 1 ? 1. / G.dx : 0.;
+  G.ddy = G.ny > 1 ? 1. / G.dy : 0.;
+  G.ddz = G.nz > 1 ? 1. / G.dz : 0.;
+  G.xc = malloc(sizeof(double) * G.nx);
+  G.yc = malloc(sizeof(double) * G.ny);
+  G.zc = malloc(sizeof(double) * G.nz);
+  for (int i = 1; i <= G.nx; i++) G.xc[i - 1] = G.x1 + (i - 0.5) * G.dx;
+  for (int i = 1; i <= G.ny; i++) G.yc[i - 1] = G.y1 + (i - 0.5) * G.dy;
+  for (int i = 1; i <= G.nz; i++) G.zc[i - 1] = G.z1 + (i - 0.5) * G.dz;
+}
+
+int orc_setup(const int *ic, const double *rc, const char *table_path) {
+  orc_free();
+  G.nx = ic[0]; G.ny = ic[1]; G.nz = ic[2];
+  G.coordinates = ic[3];
+  G.viscosity = ic[4] != 0;
+  G.bulkvis = (ic[5] != 0) && G.viscosity;
+  G.mhd = ic[6] != 0; G.rmhd = ic[6] == 2;
+  G.divclean = (ic[7] != 0) && G.mhd;
+  G.eqstate = ic[8]; G.recal = ic[9]; G.nrk = ic[10];
+  G.boundary_cond = ic[11]; G.algo_solver = ic[12]; G.obtained_zz = ic[13] != 0;
+  G.init_type = ic[14]; G.derivatives_out = ic[15] != 0;
+  G.x1 = rc[0]; G.x2 = rc[1]; G.y1 = rc[2]; G.y2 = rc[3]; G.z1 = rc[4]; G.z2 = rc[5];
+  G.tmin = rc[6]; G.cfl = rc[7]; G.maxdt = rc[8]; G.maxspeed = rc[9];
+  G.eta_over_s = rc[10]; G.tau_pi_coeff = rc[11]; G.enezero = rc[12];
+  G.glm_alpha = rc[13]; G.sigma_el = rc[14]; G.sigma_chi = rc[15];
+  G.dtout = rc[16]; G.smooth_temp = rc[17]; G.tmax = rc[18];
+  G.enable_smooth_viscosity = G.smooth_temp > 0.;
+  G.glm_ch_old = 0.5;
+  if (G.mhd) return -11;   /* MHD is not restated by this oracle */
+  system_setup();
+  common_grid();
+  if (G.eqstate == 3) {
+    if (!table_path || eos_load_table(table_path)) return -1;
+  }
+  /* find_przero, eos.f90:192-218 */
+  {
+    double d1, d2;
+    int e = eos_pressure(0., G.enezero, &G.przero, &d1, &d2);
+    if (e || G.przero <= 0.) return -2;
+  }
+  size_t n = ncell();
+  G.v = calloc(n * G.nv, sizeof(double));
+  G.u = calloc(n * G.nv, sizeof(double));
+  G.u0 = calloc(n * G.nv, sizeof(double));
+  G.du = calloc(n * G.nv, sizeof(double));
+  G.du_stiff = calloc(n * G.nv, sizeof(double));
+  G.deriv = calloc(n * NDERIV, sizeof(double));
+  G.vnewstep = calloc(n * 5, sizeof(double));
+  G.vold = calloc(n * 5, sizeof(double));
+  G.ax = calloc((size_t)(G.nx + 1) * G.ny * G.nz * 2, sizeof(double));
+  G.ay = calloc((size_t)G.nx * (G.ny + 1) * G.nz * 2, sizeof(double));
+  G.az = calloc((size_t)G.nx * G.ny * (G.nz + 1) * 2, sizeof(double));
+  G.t = G.tmin; G.timeold = G.t; G.timeinterval = 0.; G.g_cov_3_old = 0.;
+  system_metric();
+  G.dt = G.maxdt;
+  G.tout = G.t + G.dtout;
+  G.iter = 0; G.run_crashed = 0; G.errcode = 0;
+  G.smooth_first = 1;
+  return G.nv;
+}
+
+/* init.f90:2937-2958: per-cell prim2cons followed by cons2prim */
+int orc_init_from_prim(const double *vin) {
+  size_t n = ncell();
+  double vv[MAXNV], uu[MAXNV];
+  for (int iz = 0; iz < G.nz; iz++)
+    for (int iy = 0; iy < G.ny; iy++)
+      for (int ix = 0; ix < G.nx; ix++) {
+        size_t c = cell_index(ix, iy, iz);
+        for (int k = 0; k < G.nv; k++) vv[k] = vin[k * n + c];
+        int e = G.viscosity ? sys_prim2cons_0(vv, uu) : sys_prim2cons(vv, uu);
+        if (e > 0) return e;
+        if (!G.viscosity) {
+          e = sys_cons2prim(uu, vv);
+          if (e > 0) return e;
+        }
+        for (int k = 0; k < G.nv; k++) { G.v[k * n + c] = vv[k]; G.u[k * n + c] = uu[k]; }
+      }
+  return 0;
+}
+
+void orc_set_state(const double *v, const double *u) {
+  size_t n = ncell() * G.nv;
+  memcpy(G.v, v, n * sizeof(double));
+  memcpy(G.u, u, n * sizeof(double));
+}
+
+void orc_get_state(double *v, double *u) {
+  size_t n = ncell() * G.nv;
+  memcpy(v, G.v, n * sizeof(double));
+  memcpy(u, G.u, n * sizeof(double));
+}
+
+void orc_get_deriv(double *d) { memcpy(d, G.deriv, ncell() * NDERIV * sizeof(double)); }
+
+void orc_get_scalars(double *s) {
+  s[0] = G.t; s[1] = G.dt; s[2] = G.timeold; s[3] = G.timeinterval;
+  s[4] = G.g_cov_3_old; s[5] = G.glm_ch_old; s[6] = G.tout; s[7] = G.iter;
+  s[8] = G.run_crashed; s[9] = G.errcode;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary fill of one pencil variable, evolve.f90:872-1127 (periodic 0,
+ * reflect +-1, zeroth-order outflow 2, third-order extrapolation 3).  p
+ * points at the first interior element; ghosts p[-NGC..-1], p[m..m+NGC-1]. */
+static void pencil_bc(double *p, int m, int ibc) {
+  switch (ibc) {
+    case 0:
+      for (int l = 1; l <= NGC; l++) { p[-NGC + l - 1] = p[m - NGC + l - 1]; p[m + l - 1] = p[l - 1]; }
+      break;
+    case 1:
+      for (int l = 1; l <= NGC; l++) { p[-NGC + l - 1] = p[NGC - l]; p[m + l - 1] = p[m - l]; }
+      break;
+    case -1:
+      for (int l = 1; l <= NGC; l++) { p[-NGC + l - 1] = -p[NGC - l]; p[m + l - 1] = -p[m - l]; }
+      break;
+    case 3:
+      for (int l = 1; l <= NGC; l++) {
+        p[-l] = p[-l + 3] - 3. * (p[-l + 2] - p[-l + 1]);
+      }
+      for (int l = 1; l <= NGC; l++) {
+        p[m + l - 1] = p[m + l - 4] - 3. * (p[m + l - 3] - p[m + l - 2]);
+      }
+      break;
+    default: /* 2 */
+      for (int l = 1; l <= NGC; l++) { p[-NGC + l - 1] = p[0]; p[m + l - 1] = p[m - 1]; }
+      break;
+  }
+}
+
+/* HLL of work.f90:195-229 on one interface: from wl/wr primitives to the
+ * flux w1[0..nv-1] and speeds w1[nv], w1[nv+1].  Returns an error code. */
+static int hll_face(double *wl, double *wr, int dir, double *w1) {
+  double ul[MAXNV], ur[MAXNV], fl[MAXNV], fr[MAXNV], vfl[2], vfr[2];
+  int e = sys_flux(wl, ul, fl, vfl, dir);
+  if (e > 0) return e;
+  e = sys_flux(wr, ur, fr, vfr, dir);
+  if (e > 0) return e;
+  double ap = fmax2(fmax2(0., vfl[0]), vfr[0]);
+  double am = fmax2(fmax2(0., -vfl[1]), -vfr[1]);
+  G.aflux[dir - 1] = fmax2(ap, am);
+  double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
+  for (int k = 0; k < G.nv; k++) w1[k] = a1 * (ap * fl[k] + am * fr[k] - ap * am * (ur[k] - ul[k]));
+  w1[G.nv] = ap;
+  w1[G.nv + 1] = am;
+  return 0;
+}
+
+/* One directional sweep, evolve_df{x,y,z} (evolve.f90:302-685).  dir = 1,2,3.
+ * Pencils are gathered along dir with NGC ghosts filled by pencil_bc; the
+ * Lorentz-velocity reconstruction feeds deriv only for viscous runs. */
+static int sweep(int dir) {
+  const int nx = G.nx, ny = G.ny, nz = G.nz, nv = G.nv;
+  const int m = dir == 1 ? nx : dir == 2 ? ny : nz;
+  const int n1 = dir == 1 ? ny : nx;           /* first transverse index */
+  const int n2 = dir == 3 ? ny : nz;           /* second transverse index */
+  const double dd = dir == 1 ? G.ddx : dir == 2 ? G.ddy : G.ddz;
+  const int len = m + 2 * NGC;
+  double *w0 = malloc(sizeof(double) * len * nv);
+  double *wd = malloc(sizeof(double) * len * 4);
+  double *wl = malloc(sizeof(double) * (m + 1) * nv), *wr = malloc(sizeof(double) * (m + 1) * nv);
+  double *wdl = malloc(sizeof(double) * (m + 1) * 4), *wdr = malloc(sizeof(double) * (m + 1) * 4);
+  double *w1 = malloc(sizeof(double) * (m + 1) * (nv + 2));
+  const size_t nc = ncell();
+  int error_flag = 0;
+  const double *g = G.g_cov;
+  double fl[MAXNV], fr[MAXNV];
+  for (int j2 = 0; j2 < n2 && !error_flag; j2++) {
+    for (int j1 = 0; j1 < n1; j1++) {
+      int ix = 0, iy = 0, iz = 0;
+      /* gather the pencil */
+      for (int i = 0; i < m; i++) {
+        if (dir == 1) { ix = i; iy = j1; iz = j2; }
+        else if (dir == 2) { ix = j1; iy = i; iz = j2; }
+        else { ix = j1; iy = j2; iz = i; }
+        size_t c = cell_index(ix, iy, iz);
+        for (int k = 0; k < nv; k++) w0[k * len + NGC + i] = G.v[k * nc + c];
+      }
+      for (int k = 0; k < nv; k++) pencil_bc(&w0[k * len + NGC], m, G.boundary_cond);
+      if (G.viscosity) {
+        for (int i = 0; i < len; i++) {
+          double vx = w0[KVX * len + i], vy = w0[KVY * len + i], vz = w0[KVZ * len + i];
+          double gl = 1. / sqrt(1. - vx * vx - vy * vy - g[2] * (vz * vz));
+          wd[0 * len + i] = gl;
+          wd[1 * len + i] = gl * vx;
+          wd[2 * len + i] = gl * vy;
+          wd[3 * len + i] = gl * vz;
+        }
+      }
+      for (int k = 0; k < nv; k++)
+        rec_pencil(G.recal, &w0[k * len + NGC - 1], &wl[k * (m + 1)], &wr[k * (m + 1)], m);
+      if (G.viscosity)
+        for (int k = 0; k < 4; k++)
+          rec_pencil(G.recal, &wd[k * len + NGC - 1], &wdl[k * (m + 1)], &wdr[k * (m + 1)], m);
+      /* reconstruction repair, evolve.f90:334-369.  Index i is the interface
+       * i+1/2; the reference repairs both sides with the cell left of it
+       * (w0(ix) with pencil index ix = i, i.e. cell i of the 1-based pencil,
+       * which is w0[NGC + i - 1] here). */
+      for (int i = 0; i <= m; i++) {
+        double *c0 = &w0[NGC + i - 1];
+#define W0(k) c0[(k) * len]
+#define WR(k) wr[(k) * (m + 1) + i]
+#define WL(k) wl[(k) * (m + 1) + i]
+        if (WR(KPR) <= 0) WR(KPR) = W0(KPR);
+        double v2 = WR(KVX) * WR(KVX) * g[0] + WR(KVY) * WR(KVY) * g[1] + WR(KVZ) * WR(KVZ) * g[2];
+        if (v2 >= 1.) {
+          WR(KVX) = W0(KVX); WR(KVY) = W0(KVY); WR(KVZ) = W0(KVZ);
+          v2 = WR(KVX) * WR(KVX) * g[0] + WR(KVY) * WR(KVY) * g[1] + WR(KVZ) * WR(KVZ) * g[2];
+          if (v2 >= 1) {
+            double s = sqrt(v2);
+            for (int k = KVX; k <= KVZ; k++) { W0(k) = W0(k) * G.maxspeed / s; WR(k) = W0(k); }
+          }
+        }
+        if (WL(KPR) <= 0) WL(KPR) = W0(KPR);
+        v2 = WL(KVX) * WL(KVX) * g[0] + WL(KVY) * WL(KVY) * g[1] + WL(KVZ) * WL(KVZ) * g[2];
+        if (v2 >= 1.) {
+          WL(KVX) = W0(KVX); WL(KVY) = W0(KVY); WL(KVZ) = W0(KVZ);
+          v2 = WL(KVX) * WL(KVX) * g[0] + WL(KVY) * WL(KVY) * g[1] + WL(KVZ) * WL(KVZ) * g[2];
+          if (v2 >= 1) {
+            double s = sqrt(v2);
+            for (int k = KVX; k <= KVZ; k++) { W0(k) = W0(k) * G.maxspeed / s; WL(k) = W0(k); }
+          }
+        }
+#undef W0
+#undef WR
+#undef WL
+      }
+      /* deriv(d?x) from face-averaged Lorentz velocities, evolve.f90:371-374 */
+      if (G.viscosity) {
+        static const int slot[3][4] = {{DTX, DXX, DYX, DZX}, {DTY, DXY, DYY, DZY}, {DTZ, DXZ, DYZ, DZZ}};
+        for (int i = 1; i <= m; i++) {
+          if (dir == 1) { ix = i - 1; iy = j1; iz = j2; }
+          else if (dir == 2) { ix = j1; iy = i - 1; iz = j2; }
+          else { ix = j1; iy = j2; iz = i - 1; }
+          for (int q = 0; q < 4; q++) {
+            double r = dd * 0.5 * (wdl[q * (m + 1) + i] + wdr[q * (m + 1) + i]
+                                   - wdl[q * (m + 1) + i - 1] - wdr[q * (m + 1) + i - 1]);
+            DERIV(ix, iy, iz, slot[dir - 1][q]) = r;
+          }
+        }
+      }
+      /* work_fup? HLL, work.f90:195-229 */
+      for (int i = 0; i <= m; i++) {
+        for (int k = 0; k < nv; k++) { fl[k] = wl[k * (m + 1) + i]; fr[k] = wr[k * (m + 1) + i]; }
+        int e = hll_face(fl, fr, dir, &w1[i * (nv + 2)]);
+        if (e > 0) { error_flag = e; break; }
+      }
+      if (error_flag) break;
+      /* du += dd*(F(i)-F(i-1)), evolve.f90:383; ax = (ap,am), 387 */
+      for (int i = 1; i <= m; i++) {
+        if (dir == 1) { ix = i - 1; iy = j1; iz = j2; }
+        else if (dir == 2) { ix = j1; iy = i - 1; iz = j2; }
+        else { ix = j1; iy = j2; iz = i - 1; }
+        size_t c = cell_index(ix, iy, iz);
+        for (int k = 0; k < G.nu; k++)
+          G.du[k * nc + c] = G.du[k * nc + c] + dd * (w1[i * (nv + 2) + k] - w1[(i - 1) * (nv + 2) + k]);
+      }
+      for (int i = 0; i <= m; i++) {
+        double *a;
+        if (dir == 1) a = &G.ax[(((size_t)j2 * ny + j1) * (nx + 1) + i) * 2];
+        else if (dir == 2) a = &G.ay[(((size_t)j2 * (ny + 1) + i) * nx + j1) * 2];
+        else a = &G.az[(((size_t)i * ny + j2) * nx + j1) * 2];
+        a[0] = w1[i * (nv + 2) + nv];
+        a[1] = w1[i * (nv + 2) + nv + 1];
+      }
+    }
+  }
+  free(w0); free(wd); free(wl); free(wr); free(wdl); free(wdr); free(w1);
+  if (error_flag) { G.errcode = error_flag; G.run_crashed = 1; }
+  return error_flag;
+}
+
+/* dts, viscous.f90:56-97: lagged first-order time derivatives of u^mu */
+static void dts(void) {
+  size_t nc = ncell();
+  if (G.timeinterval == 0) {
+    for (int k = DTT; k <= DZT; k++) memset(&G.deriv[k * nc], 0, nc * sizeof(double));
+    return;
+  }
+  const double *g = G.g_cov;
+  for (size_t c = 0; c < nc; c++) {
+    const double *qo = G.vold, *qn = G.vnewstep;
+    double ox = qo[KVX * nc + c], oy = qo[KVY * nc + c], oz = qo[KVZ * nc + c];
+    double nxv = qn[KVX * nc + c], nyv = qn[KVY * nc + c], nzv = qn[KVZ * nc + c];
+    double gamma1 = 1. / sqrt(1. - g[0] * (ox * ox) - g[1] * (oy * oy) - G.g_cov_3_old * (oz * oz));
+    double gamma2 = 1. / sqrt(1. - g[0] * (nxv * nxv) - g[1] * (nyv * nyv) - g[2] * (nzv * nzv));
+    G.deriv[DTT * nc + c] = (gamma2 - gamma1) / G.timeinterval;
+    G.deriv[DXT * nc + c] = (gamma2 * nxv - gamma1 * ox) / G.timeinterval;
+    G.deriv[DYT * nc + c] = (gamma2 * nyv - gamma1 * oy) / G.timeinterval;
+    G.deriv[DZT * nc + c] = (gamma2 * nzv - gamma1 * oz) / G.timeinterval;
+  }
+}
+
+/* evolve_src, evolve.f90:689-718 */
+static int evolve_src(void) {
+  size_t nc = ncell();
+  if (G.viscosity) dts();
+  double vv[MAXNV], src[MAXNV], srcs[MAXNV];
+  for (int iz = 0; iz < G.nz; iz++)
+    for (int iy = 0; iy < G.ny; iy++)
+      for (int ix = 0; ix < G.nx; ix++) {
+        size_t c = cell_index(ix, iy, iz);
+        for (int k = 0; k < G.nv; k++) vv[k] = G.v[k * nc + c];
+        int e = sys_source(ix, iy, iz, vv, src, srcs);
+        if (e > 0) { G.errcode = e; return e; }
+        for (int k = 0; k < G.nu; k++) {
+          G.du[k * nc + c] = G.du[k * nc + c] - src[k];
+          G.du_stiff[k * nc + c] = G.du_stiff[k * nc + c] - srcs[k];
+        }
+      }
+  return 0;
+}
+
+/* evolve_dt, evolve.f90:723-764 */
+static double evolve_dt(void) {
+  double amax = 0.;
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int ix = 0; ix < nx; ix++) {
+        if (nx > 1) {
+          const double *a = &G.ax[(((size_t)iz * ny + iy) * (nx + 1) + ix) * 2];
+          double m = fmax2(fmax2(fmax2(a[0], a[1]), a[2]), a[3]);
+          amax = fmax2(amax, G.ddx * m);
+        }
+        if (ny > 1) {
+          const double *a0 = &G.ay[(((size_t)iz * (ny + 1) + iy) * nx + ix) * 2];
+          const double *a1 = &G.ay[(((size_t)iz * (ny + 1) + iy + 1) * nx + ix) * 2];
+          double m = fmax2(fmax2(fmax2(a0[0], a1[0]), a0[1]), a1[1]);
+          amax = fmax2(amax, G.ddy * m);
+        }
+        if (nz > 1) {
+          const double *a0 = &G.az[(((size_t)iz * ny + iy) * nx + ix) * 2];
+          const double *a1 = &G.az[(((size_t)(iz + 1) * ny + iy) * nx + ix) * 2];
+          double m = fmax2(fmax2(fmax2(a0[0], a1[0]), a0[1]), a1[1]);
+          amax = fmax2(amax, G.ddz * m);
+        }
+      }
+  if (amax == 0) {
+    if (nx > 1) amax = fmax2(amax, G.ddx);
+    if (ny > 1) amax = fmax2(amax, G.ddy);
+    if (nz > 1) amax = fmax2(amax, G.ddz);
+  }
+  return G.cfl / amax;
+}
+
+/* evolve_u2v, evolve.f90:769-868.  The neighbour-average repair reads the
+ * six face neighbours; on boundary cells the reference reads outside the
+ * array (its u has no ghosts), here the index is clamped to the grid. */
+static int evolve_u2v(void) {
+  const size_t nc = ncell();
+  const double prlimit = 0.01;
+  (void)prlimit;
+  double uu[MAXNV], vv[MAXNV];
+  for (int iz = 0; iz < G.nz; iz++)
+    for (int iy = 0; iy < G.ny; iy++)
+      for (int ix = 0; ix < G.nx; ix++) {
+        size_t c = cell_index(ix, iy, iz);
+        for (int k = 0; k < G.nv; k++) { vv[k] = G.v[k * nc + c]; uu[k] = G.u[k * nc + c]; }
+        int e = sys_cons2prim(uu, vv);
+        if (e != 0) {
+          int xl = ix > 0 ? ix - 1 : ix, xr = ix < G.nx - 1 ? ix + 1 : ix;
+          int yl = iy > 0 ? iy - 1 : iy, yr = iy < G.ny - 1 ? iy + 1 : iy;
+          int zl = iz > 0 ? iz - 1 : iz, zr = iz < G.nz - 1 ? iz + 1 : iz;
+          size_t n6[6] = {cell_index(xl, iy, iz), cell_index(xr, iy, iz), cell_index(ix, yl, iz),
+                          cell_index(ix, yr, iz), cell_index(ix, iy, zl), cell_index(ix, iy, zr)};
+          for (int k = 0; k < G.nu; k++) {
+            double *a = &G.u[k * nc];
+            a[c] = (a[n6[0]] + a[n6[1]] + a[n6[2]] + a[n6[3]] + a[n6[4]] + a[n6[5]]) / 6.;
+            uu[k] = a[c];
+          }
+          sys_cons2prim(uu, vv);
+          const double *g = G.g_cov;
+          double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * g[2];
+          if (v2 > 1) {
+            double s = G.maxspeed / sqrt(v2);
+            vv[KVX] = s * vv[KVX]; vv[KVY] = s * vv[KVY]; vv[KVZ] = s * vv[KVZ];
+          }
+          if (vv[KPR] < 0) {
+            const double *p = &G.v[KPR * nc];
+            vv[KPR] = (p[n6[0]] + p[n6[1]] + p[n6[2]] + p[n6[3]] + p[n6[4]] + p[n6[5]]) / 6.;
+          }
+          sys_prim2cons(vv, uu);
+          sys_cons2prim(uu, vv);
+          v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * g[2];
+          e = ((vv[KPR] > 0) && (v2 < 1)) ? 0 : 1;
+        }
+        if (G.viscosity && G.obtained_zz) G.v[KPIZZ * nc + c] = vv[KPIZZ];
+        if (e > 0) { G.errcode = e; G.run_crashed = 1; return e; }
+        for (int k = 0; k < G.nv; k++) { G.v[k * nc + c] = vv[k]; G.u[k * nc + c] = uu[k]; }
+      }
+  return 0;
+}
+
+/* evolve_smooth_viscosity, evolve.f90:1159-1207.  The threshold pressure
+ * comes from find_pressure_from_temperature (eos.f90:1221-1260), restated
+ * here as a bisection on T(p) over the EoS range. */
+static int find_pressure_from_temperature(double temp, double *pres);
+static int evolve_smooth_viscosity(void) {
+  const size_t nc = ncell();
+  if (G.smooth_first) {
+    find_pressure_from_temperature(G.smooth_temp, &G.smooth_pressure);
+    G.smooth_first = 0;
+  }
+  double vv[MAXNV], uu[MAXNV];
+  for (size_t c = 0; c < nc; c++) {
+    double pr = G.v[KPR * nc + c];
+    if (pr < G.smooth_pressure) {
+      double r = pr / G.smooth_pressure;
+      for (int k = 0; k < G.nv; k++) vv[k] = G.v[k * nc + c];
+      for (int k = KPIBU; k <= KPIZZ; k++) vv[k] = vv[k] * (r * r);
+      if (G.obtained_zz) {
+        double a, b, cc, d, pizz;
+        get_derived_pi_zz(vv, &a, &b, &cc, &d, &pizz);
+        vv[KPIZZ] = pizz;
+      }
+      int e = sys_prim2cons(vv, uu);
+      if (e > 0) { G.errcode = e; G.run_crashed = 1; return e; }
+      for (int k = 0; k < G.nv; k++) { G.v[k * nc + c] = vv[k]; G.u[k * nc + c] = uu[k]; }
+    }
+  }
+  return 0;
+}
+
+/* find_pressure_from_temperature, eos.f90:1221-1252.  The reference leaves
+ * the lower bisection bound pr_low uninitialised; it starts at 0 here.  The
+ * bisection stops at |T(p) - T_cut| <= 1e-10 T_cut, so the threshold agrees
+ * with the reference to that tolerance whatever the starting bound. */
+static int find_pressure_from_temperature(double temp_input, double *pres) {
+  const double tolerance = 1.e-10;
+  double en_try = G.eqstate == 3 ? G.tab_e[0] : 0.;
+  if (en_try == 0.) en_try = tolerance / 2.;
+  double pr_try, d1, d2, en, temp_try, s;
+  eos_pressure(1., en_try, &pr_try, &d1, &d2);
+  get_derived_data(1., pr_try, &en, &temp_try, &s);
+  while (temp_try <= temp_input) {
+    get_derived_data(1., pr_try, &en, &temp_try, &s);
+    pr_try = pr_try * 1.1;
+  }
+  double pr_low = 0., pr_high = pr_try;
+  while (fabs(temp_try - temp_input) > temp_input * tolerance) {
+    pr_try = (pr_low + pr_high) / 2.;
+    get_derived_data(1., pr_try, &en, &temp_try, &s);
+    if (temp_try >= temp_input) pr_high = pr_try;
+    else pr_low = pr_try;
+  }
+  *pres = pr_try;
+  return 0;
+}
+
+/* evolve_main for explicit RK2/RK3, evolve.f90:114-203 + 276-295 */
+static int evolve_main(int irk) {
+  static const double crk[2][3][3] = {
+    /* [c][irk][nrk]: crk(c,irk,nrk) of evolve.f90:122-125 */
+    {{0., 0., 0.}, {0., 1. / 2., 3. / 4.}, {0., 0., 1. / 3.}},
+    {{1., 1., 1.}, {0., 1. / 2., 1. / 4.}, {0., 0., 2. / 3.}}};
+  const int nrk = G.nrk;
+  const double c1 = crk[0][irk - 1][nrk - 1], c2 = crk[1][irk - 1][nrk - 1];
+  const size_t n = ncell() * G.nu;
+  memset(G.du, 0, n * sizeof(double));
+  G.glm_ch = G.glm_ch_old;
+  if (G.nx > 1 && sweep(1)) return G.errcode;
+  if (G.ny > 1 && sweep(2)) return G.errcode;
+  if (G.nz > 1 && sweep(3)) return G.errcode;
+  if (evolve_src()) { G.run_crashed = 1; return G.errcode; }
+  G.glm_ch_old = fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
+  if (irk == 1) {
+    double dt_local = evolve_dt();
+    dt_local = fmin2(dt_local, G.maxdt);
+    dt_local = fmin2(dt_local, G.tout - G.t);
+    if (fabs(G.t + G.dt - G.tout) < 1.e-9) dt_local = G.tout - G.t;
+    else if ((G.t + 2. * dt_local - G.tout) > 1.e-9) dt_local = (G.tout - G.t) / 2.;
+    G.dt = dt_local;
+    memcpy(G.u0, G.u, n * sizeof(double));
+  }
+  for (size_t i = 0; i < n; i++) G.u[i] = c1 * G.u0[i] + c2 * (G.u[i] - G.dt * G.du[i]);
+  if (evolve_u2v()) return G.errcode;
+  if (G.viscosity && G.enable_smooth_viscosity) {
+    if (evolve_smooth_viscosity()) return G.errcode;
+  }
+  return 0;
+}
+
+/* main-loop body of echo.f90:185-262 without output */
+int orc_steps(int nsteps) {
+  const size_t nc = ncell();
+  for (int s = 0; s < nsteps; s++) {
+    memcpy(G.vnewstep, G.v, 5 * nc * sizeof(double));
+    for (int irk = 1; irk <= G.nrk; irk++) {
+      int e = evolve_main(irk);
+      if (e || G.run_crashed) { G.run_crashed = 1; return e ? e : 1; }
+    }
+    G.timeold = G.t;
+    G.t = G.t + G.dt;
+    G.g_cov_3_old = G.g_cov[2];
+    system_metric();
+    G.iter++;
+    G.timeinterval = G.dt;
+    memcpy(G.vold, G.vnewstep, 5 * nc * sizeof(double));
+    if ((G.t > G.tout) || ((G.tout - G.t) < 1.e-10)) {
+      G.tout = G.tout + G.dtout;
+      if ((fabs(G.tout - G.tmax) < 1.e-10) || (G.tout > G.tmax)) G.tout = G.tmax;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* pointwise wrappers */
+void orc_set_time(double t) { G.t = t; system_metric(); }
+
+void orc_rec(int code, int n, int ngc_in, const double *f, double *gl, double *gr) {
+  rec_pencil(code, f + ngc_in - 1, gl, gr, n);
+}
+int orc_prim2cons(double *v, double *u) { return sys_prim2cons(v, u); }
+int orc_cons2prim(double *u, double *v) { return sys_cons2prim(u, v); }
+int orc_flux(double *v, int dir, double *u, double *f, double *vf) { return sys_flux(v, u, f, vf, dir); }
+int orc_eos_pressure(double rh, double en, double *o) { return eos_pressure(rh, en, &o[0], &o[1], &o[2]); }
+int orc_eos_energy(double rh, double pr, double *en) { return eos_energy(rh, pr, en); }
+int orc_eos_sound(double rh, double en, double pr, double *cs2) { return eos_sound(rh, en, pr, cs2); }
+int orc_eos_derived(double rh, double pr, double *o) { return get_derived_data(rh, pr, &o[0], &o[1], &o[2]); }
+int orc_eos_table(int n, double *t, double *e, double *p, double *c) {
+  int m = n < G.entries ? n : G.entries;
+  for (int i = 0; i < m; i++) { t[i] = G.tab_t[i]; e[i] = G.tab_e[i]; p[i] = G.tab_p[i]; c[i] = G.tab_c[i]; }
+  return G.entries;
+}
+double orc_przero(void) { return G.przero; }
