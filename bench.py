@@ -1,0 +1,201 @@
+"""Benchmark of the MI355X grid update: zone-updates/s of full RK3 steps.
+
+Workload (one GPU): the 192 x 192 x 128 (x, y, eta) Milne grid of the
+single-event Au-Au run (dx = dy = 0.15 fm, deta = 0.1, tau0 = 0.4 fm,
+tabulated lattice-QCD EoS from qcdIEOS0.dat, eta/s = 0.1, zeta/s = 0,
+WENO5 + HLL + RK3, CFL 0.4), evolved as 3+1D second-order viscous hydro.
+The initial energy density is a seeded Monte-Carlo Glauber event (Woods-Saxon
+R = 6.38 fm, a = 0.535 fm, sigma_NN = 42 mb, b = 7 fm, Gaussian smearing
+0.4 fm) with a boost-invariant plateau and Gaussian tails in eta, and a
+Navier-Stokes initial shear tensor; all synthetic (no files read).
+
+One zone-update = one cell advanced through one full time step (3 RK
+stages).  value = cells * steps / wall time of the timed steps, with the
+state resident in HBM; each step is one replay of a captured HIP graph.
+
+Prints one JSON line (rank 0, N = 1 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "echo-qgp-mhd-devel_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+METRIC = "zone-updates/s per RK step; surrogate vs plain con2prim speedup at 1e-10 err"
+HBAR = 0.197326
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config(nx=192, ny=192, nz=128):
+    from libs import make_config
+    dx, deta = 0.15, 0.1
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=0, eqstate=3,
+                         rec="WENO5", nrk=3, bc=2, obtained_zz=1, init_type=3,
+                         xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
+                         zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=0.1,
+                         eta_over_s=0.1, tau_pi_coeff=5.0, enezero=0.001)
+    return ic, rc
+
+
+def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_nn=4.2,
+                   smear=0.4, e_per_participant=25.0):
+    """Seeded MC-Glauber transverse energy density (GeV/fm^3 at tau0)."""
+    rng = np.random.default_rng(seed)
+
+    def nucleus():
+        pts = []
+        while len(pts) < A:
+            r = rng.uniform(0, R + 6 * a, 4 * A)
+            keep = rng.uniform(0, 1, r.size) < (r / (R + 6 * a)) ** 2 / (1 + np.exp((r - R) / a))
+            pts.extend(r[keep].tolist())
+        r = np.array(pts[:A])
+        cth = rng.uniform(-1, 1, A)
+        phi = rng.uniform(0, 2 * np.pi, A)
+        sth = np.sqrt(1 - cth ** 2)
+        return np.stack([r * sth * np.cos(phi), r * sth * np.sin(phi)], axis=1)
+
+    na = nucleus() + [b / 2, 0]
+    nb = nucleus() - [b / 2, 0]
+    d2 = ((na[:, None, :] - nb[None, :, :]) ** 2).sum(-1)
+    hit = d2 < sigma_nn / np.pi
+    part = np.concatenate([na[hit.any(1)], nb[hit.any(0)]])
+    x = (np.arange(nx) + 0.5) * dx - nx * dx / 2
+    y = (np.arange(ny) + 0.5) * dx - ny * dx / 2
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    e = np.zeros_like(X)
+    for px, py in part:
+        e += np.exp(-((X - px) ** 2 + (Y - py) ** 2) / (2 * smear ** 2))
+    return e_per_participant * e / (2 * np.pi * smear ** 2), len(part)
+
+
+def workload_state(ic, rc, eos_rows):
+    nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
+    dx = (rc[1] - rc[0]) / nx
+    tau0 = rc[6]
+    eT, npart = glauber_energy(nx, ny, dx)
+    eta = (np.arange(nz) + 0.5) * (rc[5] - rc[4]) / nz + rc[4]
+    prof = np.exp(-np.maximum(np.abs(eta) - 2.0, 0.0) ** 2 / (2 * 1.0 ** 2))
+    e = eT[None, :, :] * prof[:, None, None] + 0.05
+    # table in physical units, eos.f90:323-327
+    hc3 = .00768351
+    T = eos_rows[:, 0]
+    te = eos_rows[:, 1] / hc3 * T ** 4
+    tp = eos_rows[:, 2] / hc3 * T ** 4
+    p = np.interp(e, te, tp)
+    temp = np.interp(e, te, T)
+    s = (e + p) / temp
+    eta_v = HBAR * rc[10] * s
+    v = np.zeros((12, nz, ny, nx))
+    v[0] = 1.0
+    v[4] = p
+    # Navier-Stokes initial shear of viscous_initio (viscous.f90:101-136)
+    v[9] = 2. * eta_v / (3. * tau0)
+    v[10] = 2. * eta_v / (3. * tau0)
+    v[11] = -tau0 ** 2 * 4. * eta_v / (3. * tau0)
+    return v, npart
+
+
+def algorithmic_bytes(nx, ny, nz, nv, visc):
+    N = nx * ny * nz
+    nf = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
+    lw = 4 if visc else 0
+    flux = [8 * (nv * N + (nv + lw) * f) for f in nf]
+    # stage >= 2: read F and L of 3 dirs, V, u0, u_cur (+ 2 x 3 velocity
+    # snapshots for dts); write u_out and V
+    upd = 8 * (sum((nv + lw) * f for f in nf) + nv * N * 3 + (6 * N if visc else 0) + 2 * nv * N)
+    return flux, upd
+
+
+def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=32, steps=1):
+    """Reference Fortran (oracle/_ref) or, if absent, the C oracle, on a
+    192x192xnz_sample slab of the same workload: one thread."""
+    from libs import Oracle, Ref, ref_available
+    ic = ic_full.copy()
+    rc = rc_full.copy()
+    ic[2] = nz_sample
+    deta = (rc_full[5] - rc_full[4]) / ic_full[2]
+    rc[4], rc[5] = -nz_sample * deta / 2, nz_sample * deta / 2
+    kind = "reference" if ref_available() else "port"
+    lib = Ref() if kind == "reference" else Oracle()
+    lib.setup(ic, rc)
+    v, _ = workload_state(ic, rc, eos_rows)
+    lib.init_from_prim(v)
+    t0 = time.perf_counter()
+    e = lib.steps(steps)
+    dt = time.perf_counter() - t0
+    cells = int(ic[0]) * int(ic[1]) * int(ic[2])
+    return {"value": cells * steps / dt, "unit": "zone-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{int(ic[0])}x{int(ic[1])}x{nz_sample} slab of the same workload, "
+                      f"{steps} RK3 step(s), {dt:.1f} s, single thread"
+                      + ("" if e == 0 else f", crashed with code {e}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=192)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU sharding "
+              "of the grid is a design note, not part of this benchmark", file=sys.stderr)
+        sys.exit(2)
+    from echoamd import EchoGrid, read_eos_rows
+    eos_rows = read_eos_rows()
+    ic, rc = workload_config(args.nx, args.ny, args.nz)
+    v0, npart = workload_state(ic, rc, eos_rows)
+    g = EchoGrid(ic, rc, device=0, eos_rows=eos_rows)
+    g.init_from_prim(v0)
+    nx, ny, nz, nv = args.nx, args.ny, args.nz, g.nv
+    N = nx * ny * nz
+    g.steps(args.warmup)
+    t0 = time.perf_counter()
+    g.steps(args.steps)          # returns after the device has finished
+    wall = time.perf_counter() - t0
+    sc = g.scalars()
+    # per-kernel durations, HIP events on the grid's stream (plain launches)
+    g.set_timing(True)
+    g.steps(3)
+    tk = g.timing()
+    g.set_timing(False)
+    flux_b, upd_b = algorithmic_bytes(nx, ny, nz, nv, True)
+    kern = {"flux_x": (tk["flux_x"], flux_b[0]), "flux_y": (tk["flux_y"], flux_b[1]),
+            "flux_z": (tk["flux_z"], flux_b[2]), "update": (tk["update"], upd_b)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, byt = kern[dom]
+    achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    value = N * args.steps / wall
+    out = {
+        "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded MC-Glauber initial state)",
+        "config": {"workload": f"single-event Au-Au grid {nx}x{ny}x{nz} (x,y,eta), 3+1D "
+                               "second-order viscous hydro (eta/s=0.1), tabulated EoS, "
+                               "WENO5+HLL+RK3; magnetic/resistive terms not in this build",
+                   "cells": N, "nv": nv, "stages_per_step": 3, "participants": npart,
+                   "t_end": sc["t"]},
+        "roofline": {"kernel": f"k_flux/k_update ({dom})", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "ms_per_launch": {k: kern[k][0] for k in kern},
+                     "algorithmic_bytes_per_launch": {k: kern[k][1] for k in kern}},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ic, rc, eos_rows)
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,487 @@
+// C-ABI of the MI355X grid update (declared in include/echo_amd.h): device
+// buffers, launch sequence of one RK step, and the HIP graph that replays a
+// whole step without host involvement.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/echo_amd.h"
+#include "kernels.hpp"
+
+using namespace echo;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(x)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(ECHO_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct echo_grid {
+  echo_config cfg;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int nv = 0;
+  long long N = 0;
+  long long nf[3] = {0, 0, 0};
+  int nb[3] = {0, 0, 0};
+  bool active[3] = {false, false, false};
+  double* V = nullptr;
+  double* U[2] = {nullptr, nullptr};
+  double* F[3] = {nullptr, nullptr, nullptr};
+  double* L[3] = {nullptr, nullptr, nullptr};
+  double* VS[2] = {nullptr, nullptr};
+  double* partial = nullptr;
+  double* tab = nullptr;
+  int n_tab = 0;
+  DevScalars* S = nullptr;
+  int* faillist = nullptr;
+  int* status = nullptr;
+  GridArgs ga;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  double tms[4] = {0, 0, 0, 0};
+  int next_stage = 1;
+};
+
+// RK2/RK3 Shu-Osher coefficients crk(1:2, irk, nrk) of evolve.f90:122-125
+static void rk_coeffs(int nrk, int irk, double& c1, double& c2) {
+  static const double C[2][3][3] = {
+      {{0., 0., 0.}, {0., 1. / 2., 3. / 4.}, {0., 0., 1. / 3.}},
+      {{1., 1., 1.}, {0., 1. / 2., 1. / 4.}, {0., 0., 2. / 3.}}};
+  c1 = C[0][irk - 1][nrk - 1];
+  c2 = C[1][irk - 1][nrk - 1];
+}
+
+static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// kernel dispatch
+// ---------------------------------------------------------------------------
+template <int DIR, bool VISC, bool ZZ>
+static void launch_flux_dir(echo_grid* g, double* partial) {
+  dim3 grid(g->nb[DIR - 1]), block(FLUX_BLOCK);
+  double* F = g->F[DIR - 1];
+  double* L = g->L[DIR - 1];
+  switch (g->cfg.recal) {
+    case REC_TVD2: k_flux<DIR, REC_TVD2, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_CENO3: k_flux<DIR, REC_CENO3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_WENO3: k_flux<DIR, REC_WENO3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_PPM4: k_flux<DIR, REC_PPM4, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_MPE3: k_flux<DIR, REC_MPE3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_MPE5: k_flux<DIR, REC_MPE5, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    default: k_flux<DIR, REC_WENO5, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+  }
+}
+
+template <bool VISC, bool ZZ>
+static void enqueue_stage_t(echo_grid* g, int irk) {
+  const int nrk = g->cfg.nrk;
+  double c1, c2;
+  rk_coeffs(nrk, irk, c1, c2);
+  double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
+  if (g->timing) hipEventRecord(g->ev[0], g->stream);
+  if (g->active[0]) launch_flux_dir<1, VISC, ZZ>(g, part[0]);
+  if (g->timing) hipEventRecord(g->ev[1], g->stream);
+  if (g->active[1]) launch_flux_dir<2, VISC, ZZ>(g, part[1]);
+  if (g->timing) hipEventRecord(g->ev[2], g->stream);
+  if (g->active[2]) launch_flux_dir<3, VISC, ZZ>(g, part[2]);
+  if (g->timing) hipEventRecord(g->ev[3], g->stream);
+  if (irk == 1) {
+    k_cfl<<<1, 1024, 0, g->stream>>>(g->ga, g->partial, g->nb[0], g->nb[1], g->nb[2]);
+  }
+  if (g->timing) hipEventRecord(g->ev[4], g->stream);
+  // stage 1 reads u^n from U[0] and writes U[1]; later stages update U[1]
+  // in place; the last stage writes u^{n+1} back into U[0] (each cell reads
+  // its own u0 before overwriting it), so buffer roles repeat every step.
+  const bool last = (irk == nrk);
+  double* Ucur = (irk == 1) ? g->U[0] : g->U[1];
+  double* Uout = last ? g->U[0] : g->U[1];
+  k_update<VISC, ZZ><<<nblocks(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+      g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
+      g->VS[0], g->VS[1], g->faillist, c1, c2, last ? 1 : 0);
+  if (g->timing) hipEventRecord(g->ev[5], g->stream);
+  // con2prim repair of the cells k_update could not invert (rare); it
+  // writes the next step's vnewstep too, before k_end_step flips the parity
+  k_fixup<VISC, ZZ><<<1, 64, 0, g->stream>>>(g->ga, g->V, Uout, g->faillist, g->VS[0],
+                                              g->VS[1], last ? 1 : 0);
+}
+
+static void enqueue_stage(echo_grid* g, int irk) {
+  if (!g->cfg.viscosity) enqueue_stage_t<false, false>(g, irk);
+  else if (g->cfg.obtained_zz) enqueue_stage_t<true, true>(g, irk);
+  else enqueue_stage_t<true, false>(g, irk);
+}
+
+static void enqueue_end_step(echo_grid* g) {
+  k_end_step<<<1, 64, 0, g->stream>>>(g->S, g->cfg.coordinates, g->cfg.viscosity ? 1 : 0);
+}
+
+static void enqueue_step(echo_grid* g) {
+  for (int irk = 1; irk <= g->cfg.nrk; irk++) enqueue_stage(g, irk);
+  enqueue_end_step(g);
+}
+
+// ---------------------------------------------------------------------------
+// host restatement of find_pressure_from_temperature (eos.f90:1221-1252) for
+// the viscous smoothing threshold, with the lower bound started at 0 (the
+// reference leaves it uninitialised)
+static double smooth_threshold(const Eos& E, double temp_input) {
+  const double tolerance = 1.e-10;
+  double en_try = (E.eqstate == 3) ? E.te[0] : 0.;
+  if (en_try == 0.) en_try = tolerance / 2.;
+  double pr_try, d1, d2, en, temp_try, s;
+  eos_pressure(E, 1., en_try, pr_try, d1, d2);
+  get_derived_data(E, 1., pr_try, en, temp_try, s);
+  while (temp_try <= temp_input) {
+    get_derived_data(E, 1., pr_try, en, temp_try, s);
+    pr_try = pr_try * 1.1;
+  }
+  double pr_low = 0., pr_high = pr_try;
+  while (fabs(temp_try - temp_input) > temp_input * tolerance) {
+    pr_try = (pr_low + pr_high) / 2.;
+    get_derived_data(E, 1., pr_try, en, temp_try, s);
+    if (temp_try >= temp_input) pr_high = pr_try; else pr_low = pr_try;
+  }
+  return pr_try;
+}
+
+static void release(echo_grid* g) {
+  if (!g) return;
+  hipSetDevice(g->dev);
+  if (g->gexec) hipGraphExecDestroy(g->gexec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  hipFree(g->V); hipFree(g->U[0]); hipFree(g->U[1]);
+  for (int d = 0; d < 3; d++) { hipFree(g->F[d]); hipFree(g->L[d]); }
+  hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->partial); hipFree(g->tab);
+  hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
+  for (auto& e : g->ev) if (e) hipEventDestroy(e);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+}
+
+extern "C" {
+
+const char* echo_last_error(void) { return g_err.c_str(); }
+
+int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, int32_t device,
+                echo_grid** out) {
+  if (!cfg || !out) return fail(ECHO_E_ARG, "echo_create: null argument");
+  *out = nullptr;
+  const echo_config& c = *cfg;
+  if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(ECHO_E_ARG, "grid sizes must be >= 1");
+  if (c.mhd != 0) return fail(ECHO_E_ARG, "MHD=%d is not supported by this build", c.mhd);
+  if (c.coordinates != 1 && c.coordinates != 2) return fail(ECHO_E_ARG, "COORD must be 1 or 2");
+  if (c.nrk != 2 && c.nrk != 3) return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2) or RK3 (3)");
+  if (c.boundary_cond != 0 && c.boundary_cond != 2)
+    return fail(ECHO_E_ARG, "boundary_cond must be 0 (periodic) or 2 (outflow)");
+  if (c.recal < 1 || c.recal > 7) return fail(ECHO_E_ARG, "unsupported reconstruction code %d", c.recal);
+  if (c.eqstate < 1 || c.eqstate > 3) return fail(ECHO_E_ARG, "EQ_STATE must be 1, 2 or 3");
+  if (c.eqstate == 3 && (!eos_rows || n_rows < 2)) return fail(ECHO_E_ARG, "tabulated EoS needs rows");
+  for (int d = 0; d < 3; d++) {
+    int n = d == 0 ? c.nx : d == 1 ? c.ny : c.nz;
+    if (n > 1 && n < 3) return fail(ECHO_E_ARG, "active dimensions need at least 3 cells");
+  }
+  echo_grid* g = new echo_grid();
+  g->cfg = c;
+  g->dev = device;
+  hipError_t he = hipSetDevice(device);
+  if (he != hipSuccess) { delete g; return fail(ECHO_E_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
+  g->nv = c.viscosity ? 12 : 5;
+  g->N = (long long)c.nx * c.ny * c.nz;
+  g->nf[0] = (long long)(c.nx + 1) * c.ny * c.nz;
+  g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
+  g->nf[2] = (long long)c.nx * c.ny * (c.nz + 1);
+  g->active[0] = c.nx > 1; g->active[1] = c.ny > 1; g->active[2] = c.nz > 1;
+  for (int d = 0; d < 3; d++) g->nb[d] = (int)nblocks(g->nf[d], FLUX_BLOCK);
+  const size_t szs = sizeof(double) * (size_t)g->nv * g->N;
+#define ALLOC(p, bytes)                                                           \
+  do {                                                                            \
+    if (hipMalloc(&(p), (bytes)) != hipSuccess) {                                 \
+      release(g);                                                                 \
+      return fail(ECHO_E_NOMEM, "hipMalloc of %zu bytes failed", (size_t)(bytes)); \
+    }                                                                             \
+  } while (0)
+  ALLOC(g->V, szs);
+  ALLOC(g->U[0], szs);
+  ALLOC(g->U[1], szs);
+  for (int d = 0; d < 3; d++) {
+    if (!g->active[d]) continue;
+    ALLOC(g->F[d], sizeof(double) * (size_t)g->nv * g->nf[d]);
+    if (c.viscosity) ALLOC(g->L[d], sizeof(double) * 4 * (size_t)g->nf[d]);
+  }
+  if (c.viscosity) {
+    ALLOC(g->VS[0], sizeof(double) * 3 * (size_t)g->N);
+    ALLOC(g->VS[1], sizeof(double) * 3 * (size_t)g->N);
+  }
+  ALLOC(g->partial, sizeof(double) * (size_t)(g->nb[0] + g->nb[1] + g->nb[2] + 1));
+  ALLOC(g->S, sizeof(DevScalars));
+  ALLOC(g->faillist, sizeof(int) * FAIL_CAP);
+  ALLOC(g->status, sizeof(int));
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+    release(g);
+    return fail(ECHO_E_HIP, "stream creation failed");
+  }
+  // EoS: the table conversion of read_data_from_tabulated_file (eos.f90:323-327)
+  Eos E{};
+  E.eqstate = c.eqstate;
+  E.enezero = c.enezero;
+  std::vector<double> tab;
+  if (c.eqstate == 3) {
+    const double hc3 = .00768351;
+    g->n_tab = n_rows;
+    tab.resize(4 * (size_t)n_rows);
+    for (int i = 0; i < n_rows; i++) {
+      double T = eos_rows[4 * i], e = eos_rows[4 * i + 1], p = eos_rows[4 * i + 2], s = eos_rows[4 * i + 3];
+      tab[i] = T;
+      tab[n_rows + i] = e / hc3 * pow(T, 4.0);
+      tab[2 * n_rows + i] = p / hc3 * pow(T, 4.0);
+      tab[3 * n_rows + i] = s;
+    }
+    ALLOC(g->tab, sizeof(double) * tab.size());
+    if (hipMemcpy(g->tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      release(g);
+      return fail(ECHO_E_HIP, "table upload failed");
+    }
+    E.n = n_rows;
+  }
+#undef ALLOC
+  // find_przero (eos.f90:192-218), evaluated on the host table copy
+  {
+    Eos Eh = E;
+    if (c.eqstate == 3) {
+      Eh.tt = tab.data(); Eh.te = tab.data() + n_rows; Eh.tp = tab.data() + 2 * n_rows; Eh.tc = tab.data() + 3 * n_rows;
+    }
+    double pz, d1, d2;
+    int e = eos_pressure(Eh, 0., c.enezero, pz, d1, d2);
+    if (e || pz <= 0.) { release(g); return fail(ECHO_E_ARG, "ENEZERO gives no positive pressure"); }
+    E.przero = pz;
+    Eh.przero = pz;
+    DevScalars S0{};
+    S0.t = c.tmin; S0.timeold = c.tmin; S0.timeinterval = 0.; S0.g_cov_3_old = 0.;
+    S0.dt = c.maxdt; S0.tout = c.tmin + c.dtout; S0.dtout = c.dtout; S0.tmax = c.tmax;
+    S0.glm_ch_old = 0.5;
+    S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? smooth_threshold(Eh, c.smooth_temp) : 0.;
+    if (hipMemcpy(g->S, &S0, sizeof S0, hipMemcpyHostToDevice) != hipSuccess) {
+      release(g);
+      return fail(ECHO_E_HIP, "scalar upload failed");
+    }
+  }
+  if (c.eqstate == 3) {
+    E.tt = g->tab; E.te = g->tab + n_rows; E.tp = g->tab + 2 * n_rows; E.tc = g->tab + 3 * n_rows;
+  }
+  GridArgs& a = g->ga;
+  a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.coordinates = c.coordinates; a.bc = c.boundary_cond;
+  a.N = g->N;
+  // common_grid, common.f90:441-481
+  a.ddx = c.nx > 1 ? 1. / ((c.x2 - c.x1) / c.nx) : 0.;
+  a.ddy = c.ny > 1 ? 1. / ((c.y2 - c.y1) / c.ny) : 0.;
+  a.ddz = c.nz > 1 ? 1. / ((c.z2 - c.z1) / c.nz) : 0.;
+  a.maxspeed = c.maxspeed; a.cfl = c.cfl; a.maxdt = c.maxdt;
+  a.eos = E;
+  a.vp.eta_over_s = c.eta_over_s; a.vp.tau_pi_coeff = c.tau_pi_coeff;
+  a.vp.bulkvis = c.viscosity && c.bulk; a.vp.init_type = c.init_type;
+  a.enable_smooth = c.viscosity && c.smooth_temp > 0.;
+  a.S = g->S;
+  if (hipMemset(g->faillist, 0, sizeof(int) * FAIL_CAP) != hipSuccess) {
+    release(g);
+    return fail(ECHO_E_HIP, "memset failed");
+  }
+  for (auto& e : g->ev) hipEventCreate(&e);
+  *out = g;
+  return 0;
+}
+
+void echo_destroy(echo_grid* g) { release(g); }
+
+int echo_nv(const echo_grid* g) { return g ? g->nv : 0; }
+
+int echo_upload_state(echo_grid* g, const double* v, const double* u) {
+  if (!g || !v || !u) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t sz = sizeof(double) * (size_t)g->nv * g->N;
+  HIPCHECK(hipMemcpyAsync(g->V, v, sz, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(g->U[0], u, sz, hipMemcpyHostToDevice, g->stream));
+  if (g->cfg.viscosity) {
+    // vnewstep of the first step: the uploaded velocities
+    HIPCHECK(hipMemcpyAsync(g->VS[0], v + g->N * 1, sizeof(double) * 3 * g->N, hipMemcpyHostToDevice, g->stream));
+    int par = 0;
+    HIPCHECK(hipMemcpyAsync(&g->S->vs_parity, &par, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int echo_download_state(echo_grid* g, double* v, double* u) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t sz = sizeof(double) * (size_t)g->nv * g->N;
+  if (v) HIPCHECK(hipMemcpyAsync(v, g->V, sz, hipMemcpyDeviceToHost, g->stream));
+  if (u) HIPCHECK(hipMemcpyAsync(u, g->U[0], sz, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int echo_init_from_prim(echo_grid* g, const double* v) {
+  if (!g || !v) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t sz = sizeof(double) * (size_t)g->nv * g->N;
+  HIPCHECK(hipMemcpyAsync(g->V, v, sz, hipMemcpyHostToDevice, g->stream));
+  int zero = 0;
+  HIPCHECK(hipMemcpyAsync(g->status, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(&g->S->vs_parity, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  dim3 grid(nblocks(g->N, UPDATE_BLOCK));
+  if (!g->cfg.viscosity)
+    k_init<false, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
+  else if (g->cfg.obtained_zz)
+    k_init<true, true><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
+  else
+    k_init<true, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
+  HIPCHECK(hipGetLastError());
+  int st = 0;
+  HIPCHECK(hipMemcpyAsync(&st, g->status, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return st;
+}
+
+static int read_crash(echo_grid* g) {
+  DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  if (S.run_crashed) return S.errcode > 0 ? S.errcode : ECHO_E_CRASHED;
+  return 0;
+}
+
+int echo_rk_stage(echo_grid* g, int32_t irk) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (irk != g->next_stage || irk < 1 || irk > g->cfg.nrk)
+    return fail(ECHO_E_ARG, "stages must be called in order 1..nrk (expected %d)", g->next_stage);
+  HIPCHECK(hipSetDevice(g->dev));
+  enqueue_stage(g, irk);
+  HIPCHECK(hipGetLastError());
+  g->next_stage = irk + 1;
+  return read_crash(g);
+}
+
+int echo_end_step(echo_grid* g) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != g->cfg.nrk + 1) return fail(ECHO_E_ARG, "echo_end_step before the last stage");
+  HIPCHECK(hipSetDevice(g->dev));
+  enqueue_end_step(g);
+  HIPCHECK(hipGetLastError());
+  g->next_stage = 1;
+  return read_crash(g);
+}
+
+int echo_steps(echo_grid* g, int32_t nsteps) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_steps inside a step");
+  HIPCHECK(hipSetDevice(g->dev));
+  if (g->timing) {
+    // timed path: plain launches with events around each kernel
+    double acc[4] = {0, 0, 0, 0};
+    int nst = 0;
+    for (int s = 0; s < nsteps; s++) {
+      for (int irk = 1; irk <= g->cfg.nrk; irk++) {
+        enqueue_stage(g, irk);
+        HIPCHECK(hipEventSynchronize(g->ev[5]));
+        float ms;
+        for (int d = 0; d < 3; d++) {
+          hipEventElapsedTime(&ms, g->ev[d], g->ev[d + 1]);
+          acc[d] += ms;
+        }
+        hipEventElapsedTime(&ms, g->ev[4], g->ev[5]);
+        acc[3] += ms;
+        nst++;
+      }
+      enqueue_end_step(g);
+    }
+    for (int i = 0; i < 4; i++) g->tms[i] = nst ? acc[i] / nst : 0.;
+    HIPCHECK(hipGetLastError());
+    return read_crash(g);
+  }
+  if (!g->gexec) {
+    HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(g);
+    hipGraph_t graph;
+    HIPCHECK(hipStreamEndCapture(g->stream, &graph));
+    g->graph = graph;
+    HIPCHECK(hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHECK(hipGraphLaunch(g->gexec, g->stream));
+  HIPCHECK(hipGetLastError());
+  return read_crash(g);
+}
+
+int echo_get_scalars(echo_grid* g, double* s) {
+  if (!g || !s) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  s[0] = S.t; s[1] = S.dt; s[2] = S.timeold; s[3] = S.timeinterval; s[4] = S.g_cov_3_old;
+  s[5] = S.glm_ch_old; s[6] = S.tout; s[7] = S.iter; s[8] = S.run_crashed; s[9] = S.errcode;
+  return 0;
+}
+
+int echo_set_tout(echo_grid* g, double tout) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  HIPCHECK(hipMemcpyAsync(&g->S->tout, &tout, sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int echo_cons2prim_batch(echo_grid* g, const double* u, double* v, int32_t* status, int64_t n) {
+  if (!g || !u || !v || !status || n < 0) return fail(ECHO_E_ARG, "invalid argument");
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t sz = sizeof(double) * (size_t)g->nv * n;
+  double *du = nullptr, *dv = nullptr;
+  int* ds = nullptr;
+  HIPCHECK(hipMalloc(&du, sz));
+  HIPCHECK(hipMalloc(&dv, sz));
+  HIPCHECK(hipMalloc(&ds, sizeof(int) * n));
+  HIPCHECK(hipMemcpyAsync(du, u, sz, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(dv, v, sz, hipMemcpyHostToDevice, g->stream));
+  dim3 grid(nblocks(n, UPDATE_BLOCK));
+  if (!g->cfg.viscosity) k_c2p_batch<false, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
+  else if (g->cfg.obtained_zz) k_c2p_batch<true, true><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
+  else k_c2p_batch<true, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(v, dv, sz, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipMemcpyAsync(status, ds, sizeof(int) * n, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  hipFree(du); hipFree(dv); hipFree(ds);
+  return 0;
+}
+
+int echo_set_timing(echo_grid* g, int32_t on) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  g->timing = on != 0;
+  return 0;
+}
+
+int echo_get_timing(echo_grid* g, double* t) {
+  if (!g || !t) return fail(ECHO_E_ARG, "null argument");
+  for (int i = 0; i < 4; i++) t[i] = g->tms[i];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,497 @@
+// Kernels of one explicit RK stage of the ECHO-QGP grid update on gfx950.
+//
+// Data layout in HBM (FP64, variable-major SoA, x fastest; the reference's
+// Fortran layout of v/u, common.f90:203-214):
+//   V            nv * N       primitives (updated in place by k_update)
+//   U[0], U[1]   nv * N each  conserved u^n and the stage value
+//   F[d]         nv * Nf[d]   HLL flux on the interfaces normal to d
+//   L[d]         4 * Nf[d]    face sums wdl+wdr of (gamma, gamma v^i), viscous
+//   VS[0], VS[1] 3 * N        v^i at the start of the current / previous step
+//                             (vnewstep / vold of echo.f90:192-241), viscous
+// Interfaces normal to x are indexed i + (nx+1)*(y + ny*z), i = 0..nx, the
+// interface i lying between cells i-1 and i (0-based); likewise for y and z
+// with the interface index in its own dimension and x always fastest, so a
+// wavefront of consecutive threads reads consecutive x.
+#pragma once
+#include "source.hpp"
+
+namespace echo {
+
+struct DevScalars {
+  double t, dt, timeold, timeinterval, g_cov_3_old, tout, glm_ch, glm_ch_old;
+  double smooth_pressure, dtout, tmax, pad;
+  int iter, run_crashed, errcode, nfail;
+  int vs_parity, pad2, pad3, pad4;
+};
+
+struct GridArgs {
+  int nx, ny, nz, coordinates, bc;
+  long long N;
+  double ddx, ddy, ddz, maxspeed, cfl, maxdt;
+  Eos eos;
+  ViscParams vp;
+  int enable_smooth;
+  DevScalars* S;
+};
+
+constexpr int FLUX_BLOCK = 256;
+constexpr int UPDATE_BLOCK = 256;
+constexpr int FAIL_CAP = 4096;
+
+// cell index along a direction of length m for a stencil position c0
+// (0-based, may lie in the ghost range -3..m+2): boundary conditions of
+// evolve_bc{x,y,z}, evolve.f90:872-1127 (0 periodic, 2 zeroth-order outflow)
+__device__ __forceinline__ int bc_map(int c0, int m, int bc) {
+  if (bc == 0) return c0 < 0 ? c0 + m : (c0 >= m ? c0 - m : c0);
+  return c0 < 0 ? 0 : (c0 >= m ? m - 1 : c0);
+}
+
+// ---------------------------------------------------------------------------
+// k_flux: one thread per interface normal to DIR.  Reconstructs the nv
+// primitives (WENO5/PPM4/TVD2/...) from the six cells around the interface,
+// applies the reference's repair of unphysical states (evolve.f90:334-369),
+// evaluates the HLL flux (work.f90:195-229 with system_flux of
+// system.f90:1316-1583) and writes it, plus a per-block max of the fast
+// speeds for the CFL condition (evolve.f90:723-764).
+// ---------------------------------------------------------------------------
+template <int DIR, int REC, bool VISC, bool ZZ>
+__global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* __restrict__ V,
+                                                     double* __restrict__ F, double* __restrict__ L,
+                                                     double* __restrict__ partial) {
+  constexpr int NV = VISC ? 12 : 5;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int m = DIR == 1 ? nx : (DIR == 2 ? ny : nz);
+  const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
+                     : (DIR == 2 ? (long long)nx * (ny + 1) * nz : (long long)nx * ny * (nz + 1));
+  const long long f = (long long)blockIdx.x * FLUX_BLOCK + threadIdx.x;
+  double amax = 0.;
+  if (f < nf) {
+    int x, y, z, i;
+    long long base, stride;
+    if constexpr (DIR == 1) {
+      i = (int)(f % (nx + 1)); long long r = f / (nx + 1); y = (int)(r % ny); z = (int)(r / ny);
+      base = ((long long)z * ny + y) * nx; stride = 1;
+    } else if constexpr (DIR == 2) {
+      x = (int)(f % nx); long long r = f / nx; i = (int)(r % (ny + 1)); z = (int)(r / (ny + 1));
+      base = (long long)z * ny * nx + x; stride = nx;
+    } else {
+      x = (int)(f % nx); long long r = f / nx; y = (int)(r % ny); i = (int)(r / ny);
+      base = (long long)y * nx + x; stride = (long long)nx * ny;
+    }
+    (void)x; (void)y; (void)z;
+    // interface i lies between cells i-1 and i (0-based); stencil cells i-3 .. i+2
+    long long off[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) off[j] = base + stride * bc_map(i - 3 + j, m, a.bc);
+
+    const DevScalars* S = a.S;
+    const Metric M = make_metric(a.coordinates, S->t);
+    double wl[NV], wr[NV], c0[NV];
+    double vel[3][6];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s[6];
+      const double* Vk = V + (long long)k * a.N;
+#pragma unroll
+      for (int j = 0; j < 6; j++) s[j] = Vk[off[j]];
+      rec_face<REC>(s, wl[k], wr[k]);
+      c0[k] = s[2];
+      if constexpr (VISC) {
+        if (k >= KVX && k <= KVZ) {
+#pragma unroll
+          for (int j = 0; j < 6; j++) vel[k - KVX][j] = s[j];
+        }
+      }
+    }
+    if constexpr (VISC) {
+      // Lorentz-velocity reconstruction feeding deriv, evolve.f90:324-331,371-374
+      double wd[4][6];
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        double vx = vel[0][j], vy = vel[1][j], vz = vel[2][j];
+        double gl = 1. / sqrt(1. - vx * vx - vy * vy - M.g3 * (vz * vz));
+        wd[0][j] = gl; wd[1][j] = gl * vx; wd[2][j] = gl * vy; wd[3][j] = gl * vz;
+      }
+      const long long nfl = nf;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        double l, r;
+        rec_face<REC>(wd[q], l, r);
+        L[q * nfl + f] = l + r;
+      }
+    }
+    // repair of reconstructed states with the cell left of the interface
+    // (evolve.f90:334-369; the reference uses w0(ix) for both sides)
+    {
+      if (wr[KPR] <= 0) wr[KPR] = c0[KPR];
+      double v2 = wr[KVX] * wr[KVX] * M.g1 + wr[KVY] * wr[KVY] * M.g2 + wr[KVZ] * wr[KVZ] * M.g3;
+      if (v2 >= 1.) {
+        wr[KVX] = c0[KVX]; wr[KVY] = c0[KVY]; wr[KVZ] = c0[KVZ];
+        v2 = wr[KVX] * wr[KVX] * M.g1 + wr[KVY] * wr[KVY] * M.g2 + wr[KVZ] * wr[KVZ] * M.g3;
+        if (v2 >= 1) {
+          double s = sqrt(v2);
+#pragma unroll
+          for (int k = KVX; k <= KVZ; k++) { c0[k] = c0[k] * a.maxspeed / s; wr[k] = c0[k]; }
+        }
+      }
+      if (wl[KPR] <= 0) wl[KPR] = c0[KPR];
+      v2 = wl[KVX] * wl[KVX] * M.g1 + wl[KVY] * wl[KVY] * M.g2 + wl[KVZ] * wl[KVZ] * M.g3;
+      if (v2 >= 1.) {
+        wl[KVX] = c0[KVX]; wl[KVY] = c0[KVY]; wl[KVZ] = c0[KVZ];
+        v2 = wl[KVX] * wl[KVX] * M.g1 + wl[KVY] * wl[KVY] * M.g2 + wl[KVZ] * wl[KVZ] * M.g3;
+        if (v2 >= 1) {
+          double s = sqrt(v2);
+#pragma unroll
+          for (int k = KVX; k <= KVZ; k++) { c0[k] = c0[k] * a.maxspeed / s; wl[k] = c0[k]; }
+        }
+      }
+    }
+    double ul[NV], ur[NV], fl[NV], fr[NV], vfl[2], vfr[2];
+    int e = sys_flux<DIR, VISC, ZZ>(a.eos, M, wl, ul, fl, vfl);
+    if (e == 0) e = sys_flux<DIR, VISC, ZZ>(a.eos, M, wr, ur, fr, vfr);
+    if (e != 0) {
+      // the reference stops the run (run_crashed) on a flux error
+      atomicMax(&a.S->errcode, e);
+      atomicExch(&a.S->run_crashed, 1);
+#pragma unroll
+      for (int k = 0; k < NV; k++) F[(long long)k * nf + f] = 0.;
+    } else {
+      double ap = dmax(dmax(0., vfl[0]), vfr[0]);
+      double am = dmax(dmax(0., -vfl[1]), -vfr[1]);
+      amax = dmax(ap, am);
+      double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
+#pragma unroll
+      for (int k = 0; k < NV; k++)
+        F[(long long)k * nf + f] = a1 * (ap * fl[k] + am * fr[k] - ap * am * (ur[k] - ul[k]));
+    }
+  }
+  // block max of the interface speeds -> partial[blockIdx.x]
+  __shared__ double red[FLUX_BLOCK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) amax = dmax(amax, __shfl_xor(amax, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < FLUX_BLOCK / 64; w++) r = dmax(r, red[w]);
+    partial[blockIdx.x] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_cfl: reduces the per-block speed maxima of the three sweeps and applies
+// evolve_dt (evolve.f90:723-764) and the clamps of evolve.f90:179-188.
+// One block.  partial holds nb0 + nb1 + nb2 values (the block counts of the
+// three sweeps, in this order); only the sweeps of active dimensions
+// (n > 1) wrote theirs.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* __restrict__ partial,
+                                              int nb0, int nb1, int nb2) {
+  __shared__ double red[3][16];
+  double m[3] = {0., 0., 0.};
+  const int nb[3] = {nb0, nb1, nb2};
+  const bool act[3] = {a.nx > 1, a.ny > 1, a.nz > 1};
+  int off = 0;
+  for (int d = 0; d < 3; d++) {
+    if (act[d])
+      for (int i = threadIdx.x; i < nb[d]; i += blockDim.x) m[d] = dmax(m[d], partial[off + i]);
+    off += nb[d];
+  }
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m[d] = dmax(m[d], __shfl_xor(m[d], o, 64));
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = m[d];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) / 64;
+    double mm[3];
+    for (int d = 0; d < 3; d++) {
+      mm[d] = red[d][0];
+      for (int w = 1; w < nw; w++) mm[d] = dmax(mm[d], red[d][w]);
+    }
+    // max over cells of dd*max(speeds) == dd*max(speeds) for a uniform grid
+    double amax = 0.;
+    if (a.nx > 1) amax = dmax(amax, a.ddx * mm[0]);
+    if (a.ny > 1) amax = dmax(amax, a.ddy * mm[1]);
+    if (a.nz > 1) amax = dmax(amax, a.ddz * mm[2]);
+    if (amax == 0) {
+      if (a.nx > 1) amax = dmax(amax, a.ddx);
+      if (a.ny > 1) amax = dmax(amax, a.ddy);
+      if (a.nz > 1) amax = dmax(amax, a.ddz);
+    }
+    DevScalars* S = a.S;
+    double dt_local = a.cfl / amax;
+    dt_local = dmin(dt_local, a.maxdt);
+    dt_local = dmin(dt_local, S->tout - S->t);
+    if (fabs(S->t + S->dt - S->tout) < 1.e-9) dt_local = S->tout - S->t;
+    else if ((S->t + 2. * dt_local - S->tout) > 1.e-9) dt_local = (S->tout - S->t) / 2.;
+    S->dt = dt_local;
+  }
+}
+
+// lagged time derivatives of u^mu (dts, viscous.f90:56-97) and the face
+// differences of the reconstructed Lorentz velocities (evolve.f90:371-374)
+__device__ __forceinline__ Deriv cell_deriv(const GridArgs& a, const Metric& M, long long c, int x,
+                                            int y, int z, const double* __restrict__ LX,
+                                            const double* __restrict__ LY,
+                                            const double* __restrict__ LZ,
+                                            const double* __restrict__ VSn,
+                                            const double* __restrict__ VSo) {
+  Deriv D;
+  double dx[4] = {0., 0., 0., 0.}, dy[4] = {0., 0., 0., 0.}, dz[4] = {0., 0., 0., 0.};
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  if (nx > 1) {
+    long long nf = (long long)(nx + 1) * ny * nz, f0 = x + (long long)(nx + 1) * (y + (long long)ny * z);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dx[q] = a.ddx * 0.5 * (LX[q * nf + f0 + 1] - LX[q * nf + f0]);
+  }
+  if (ny > 1) {
+    long long nf = (long long)nx * (ny + 1) * nz, f0 = x + (long long)nx * (y + (long long)(ny + 1) * z);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dy[q] = a.ddy * 0.5 * (LY[q * nf + f0 + nx] - LY[q * nf + f0]);
+  }
+  if (nz > 1) {
+    long long nf = (long long)nx * ny * (nz + 1), f0 = x + (long long)nx * (y + (long long)ny * z);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dz[q] = a.ddz * 0.5 * (LZ[q * nf + f0 + (long long)nx * ny] - LZ[q * nf + f0]);
+  }
+  D.dutdx = dx[0]; D.duxdx = dx[1]; D.duydx = dx[2]; D.duzdx = dx[3];
+  D.dutdy = dy[0]; D.duxdy = dy[1]; D.duydy = dy[2]; D.duzdy = dy[3];
+  D.dutdz = dz[0]; D.duxdz = dz[1]; D.duydz = dz[2]; D.duzdz = dz[3];
+  const DevScalars* S = a.S;
+  const double ti = S->timeinterval;
+  if (ti == 0) {
+    D.dutdt = D.duxdt = D.duydt = D.duzdt = 0.;
+  } else {
+    const long long N = a.N;
+    double ox = VSo[c], oy = VSo[N + c], oz = VSo[2 * N + c];
+    double nxv = VSn[c], nyv = VSn[N + c], nzv = VSn[2 * N + c];
+    double gamma1 = 1. / sqrt(1. - M.g1 * (ox * ox) - M.g2 * (oy * oy) - S->g_cov_3_old * (oz * oz));
+    double gamma2 = 1. / sqrt(1. - M.g1 * (nxv * nxv) - M.g2 * (nyv * nyv) - M.g3 * (nzv * nzv));
+    D.dutdt = (gamma2 - gamma1) / ti;
+    D.duxdt = (gamma2 * nxv - gamma1 * ox) / ti;
+    D.duydt = (gamma2 * nyv - gamma1 * oy) / ti;
+    D.duzdt = (gamma2 * nzv - gamma1 * oz) / ti;
+  }
+  return D;
+}
+
+// ---------------------------------------------------------------------------
+// k_update: one thread per cell.  Flux divergence (evolve.f90:383 and the
+// y/z analogues) minus the sources (evolve_src, evolve.f90:689-718), the RK
+// combination u = c1*u0 + c2*(u - dt*du) (evolve.f90:199), con2prim
+// (evolve_u2v, evolve.f90:769-868) and, for viscous runs, the low-pressure
+// smoothing (evolve_smooth_viscosity, evolve.f90:1159-1207).  Cells whose
+// con2prim fails are appended to a list for k_fixup.
+// ---------------------------------------------------------------------------
+template <bool VISC, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
+    GridArgs a, double* __restrict__ V, const double* __restrict__ U0,
+    const double* __restrict__ Ucur, double* __restrict__ Uout, const double* __restrict__ FX,
+    const double* __restrict__ FY, const double* __restrict__ FZ, const double* __restrict__ LX,
+    const double* __restrict__ LY, const double* __restrict__ LZ, double* __restrict__ VS0,
+    double* __restrict__ VS1, int* __restrict__ faillist, double c1, double c2, int last_stage) {
+  constexpr int NV = VISC ? 12 : 5;
+  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N) return;
+  DevScalars* S = a.S;
+  if (S->run_crashed) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x = (int)(c % nx);
+  const long long r = c / nx;
+  const int y = (int)(r % ny), z = (int)(r / ny);
+  const Metric M = make_metric(a.coordinates, S->t);
+  const long long N = a.N;
+  double du[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) du[k] = 0.;
+  if (nx > 1) {
+    long long nf = (long long)(nx + 1) * ny * nz, f0 = x + (long long)(nx + 1) * (y + (long long)ny * z);
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddx * (FX[k * nf + f0 + 1] - FX[k * nf + f0]);
+  }
+  if (ny > 1) {
+    long long nf = (long long)nx * (ny + 1) * nz, f0 = x + (long long)nx * (y + (long long)(ny + 1) * z);
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddy * (FY[k * nf + f0 + nx] - FY[k * nf + f0]);
+  }
+  if (nz > 1) {
+    long long nf = (long long)nx * ny * (nz + 1), f0 = x + (long long)nx * (y + (long long)ny * z);
+    const long long st = (long long)nx * ny;
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddz * (FZ[k * nf + f0 + st] - FZ[k * nf + f0]);
+  }
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = V[k * N + c];
+  Deriv D{};
+  const int par = S->vs_parity;
+  double* VSn = par ? VS1 : VS0;
+  double* VSo = par ? VS0 : VS1;
+  if constexpr (VISC) D = cell_deriv(a, M, c, x, y, z, LX, LY, LZ, VSn, VSo);
+  {
+    double vv[NV], src[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) vv[k] = v[k];
+    int e = sys_source<VISC, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
+    if (e > 0) { atomicMax(&S->errcode, e); atomicExch(&S->run_crashed, 1); return; }
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] - src[k];
+  }
+  const double dt = S->dt;
+  double un[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) un[k] = c1 * U0[k * N + c] + c2 * (Ucur[k * N + c] - dt * du[k]);
+  int e = cons2prim<VISC, ZZ>(a.eos, M, un, v);
+  if (e != 0) {
+    int slot = atomicAdd(&S->nfail, 1);
+    if (slot < FAIL_CAP) faillist[slot] = (int)c;
+  } else if (VISC && a.enable_smooth) {
+    const double ps = S->smooth_pressure;
+    if (v[KPR] < ps) {
+      double q = v[KPR] / ps;
+#pragma unroll
+      for (int k = KPIBU; k <= KPIZZ; k++) v[k] = v[k] * (q * q);
+      if constexpr (ZZ) derived_pi<true>(v, M);
+      prim2cons<VISC, ZZ>(a.eos, M, v, un);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) { Uout[k * N + c] = un[k]; V[k * N + c] = v[k]; }
+  if (VISC && last_stage) {
+    // velocities at the start of the next step (vnewstep, echo.f90:192)
+    double* VSnext = VSo;
+    VSnext[c] = v[KVX]; VSnext[N + c] = v[KVY]; VSnext[2 * N + c] = v[KVZ];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_fixup: the con2prim repair of evolve_u2v (evolve.f90:795-842) for the
+// cells listed by k_update, in increasing cell order, by one thread.  The
+// neighbour averages read the six face neighbours, clamped to the grid
+// (the reference reads outside its arrays on boundary cells).
+// ---------------------------------------------------------------------------
+template <bool VISC, bool ZZ>
+__global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
+                        int* __restrict__ faillist, double* __restrict__ VS0,
+                        double* __restrict__ VS1, int last_stage) {
+  constexpr int NV = VISC ? 12 : 5;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DevScalars* S = a.S;
+  int n = S->nfail;
+  if (n == 0) return;
+  S->nfail = 0;
+  if (n > FAIL_CAP) { S->errcode = 1; S->run_crashed = 1; return; }
+  for (int i = 1; i < n; i++) {   // insertion sort: n is small
+    int key = faillist[i], j = i - 1;
+    while (j >= 0 && faillist[j] > key) { faillist[j + 1] = faillist[j]; j--; }
+    faillist[j + 1] = key;
+  }
+  const Metric M = make_metric(a.coordinates, S->t);
+  const long long N = a.N;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  double* VSnext = S->vs_parity ? VS0 : VS1;
+  for (int i = 0; i < n; i++) {
+    long long c = faillist[i];
+    int x = (int)(c % nx); long long r = c / nx; int y = (int)(r % ny), z = (int)(r / ny);
+    long long nb[6] = {
+        c - (x > 0 ? 1 : 0), c + (x < nx - 1 ? 1 : 0),
+        c - (y > 0 ? nx : 0), c + (y < ny - 1 ? nx : 0),
+        c - (z > 0 ? (long long)nx * ny : 0), c + (z < nz - 1 ? (long long)nx * ny : 0)};
+    double uu[NV], vv[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      const double* Uk = U + k * N;
+      uu[k] = (Uk[nb[0]] + Uk[nb[1]] + Uk[nb[2]] + Uk[nb[3]] + Uk[nb[4]] + Uk[nb[5]]) / 6.;
+      vv[k] = V[k * N + c];
+    }
+    cons2prim<VISC, ZZ>(a.eos, M, uu, vv);
+    double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
+    if (v2 > 1) {
+      double s = a.maxspeed / sqrt(v2);
+      vv[KVX] = s * vv[KVX]; vv[KVY] = s * vv[KVY]; vv[KVZ] = s * vv[KVZ];
+    }
+    if (vv[KPR] < 0) {
+      const double* P = V + KPR * N;
+      vv[KPR] = (P[nb[0]] + P[nb[1]] + P[nb[2]] + P[nb[3]] + P[nb[4]] + P[nb[5]]) / 6.;
+    }
+    prim2cons<VISC, ZZ>(a.eos, M, vv, uu);
+    cons2prim<VISC, ZZ>(a.eos, M, uu, vv);
+    v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
+    if (!((vv[KPR] > 0) && (v2 < 1))) {
+      S->errcode = 1; S->run_crashed = 1; return;
+    }
+    if (VISC && a.enable_smooth && vv[KPR] < S->smooth_pressure) {
+      double q = vv[KPR] / S->smooth_pressure;
+#pragma unroll
+      for (int k = KPIBU; k <= KPIZZ; k++) vv[k] = vv[k] * (q * q);
+      if constexpr (ZZ) derived_pi<true>(vv, M);
+      prim2cons<VISC, ZZ>(a.eos, M, vv, uu);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) { U[k * N + c] = uu[k]; V[k * N + c] = vv[k]; }
+    if (VISC && last_stage) {
+      VSnext[c] = vv[KVX]; VSnext[N + c] = vv[KVY]; VSnext[2 * N + c] = vv[KVZ];
+    }
+  }
+}
+
+// end of step bookkeeping, echo.f90:211-247 (time, metric, dts interval,
+// vnewstep -> vold, tout cadence)
+__global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (S->run_crashed) return;
+  S->timeold = S->t;
+  S->t = S->t + S->dt;
+  // g_cov_3_old = g_cov(3) of the step just taken (echo.f90:213)
+  S->g_cov_3_old = make_metric(coordinates, S->timeold).g3;
+  S->iter = S->iter + 1;
+  S->timeinterval = S->dt;
+  if (flip_vs) S->vs_parity = S->vs_parity ^ 1;
+  if ((S->t > S->tout) || ((S->tout - S->t) < 1.e-10)) {
+    S->tout = S->tout + S->dtout;
+    if ((fabs(S->tout - S->tmax) < 1.e-10) || (S->tout > S->tmax)) S->tout = S->tmax;
+  }
+}
+
+// initial state: per-cell prim2cons then cons2prim (init.f90:2937-2958)
+template <bool VISC, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_init(GridArgs a, double* __restrict__ V,
+                                                       double* __restrict__ U, double* __restrict__ VS0,
+                                                       int* __restrict__ status) {
+  constexpr int NV = VISC ? 12 : 5;
+  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N) return;
+  const long long N = a.N;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  double v[NV], u[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = V[k * N + c];
+  int e = prim2cons<VISC, ZZ>(a.eos, M, v, u);
+  if (e == 0 && !VISC) e = cons2prim<VISC, ZZ>(a.eos, M, u, v);
+  if (e) atomicMax(status, e);
+#pragma unroll
+  for (int k = 0; k < NV; k++) { V[k * N + c] = v[k]; U[k * N + c] = u[k]; }
+  if (VISC) { VS0[c] = v[KVX]; VS0[N + c] = v[KVY]; VS0[2 * N + c] = v[KVZ]; }
+}
+
+// batched con2prim over n independent states (system_cons2prim)
+template <bool VISC, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_c2p_batch(GridArgs a, const double* __restrict__ Ub,
+                                                            double* __restrict__ Vb,
+                                                            int* __restrict__ status, long long n) {
+  constexpr int NV = VISC ? 12 : 5;
+  const long long i = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  double u[NV], v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) { u[k] = Ub[k * n + i]; v[k] = Vb[k * n + i]; }
+  status[i] = cons2prim<VISC, ZZ>(a.eos, M, u, v);
+#pragma unroll
+  for (int k = 0; k < NV; k++) Vb[k * n + i] = v[k];
+}
+
+}  // namespace echo

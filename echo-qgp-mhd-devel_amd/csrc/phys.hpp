@@ -1,0 +1,516 @@
+// Device physics for the ECHO-QGP grid update on gfx950: equation of state,
+// reconstruction, fluxes, con2prim and sources, one cell or one interface per
+// thread, FP64 throughout.  Operation order follows the reference so that the
+// FP64 results agree with it to rounding (the library is built with FP
+// contraction off).  Reference file:line citations on each function.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace echo {
+
+// primitive / conserved slots, common.f90:63 (0-based)
+enum { KRH = 0, KVX = 1, KVY = 2, KVZ = 3, KPR = 4, KPIBU = 5, KPIXY = 6,
+       KPIXZ = 7, KPIYZ = 8, KPIXX = 9, KPIYY = 10, KPIZZ = 11 };
+
+enum { MINKOWSKI = 1, BJORKEN = 2 };
+enum { SHEAR_VISCOUS_1D = 2, GUBSER_VISCOUS = 5 };
+
+// Reconstruction codes (same numbering as the C-ABI echo_config.recal).
+enum { REC_TVD2 = 1, REC_CENO3 = 2, REC_WENO3 = 3, REC_WENO5 = 4, REC_PPM4 = 5,
+       REC_MPE3 = 6, REC_MPE5 = 7 };
+
+struct Eos {
+  int eqstate;          // 1 p=e/3, 2 ideal gas, 3 table
+  int n;                // table entries
+  double przero, enezero;
+  const double* __restrict__ tt;   // temperature
+  const double* __restrict__ te;   // energy density
+  const double* __restrict__ tp;   // pressure
+  const double* __restrict__ tc;   // c_s^2
+};
+
+struct Metric {
+  double g1, g2, g3, gp, gm, g0, t;
+};
+
+__host__ __device__ __forceinline__ Metric make_metric(int coordinates, double t) {
+  // system_metric_minkowski / _bjorken, system.f90:132-160
+  Metric m;
+  m.g1 = 1.; m.g2 = 1.; m.g0 = -1.; m.t = t;
+  if (coordinates == BJORKEN) { m.g3 = t * t; m.gp = t; m.gm = 1. / t; }
+  else { m.g3 = 1.; m.gp = 1.; m.gm = 1.; }
+  return m;
+}
+
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Equation of state (eos.f90, numerically = 0)
+// ---------------------------------------------------------------------------
+constexpr double GAMMA_AD = 4.0 / 3.0;   // eos.f90:60
+
+// strict-">" bisection of eos.f90:430-442 (1-based result indxl)
+__host__ __device__ __forceinline__ int tab_bracket(const double* __restrict__ a, int n, double x) {
+  int lo = 1, hi = n, mid = (lo + hi) >> 1;
+  while (lo + 1 != hi) {
+    if (x > a[mid - 1]) lo = mid; else hi = mid;
+    mid = (lo + hi) >> 1;
+  }
+  return lo;
+}
+
+// eos_pressure_ana (eos.f90:888-925) / get_pressure (eos.f90:399-464)
+__host__ __device__ __forceinline__ int eos_pressure(const Eos& E, double rh, double en, double& pr,
+                                            double& dprdrh, double& dprden) {
+  if (E.eqstate == 1) { pr = en / 3.0; dprden = 1. / 3.; dprdrh = 0.; return 0; }
+  if (E.eqstate == 2) {
+    pr = (GAMMA_AD - 1.) * (en - rh); dprden = GAMMA_AD - 1.; dprdrh = -GAMMA_AD + 1.;
+    return 0;
+  }
+  if (en < E.te[0]) return 123;
+  if (en > E.te[E.n - 1]) return 124;
+  int l = tab_bracket(E.te, E.n, en);
+  double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  double t2 = E.tt[l], t1 = E.tt[l - 1];
+  pr = p1 + (p2 - p1) * (en - e1) / (e2 - e1);
+  double dpdt = (p2 - p1) / (t2 - t1);
+  double dedt = (e2 - e1) / (t2 - t1);
+  dprdrh = 0;
+  dprden = dpdt / dedt;
+  return 0;
+}
+
+// eos_energy_ana (eos.f90:1108-1200) / get_energy (eos.f90:468-523)
+__host__ __device__ __forceinline__ int eos_energy(const Eos& E, double rh, double pr, double& en) {
+  if (E.eqstate != 3) {
+    if (pr <= E.przero) { en = E.enezero * pr / E.przero; return 0; }
+    en = (E.eqstate == 1) ? 3. * pr : rh + pr / (GAMMA_AD - 1.);
+    return 0;
+  }
+  if (pr < E.tp[0]) return 125;
+  if (pr > E.tp[E.n - 1]) return 126;
+  int l = tab_bracket(E.tp, E.n, pr);
+  double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  en = e1 + (e2 - e1) * (pr - p1) / (p2 - p1);
+  return 0;
+}
+
+// eos_sound_ana (eos.f90:929-953) / get_sound (eos.f90:590-641)
+__host__ __device__ __forceinline__ int eos_sound(const Eos& E, double rh, double en, double pr, double& cs2) {
+  if (E.eqstate != 3) {
+    double dprden, dprdrh;
+    if (E.eqstate == 1) { dprden = 1. / 3.; dprdrh = 0.; }
+    else { dprden = GAMMA_AD - 1.; dprdrh = -GAMMA_AD + 1.; }
+    double c = (en + pr == 0) ? dprden : dprden + (rh / (en + pr)) * dprdrh;
+    cs2 = c > 0.0 ? c : 0.0;
+    return 0;
+  }
+  if (en < E.te[0]) return 131;
+  if (en > E.te[E.n - 1]) return 132;
+  int l = tab_bracket(E.te, E.n, en);
+  double s2 = E.tc[l], s1 = E.tc[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  cs2 = s1 + (s2 - s1) * (en - e1) / (e2 - e1);
+  return 0;
+}
+
+// get_derived_data_ana (eos.f90:957-969) / get_derived_data_tab (645-701)
+__host__ __device__ __forceinline__ int get_derived_data(const Eos& E, double rh, double pr, double& en,
+                                                double& temp, double& s) {
+  if (E.eqstate != 3) {
+    int err = eos_energy(E, rh, pr, en);
+    temp = 0.148909 * pow(en, 0.25);
+    s = (temp == 0) ? 0. : (en + pr) / temp;
+    return err;
+  }
+  if (pr < E.tp[0]) return 135;
+  if (pr > E.tp[E.n - 1]) return 136;
+  int l = tab_bracket(E.tp, E.n, pr);
+  double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  double t2 = E.tt[l], t1 = E.tt[l - 1];
+  double f = (pr - p1) / (p2 - p1);
+  temp = t1 + (t2 - t1) * f;
+  en = e1 + (e2 - e1) * f;
+  s = (en + pr) / temp;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (holib.f90).  f[0..5] = cells i-2 .. i+3 around the
+// interface i+1/2; returns the left (gl) and right (gr) states there.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sgn1(double d) { return copysign(1.0, d); }
+
+// mm2, holib.f90:440-452
+__device__ __forceinline__ double mm2(double d1, double d2) {
+  return .5 * (sgn1(d1) + sgn1(d2)) * dmin(fabs(d1), fabs(d2));
+}
+// mm4, holib.f90:472-487
+__device__ __forceinline__ double mm4(double d1, double d2, double d3, double d4) {
+  double s1 = sgn1(d1), s2 = sgn1(d2), s3 = sgn1(d3), s4 = sgn1(d4);
+  double m = dmin(dmin(dmin(fabs(d1), fabs(d2)), fabs(d3)), fabs(d4));
+  return .125 * (s1 + s2) * fabs((s1 + s3) * (s1 + s4)) * m;
+}
+// ceno3, holib.f90:491-511
+__device__ __forceinline__ double ceno3(double d0, double d1, double d2) {
+  int s = (int)sgn1(d0) + (int)sgn1(d1) + (int)sgn1(d2);
+  if (s == 3 || s == -3) {
+    double w0 = fabs(d0), w1 = 0.7 * fabs(d1), w2 = fabs(d2);
+    double r = d0, wm = w0;
+    if (w1 < wm) { wm = w1; r = d1; }
+    if (w2 < wm) { r = d2; }
+    return r;
+  }
+  return 0.;
+}
+// mp5, holib.f90:515-536
+__device__ __forceinline__ double mp5(double f, double f1, double f2, double f3, double f4, double f5) {
+  double r = f;
+  double ful = f3 + 2 * (f3 - f2);
+  double fmp = f3 + mm2(f4 - f3, ful - f3);
+  if ((f - f3) * (f - fmp) > 0.) {
+    double dm = f1 - 2 * f2 + f3;
+    double d0 = f2 - 2 * f3 + f4;
+    double dp = f3 - 2 * f4 + f5;
+    double fmd = .5 * (f3 + f4) - .5 * mm4(4 * d0 - dp, 4 * dp - d0, d0, dp);
+    double flc = f3 + .5 * (f3 - f2) + (4. / 3.) * mm4(4 * d0 - dm, 4 * dm - d0, d0, dm);
+    double fmin = dmax(dmin(dmin(f3, f4), fmd), dmin(dmin(f3, ful), flc));
+    double fmax = dmin(dmax(dmax(f3, f4), fmd), dmax(dmax(f3, ful), flc));
+    r = f + mm2(fmin - f, fmax - f);
+  }
+  return r;
+}
+
+// one WENO5 side, holib.f90:206-217
+__device__ __forceinline__ double weno5_side(double f1, double f2, double f3, double f4, double f5) {
+  const double d1 = 1 / 16., d2 = 10 / 16., d3 = 5 / 16., eps = 1.e-6;
+  double t, s;
+  t = f3 - 2 * f2 + f1; s = 3. * f3 - 4. * f2 + f1;
+  double beta1 = 13. * (t * t) + 3. * (s * s);
+  t = f4 - 2 * f3 + f2; s = f4 - f2;
+  double beta2 = 13. * (t * t) + 3. * (s * s);
+  t = f5 - 2 * f4 + f3; s = f5 - 4. * f4 + 3. * f3;
+  double beta3 = 13. * (t * t) + 3. * (s * s);
+  double q1 = beta1 / 12. + eps, q2 = beta2 / 12. + eps, q3 = beta3 / 12. + eps;
+  double a1 = d1 / (q1 * q1), a2 = d2 / (q2 * q2), a3 = d3 / (q3 * q3);
+  double p1 = 3 / 8. * f1 + -10 / 8. * f2 + 15 / 8. * f3;
+  double p2 = -1 / 8. * f2 + 6 / 8. * f3 + 3 / 8. * f4;
+  double p3 = 3 / 8. * f3 + 6 / 8. * f4 + -1 / 8. * f5;
+  return (a1 * p1 + a2 * p2 + a3 * p3) / (a1 + a2 + a3);
+}
+
+__device__ __forceinline__ double weno3_side(double f1, double f2, double f3) {
+  // holib.f90:148-158
+  const double eps = 1.e-6;
+  double b1 = (f2 - f1) * (f2 - f1), b2 = (f3 - f2) * (f3 - f2);
+  double a1 = (1 / 3.) / ((b1 + eps) * (b1 + eps)), a2 = (2 / 3.) / ((b2 + eps) * (b2 + eps));
+  double p1 = -1 / 2. * f1 + 3 / 2. * f2, p2 = 1 / 2. * f2 + 1 / 2. * f3;
+  return (a1 * p1 + a2 * p2) / (a1 + a2);
+}
+
+__device__ __forceinline__ double ceno3_side(double f1, double f2, double f3, double f4, double f5) {
+  // holib.f90:98-108
+  double p = f3 + .5 * mm2(f3 - f2, f4 - f3);
+  double d0 = 3 / 8. * f1 + -10 / 8. * f2 + 15 / 8. * f3 - p;
+  double d1 = -1 / 8. * f2 + 6 / 8. * f3 + 3 / 8. * f4 - p;
+  double d2 = 3 / 8. * f3 + 6 / 8. * f4 + -1 / 8. * f5 - p;
+  return p + ceno3(d0, d1, d2);
+}
+
+template <int REC>
+__device__ __forceinline__ void rec_face(const double* f, double& gl, double& gr) {
+  // f[j] = cell (i-2+j)
+  if constexpr (REC == REC_WENO5) {           // holib.f90:177-232
+    gl = weno5_side(f[0], f[1], f[2], f[3], f[4]);
+    gr = weno5_side(f[5], f[4], f[3], f[2], f[1]);
+  } else if constexpr (REC == REC_PPM4) {     // holib.f90:236-268
+    gl = f[2] + .5 * (f[3] - f[2]) - .125 * (mm2(f[4] - f[3], f[3] - f[2]) - mm2(f[3] - f[2], f[2] - f[1]));
+    gr = f[3] + .5 * (f[2] - f[3]) - .125 * (mm2(f[1] - f[2], f[2] - f[3]) - mm2(f[2] - f[3], f[3] - f[4]));
+  } else if constexpr (REC == REC_TVD2) {     // holib.f90:47-77
+    gl = f[2] + .5 * mm2(f[2] - f[1], f[3] - f[2]);
+    gr = f[3] - .5 * mm2(f[3] - f[2], f[4] - f[3]);
+  } else if constexpr (REC == REC_MPE5) {     // holib.f90:297-318
+    const double d1 = 3 / 128., d2 = -20 / 128., d3 = 90 / 128., d4 = 60 / 128., d5 = -5 / 128.;
+    double l = d1 * f[0] + d2 * f[1] + d3 * f[2] + d4 * f[3] + d5 * f[4];
+    double r = d1 * f[5] + d2 * f[4] + d3 * f[3] + d4 * f[2] + d5 * f[1];
+    gl = mp5(l, f[0], f[1], f[2], f[3], f[4]);
+    gr = mp5(r, f[5], f[4], f[3], f[2], f[1]);
+  } else if constexpr (REC == REC_WENO3) {    // holib.f90:130-173
+    gl = weno3_side(f[1], f[2], f[3]);
+    gr = weno3_side(f[4], f[3], f[2]);
+  } else if constexpr (REC == REC_CENO3) {    // holib.f90:81-126
+    gl = ceno3_side(f[0], f[1], f[2], f[3], f[4]);
+    gr = ceno3_side(f[5], f[4], f[3], f[2], f[1]);
+  } else {                                    // REC_MPE3, holib.f90:272-293
+    const double d1 = -1 / 8., d2 = 6 / 8., d3 = 3 / 8.;
+    double l = d1 * f[1] + d2 * f[2] + d3 * f[3];
+    double r = d1 * f[4] + d2 * f[3] + d3 * f[2];
+    gl = mp5(l, f[0], f[1], f[2], f[3], f[4]);
+    gr = mp5(r, f[5], f[4], f[3], f[2], f[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous helpers (viscous.f90)
+// ---------------------------------------------------------------------------
+struct PiDer { double pitt, pitx, pity, pitz; };
+
+// get_derived_pi (227-239) and get_derived_pi_zz (243-279); for 'zz' the
+// traceless pi^zz is written back into p[KPIZZ] as the reference does.
+template <bool ZZ>
+__device__ __forceinline__ PiDer derived_pi(double* p, const Metric& M) {
+  const double g1 = M.g1, g2 = M.g2, g3 = M.g3;
+  const double vx = p[KVX], vy = p[KVY], vz = p[KVZ];
+  const double xx = p[KPIXX], yy = p[KPIYY], xy = p[KPIXY], xz = p[KPIXZ], yz = p[KPIYZ];
+  PiDer r;
+  if constexpr (!ZZ) {
+    r.pitx = xx * vx * g1 + xy * vy * g2 + xz * vz * g3;
+    r.pity = xy * vx * g1 + yy * vy * g2 + yz * vz * g3;
+    r.pitz = xz * vx * g1 + yz * vy * g2 + p[KPIZZ] * vz * g3;
+    r.pitt = r.pitx * vx * g1 + r.pity * vy * g2 + r.pitz * vz * g3;
+  } else {
+    r.pitt = 1. / (1. - vz * vz * g3) * (vx * g1 * xy * vy * g2 + vx * vx * g1 * g1 * xx +
+             vy * g1 * yz * vz * g3 + 2. * vx * g1 * xz * vz * g3 + vy * g1 * g1 * xy *
+             vx + vy * vy * g1 * yy * g2 + vz * g3 * yz * vy * g2 - xx * g1 * vz * vz *
+             g3 - yy * g2 * vz * vz * g3);
+    r.pitx = xx * vx * g1 + xy * vy * g2 + xz * vz * g3;
+    r.pity = xy * vx * g1 + yy * vy * g2 + yz * vz * g3;
+    r.pitz = 1. / (1. - vz * vz * g3) * (xz * vx * g1 + xz * vx * g1 * vz * vz * g3 + yz * vy *
+             g2 + vz * vx * g1 * xy * vy * g2 - vz * xx * g1 - vz * yy * g2 + vz * vx *
+             vx * g1 * g1 * xx + vy * g1 * yz * vz * vz * g3 + vz * vy * g1 * g1 *
+             xy * vx + vz * vy * vy * g1 * yy * g2);
+    p[KPIZZ] = 1. / (g3 * (1. - vz * vz * g3)) * (vx * g1 * xy * vy * g2 - xx * g1 - yy * g2 +
+             vx * vx * g1 * g1 * xx + vy * g1 * yz * vz * g3 + 2. * vx * g1 * xz * vz *
+             g3 + vy * g1 * g1 * xy * vx + vy * vy * g1 * yy * g2 + vz * g3 * yz *
+             vy * g2);
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// prim2cons, system.f90:164-263
+// ---------------------------------------------------------------------------
+template <bool VISC, bool ZZ>
+__device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* v, double* u) {
+  constexpr int NV = VISC ? 12 : 5;
+  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
+  int err = eos_energy(E, rh, pr, en);
+  if (err > 0) return err;
+  double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
+  double glf = 1. / sqrt(1. - v2);
+  double w = en + pr;
+  double ww = w * (glf * glf);
+  u[KRH] = rh * glf;
+  if constexpr (VISC) {
+    w = w + v[KPIBU];
+    ww = w * (glf * glf);
+    PiDer d = derived_pi<ZZ>(v, M);
+    u[KVX] = (ww * vx + d.pitx);
+    u[KVY] = (ww * vy + d.pity);
+    u[KVZ] = (ww * vz + d.pitz) * M.g3;
+    u[KPR] = ww - pr - v[KPIBU] - d.pitt * M.g0;
+    u[KPIBU] = v[KPIBU] * u[KRH];
+    u[KPIZZ] = v[KPIZZ] * u[KRH];
+    u[KPIXZ] = v[KPIXZ] * u[KRH];
+    u[KPIXY] = v[KPIXY] * u[KRH];
+    u[KPIYZ] = v[KPIYZ] * u[KRH];
+    u[KPIXX] = v[KPIXX] * u[KRH];
+    u[KPIYY] = v[KPIYY] * u[KRH];
+  } else {
+    u[KVX] = ww * vx;
+    u[KVY] = ww * vy;
+    u[KVZ] = ww * vz * M.g3;
+    u[KPR] = ww - pr;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) u[k] = u[k] * M.gp;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// system_flux, system.f90:1316-1583.  DIR = 1,2,3.  Returns u, f (nv each)
+// and the two fast speeds vf[0] = lambda+, vf[1] = lambda-.
+// ---------------------------------------------------------------------------
+template <int DIR, bool VISC, bool ZZ>
+__device__ __forceinline__ int sys_flux(const Eos& E, const Metric& M, double* v, double* u,
+                                        double* f, double* vf) {
+  constexpr int NV = VISC ? 12 : 5;
+  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
+  int err = eos_energy(E, rh, pr, en);
+  if (err > 0) return err;
+  double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
+  if (v2 > 1) return 333;
+  double glf = 1. / sqrt(1. - v2);
+  double w = VISC ? en + pr + v[KPIBU] : en + pr;
+  double ww = w * (glf * glf);
+  PiDer d{0., 0., 0., 0.};
+  u[KRH] = rh * glf;
+  if constexpr (VISC) {
+    d = derived_pi<ZZ>(v, M);
+#pragma unroll
+    for (int k = KPIBU; k < NV; k++) u[k] = v[k] * u[KRH];
+    u[KVX] = ww * vx + d.pitx;
+    u[KVY] = ww * vy + d.pity;
+    u[KVZ] = (ww * vz + d.pitz) * M.g3;
+    u[KPR] = ww - pr - v[KPIBU] - d.pitt * M.g0;
+  } else {
+    u[KVX] = ww * vx;
+    u[KVY] = ww * vy;
+    u[KVZ] = ww * vz * M.g3;
+    u[KPR] = ww - pr;
+  }
+  double wwdir, vdir, gdir;
+  if constexpr (DIR == 1) {
+    wwdir = ww * vx; vdir = vx; gdir = M.g1;
+    f[KRH] = rh * glf * vx;
+    if constexpr (VISC) {
+      f[KVX] = wwdir * vx * M.g1 + pr + v[KPIBU] + v[KPIXX] * M.g1;
+      f[KVY] = wwdir * vy * M.g2 + v[KPIXY] * M.g2;
+      f[KVZ] = wwdir * vz * M.g3 + v[KPIXZ] * M.g3;
+      f[KPR] = wwdir + d.pitx;
+    } else {
+      f[KVX] = wwdir * vx * M.g1 + pr;
+      f[KVY] = wwdir * vy * M.g2;
+      f[KVZ] = wwdir * vz * M.g3;
+      f[KPR] = wwdir;
+    }
+  } else if constexpr (DIR == 2) {
+    wwdir = ww * vy; vdir = vy; gdir = M.g2;
+    f[KRH] = rh * glf * vy;
+    if constexpr (VISC) {
+      f[KVX] = wwdir * vx * M.g1 + v[KPIXY] * M.g1;
+      f[KVY] = wwdir * vy * M.g2 + pr + v[KPIBU] + v[KPIYY] * M.g2;
+      f[KVZ] = wwdir * vz * M.g3 + v[KPIYZ] * M.g3;
+      f[KPR] = wwdir + d.pity;
+    } else {
+      f[KVX] = wwdir * vx * M.g1;
+      f[KVY] = wwdir * vy * M.g2 + pr;
+      f[KVZ] = wwdir * vz * M.g3;
+      f[KPR] = wwdir;
+    }
+  } else {
+    wwdir = ww * vz; vdir = vz; gdir = M.g3;
+    f[KRH] = rh * glf * vz;
+    if constexpr (VISC) {
+      f[KVX] = wwdir * vx * M.g1 + v[KPIXZ] * M.g1;
+      f[KVY] = wwdir * vy * M.g2 + v[KPIYZ] * M.g2;
+      f[KVZ] = wwdir * vz * M.g3 + pr + v[KPIBU] + v[KPIZZ] * M.g3;
+      f[KPR] = wwdir + d.pitz;
+    } else {
+      f[KVX] = wwdir * vx * M.g1;
+      f[KVY] = wwdir * vy * M.g2;
+      f[KVZ] = wwdir * vz * M.g3 + pr;
+      f[KPR] = wwdir;
+    }
+  }
+  if constexpr (VISC) {
+#pragma unroll
+    for (int k = 5; k < NV; k++) f[k] = u[k] * vdir;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) { u[k] = u[k] * M.gp; f[k] = f[k] * M.gp; }
+  double cs2;
+  err = eos_sound(E, rh, en, pr, cs2);
+  if (err > 0) return err;
+  const double ca2 = 0.;
+  double a2 = cs2 + ca2 - cs2 * ca2;
+  double den = 1. / (1. - v2 * a2);
+  double vf1 = den * (1. - a2) * vdir;
+  double vf2 = den * sqrt(a2 * (1. - v2) * ((1. - v2 * a2) / gdir - (1. - a2) * (vdir * vdir)));
+  vf[0] = vf1 + vf2;
+  vf[1] = vf1 - vf2;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// cons2prim, system.f90:511-665 (ideal Newton on p; viscous nested
+// iteration).  v holds the guess on entry (v[KPR], velocities for viscous).
+// ---------------------------------------------------------------------------
+template <bool VISC, bool ZZ>
+__device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const double* u, double* v) {
+  constexpr int NV = VISC ? 12 : 5;
+  const double tol = 1.e-12, eps = 1.e-12;
+  const double gm = M.gm;
+  int errcode = 0;
+  double d = gm * u[KRH];
+  double sx = gm * u[KVX];
+  double sy = gm * u[KVY];
+  double sz = gm * u[KVZ] / M.g3;
+  double et = gm * u[KPR];
+  double s2 = M.g1 * sx * sx + M.g2 * sy * sy + M.g3 * sz * sz;
+  double pr, w = 0., v2, glf, rh = 0., en, drhdpr, dprdrh, dprden, f, df, dpr, pr_eos = 0.;
+  if constexpr (!VISC) {
+    pr = v[KPR];
+    for (int iter = 1; iter <= 100; iter++) {
+      w = et + pr;
+      v2 = s2 / (w * w);
+      if (v2 >= 1.) return 777;
+      glf = 1. / sqrt(1. - v2);
+      rh = d / glf;
+      en = w * (1. - v2) - pr;
+      drhdpr = rh * v2 * glf / (en + pr);
+      errcode = eos_pressure(E, rh, en, pr_eos, dprdrh, dprden);
+      f = pr_eos - pr;
+      df = dprdrh * drhdpr + dprden * v2 - 1.;
+      dpr = -f / df;
+      if (fabs(dpr) < tol * pr) break;
+      pr = pr + dpr;
+    }
+    v[KRH] = rh;
+    v[KVX] = sx / w;
+    v[KVY] = sy / w;
+    v[KVZ] = sz / w;
+    v[KPR] = pr;
+    if (pr <= 0) errcode = 777;
+    return errcode;
+  } else {
+    double uin[NV], vprim[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) uin[k] = gm * u[k];
+    d = uin[KRH];
+#pragma unroll
+    for (int k = KPIBU; k <= KPIZZ; k++) v[k] = uin[k] / uin[KRH];
+    double vx = v[KVX], vy = v[KVY], vz = v[KVZ];
+#pragma unroll
+    for (int k = 0; k < NV; k++) vprim[k] = v[k];
+    for (int iv = 1; iv <= 100; iv++) {
+      vprim[KVX] = vx; vprim[KVY] = vy; vprim[KVZ] = vz;
+      PiDer q = derived_pi<ZZ>(vprim, M);
+      double en1 = et + v[KPIBU] + q.pitt * M.g0;
+      double sx_cov = uin[KVX] - q.pitx;
+      double sy_cov = uin[KVY] - q.pity;
+      double sz_cov = uin[KVZ] - q.pitz * M.g3;
+      sx = sx_cov;
+      sy = sy_cov;
+      sz = sz_cov / M.g3;
+      s2 = sx * sx_cov + sy * sy_cov + sz * sz_cov;
+      pr = v[KPR];
+      for (int ip = 1; ip <= 100; ip++) {
+        double qq = en1 + pr;
+        v2 = s2 / (qq * qq);
+        if (v2 >= 1.) return 107;
+        glf = 1. / sqrt(1. - v2);
+        rh = d / glf;
+        en = (en1 + pr) * (1. - v2) - pr - vprim[KPIBU];
+        drhdpr = rh * v2 * glf / (en + pr);
+        errcode = eos_pressure(E, rh, en, pr_eos, dprdrh, dprden);
+        f = pr_eos - pr;
+        df = dprdrh * drhdpr + dprden * v2 - 1.;
+        dpr = -f / df;
+        if (fabs(dpr) < tol * pr) break;
+        pr = pr + dpr;
+      }
+      vx = sx / (en1 + pr); if (fabs(vx) < eps) vx = 0.;
+      vy = sy / (en1 + pr); if (fabs(vy) < eps) vy = 0.;
+      vz = sz / (en1 + pr); if (fabs(vz) < eps) vz = 0.;
+      vprim[KVX] = vx; vprim[KVY] = vy; vprim[KVZ] = vz;
+      if (fabs(M.g1 * (vx * vx) + M.g2 * (vy * vy) + M.g3 * (vz * vz) - v2) < eps) break;
+    }
+    v[KRH] = rh; v[KVX] = vx; v[KVY] = vy; v[KVZ] = vz; v[KPR] = pr;
+    if constexpr (ZZ) derived_pi<true>(v, M);
+    return errcode;
+  }
+}
+
+}  // namespace echo

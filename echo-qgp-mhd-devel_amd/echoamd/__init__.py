@@ -1,0 +1,193 @@
+"""Python binding of the C-ABI in include/echo_amd.h (libechoamd.so).
+
+The library is the MI355X product path; there is no fallback.  Loading raises
+``RuntimeError`` when libechoamd.so has not been built, and every call raises
+``EchoError`` when the C-ABI returns an error code.
+
+Arrays use the reference's Fortran layout: shape (nv, nz, ny, nx) in numpy
+C order, i.e. a(ix, iy, iz, k) of the Fortran arrays v/u (common.f90:203-214).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libechoamd.so")
+EOS_DIR = os.path.join(PKG_DIR, "eos_data")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+# the exported symbols of include/echo_amd.h
+EXPORTS = (
+    "echo_create", "echo_destroy", "echo_nv", "echo_init_from_prim",
+    "echo_upload_state", "echo_download_state", "echo_steps", "echo_rk_stage",
+    "echo_end_step", "echo_get_scalars", "echo_set_tout", "echo_cons2prim_batch",
+    "echo_set_timing", "echo_get_timing", "echo_last_error",
+)
+
+REC_CODES = {"TVD2": 1, "CENO3": 2, "WENO3": 3, "WENO5": 4, "PPM4": 5, "MPE3": 6, "MPE5": 7}
+
+
+class EchoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"echo error {code}: {msg}")
+        self.code = code
+
+
+class EchoConfig(C.Structure):
+    """Mirror of struct echo_config (16 int32 followed by 19 double)."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "nx", "ny", "nz", "coordinates", "viscosity", "bulk", "mhd", "divclean",
+        "eqstate", "recal", "nrk", "boundary_cond", "algo_solver", "obtained_zz",
+        "init_type", "derivatives_out")] + [(n, C.c_double) for n in (
+        "x1", "x2", "y1", "y2", "z1", "z2", "tmin", "cfl", "maxdt", "maxspeed",
+        "eta_over_s", "tau_pi_coeff", "enezero", "glm_alpha", "sigma_el", "sigma_chi",
+        "dtout", "smooth_temp", "tmax")]
+
+    @classmethod
+    def from_arrays(cls, ic, rc):
+        """Build from the (16 int32, 19 double) arrays used by the oracle."""
+        ic = np.ascontiguousarray(ic, dtype=np.int32)
+        rc = np.ascontiguousarray(rc, dtype=np.float64)
+        assert ic.size == 16 and rc.size == 19
+        return cls.from_buffer_copy(ic.tobytes() + rc.tobytes())
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libechoamd.so and declare the C-ABI signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run echo-qgp-mhd-devel_amd/build.sh (or "
+            "__graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "echo_create": (C.c_int, [C.POINTER(EchoConfig), _dp, C.c_int32, C.c_int32, C.POINTER(P)]),
+        "echo_destroy": (None, [P]),
+        "echo_nv": (C.c_int, [P]),
+        "echo_init_from_prim": (C.c_int, [P, _dp]),
+        "echo_upload_state": (C.c_int, [P, _dp, _dp]),
+        "echo_download_state": (C.c_int, [P, _dp, _dp]),
+        "echo_steps": (C.c_int, [P, C.c_int32]),
+        "echo_rk_stage": (C.c_int, [P, C.c_int32]),
+        "echo_end_step": (C.c_int, [P]),
+        "echo_get_scalars": (C.c_int, [P, _dp]),
+        "echo_set_tout": (C.c_int, [P, C.c_double]),
+        "echo_cons2prim_batch": (C.c_int, [P, _dp, _dp, _ip, C.c_int64]),
+        "echo_set_timing": (C.c_int, [P, C.c_int32]),
+        "echo_get_timing": (C.c_int, [P, _dp]),
+        "echo_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def read_eos_rows(name="qcdIEOS0.dat"):
+    """Rows (T, e/T^4, p/T^4, c_s^2) of a reference EoS table file."""
+    path = os.path.join(EOS_DIR, name)
+    with open(path) as fh:
+        n = int(fh.readline().split()[0])
+        rows = np.loadtxt(fh, max_rows=n)
+    return np.ascontiguousarray(rows, dtype=np.float64)
+
+
+class EchoGrid:
+    """One event's grid on one GPU (echo_grid of the C-ABI)."""
+
+    def __init__(self, ic, rc, device=0, eos_rows=None):
+        self.lib = load_library()
+        self.cfg = EchoConfig.from_arrays(ic, rc)
+        self.nx, self.ny, self.nz = self.cfg.nx, self.cfg.ny, self.cfg.nz
+        if self.cfg.eqstate == 3 and eos_rows is None:
+            eos_rows = read_eos_rows()
+        rows = np.zeros((1, 4)) if eos_rows is None else np.ascontiguousarray(eos_rows, dtype=np.float64)
+        n_rows = 0 if eos_rows is None else rows.shape[0]
+        h = C.c_void_p()
+        self._check(self.lib.echo_create(C.byref(self.cfg), rows.ravel(), n_rows, device, C.byref(h)))
+        self.h = h
+        self.nv = self.lib.echo_nv(self.h)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise EchoError(rc, self.lib.echo_last_error().decode())
+        return rc
+
+    @property
+    def shape(self):
+        return (self.nv, self.nz, self.ny, self.nx)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.echo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_from_prim(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == self.shape, (v.shape, self.shape)
+        return self._check(self.lib.echo_init_from_prim(self.h, v.ravel()))
+
+    def upload(self, v, u):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert v.shape == self.shape and u.shape == self.shape
+        self._check(self.lib.echo_upload_state(self.h, v.ravel(), u.ravel()))
+
+    def state(self):
+        v = np.zeros(self.shape)
+        u = np.zeros(self.shape)
+        self._check(self.lib.echo_download_state(self.h, v.ravel(), u.ravel()))
+        return v, u
+
+    def steps(self, n):
+        return self._check(self.lib.echo_steps(self.h, int(n)))
+
+    def rk_stage(self, irk):
+        return self._check(self.lib.echo_rk_stage(self.h, int(irk)))
+
+    def end_step(self):
+        return self._check(self.lib.echo_end_step(self.h))
+
+    def scalars(self):
+        s = np.zeros(10)
+        self._check(self.lib.echo_get_scalars(self.h, s))
+        return dict(t=s[0], dt=s[1], timeold=s[2], timeinterval=s[3], g_cov_3_old=s[4],
+                    glm_ch_old=s[5], tout=s[6], iter=int(s[7]), crashed=int(s[8]),
+                    errcode=int(s[9]))
+
+    def set_tout(self, tout):
+        self._check(self.lib.echo_set_tout(self.h, float(tout)))
+
+    def cons2prim_batch(self, u, vguess):
+        """u, vguess: (nv, n) arrays.  Returns (v, status)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.array(vguess, dtype=np.float64, order="C")
+        n = u.shape[1]
+        st = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.echo_cons2prim_batch(self.h, u.ravel(), v.ravel(), st, n))
+        return v, st
+
+    def set_timing(self, on):
+        self._check(self.lib.echo_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        t = np.zeros(4)
+        self._check(self.lib.echo_get_timing(self.h, t))
+        return dict(flux_x=t[0], flux_y=t[1], flux_z=t[2], update=t[3])

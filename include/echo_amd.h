@@ -1,0 +1,137 @@
+/*
+ * echo_amd.h - C-ABI of the MI355X (gfx950) per-timestep grid update for
+ * ECHO-QGP (reference: gabriele-inghirami/echo-qgp-mhd-devel).
+ *
+ * The reference has no FFI: its operator API is a set of Fortran module
+ * procedures acting on module arrays.  Each entry point below replaces one
+ * of them; the comment names the reference procedure (file:line).  Only
+ * plain C types cross the boundary (int32_t, double, pointers, sizes), so a
+ * Fortran caller binds it with iso_c_binding.
+ *
+ * Arrays that cross the boundary use the reference's Fortran layout:
+ * a(ix, iy, iz, k) column-major, x fastest, variable slowest, FP64, i.e. the
+ * C index ((k*nz + iz)*ny + iy)*nx + ix.  State lives on the device between
+ * calls; host copies are current only after echo_download_state.
+ *
+ * Every function returns 0 on success or a positive error code: the
+ * reference's own codes where one applies (777/107 superluminal in
+ * con2prim, 333 in the flux, 121-138 EoS out of table range) and
+ * ECHO_E_* below for errors of this library.  echo_last_error() gives a
+ * message for the last failure in the calling thread.
+ */
+#ifndef ECHO_AMD_H
+#define ECHO_AMD_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ECHO_E_ARG     1001  /* invalid argument / unsupported option */
+#define ECHO_E_HIP     1002  /* HIP runtime error */
+#define ECHO_E_NOMEM   1003  /* device allocation failed */
+#define ECHO_E_CRASHED 1004  /* the run crashed earlier (run_crashed) */
+
+/* Run configuration.  Field meaning and units follow the reference
+ * param.dat keys parsed in common.f90:874-1215 (key in brackets).  The
+ * layout is 16 int32 followed by 19 double, no padding. */
+typedef struct echo_config {
+  int32_t nx, ny, nz;          /* [NX, NY, NZ] */
+  int32_t coordinates;         /* [COORD] 1 Minkowski, 2 Bjorken (Milne) */
+  int32_t viscosity;           /* [VISCOUS] 0/1 */
+  int32_t bulk;                /* [BULK] 0/1 */
+  int32_t mhd;                 /* [MHD] 0 (1/2 not supported yet: ECHO_E_ARG) */
+  int32_t divclean;            /* [DIVCLEAN] */
+  int32_t eqstate;             /* [EQ_STATE] 1 p=e/3, 2 ideal gas, 3 table */
+  int32_t recal;               /* [REC_ALGO] 1 TVD2 2 CENO3 3 WENO3 4 WENO5
+                                  5 PPM4 6 MPE3 7 MPE5 */
+  int32_t nrk;                 /* [INT_ALGO] 2 RK2, 3 RK3 */
+  int32_t boundary_cond;       /* [BOUND_CON] 0 periodic, 2 outflow */
+  int32_t algo_solver;         /* [ALG_SOLV] (MHD only) */
+  int32_t obtained_zz;         /* [TRACE_IMP] 0 'no', 1 'zz' */
+  int32_t init_type;           /* [INIT_TYPE] (selects viscous tau_pi rules) */
+  int32_t derivatives_out;     /* [DER_OUT] */
+  double x1, x2, y1, y2, z1, z2; /* [XMIN..ZMAX] */
+  double tmin;                 /* [TSTART] */
+  double cfl;                  /* [CFL] */
+  double maxdt;                /* [MAXDT] */
+  double maxspeed;             /* [MAXSPEED] */
+  double eta_over_s;           /* [ETA_S] */
+  double tau_pi_coeff;         /* [TAU_FACT] */
+  double enezero;              /* [ENEZERO] */
+  double glm_alpha;            /* [GLM_PARAM] */
+  double sigma_el;             /* [SIGMA_EL] */
+  double sigma_chi;            /* [SIGMA_CHI] */
+  double dtout;                /* [DTOUT] */
+  double smooth_temp;          /* [CUT_TEMP] */
+  double tmax;                 /* [TSTOP] */
+} echo_config;
+
+typedef struct echo_grid echo_grid;   /* one event's grid on one device */
+
+/* Set-up of one grid: replaces echo.f90:73-109 (system_setup,
+ * parallel_grid_setup, common_alloc, common_grid, set_eos_pointers,
+ * read_data_from_tabulated_file, find_przero, evolve_alloc).
+ * eos_rows: for eqstate 3, the table rows as stored in the reference's
+ * eos_data files (T, e/T^4, p/T^4, c_s^2), n_rows of them, converted as
+ * eos.f90:323-327 does; NULL otherwise.  device: HIP device ordinal. */
+int echo_create(const echo_config *cfg, const double *eos_rows, int32_t n_rows,
+                int32_t device, echo_grid **out);
+void echo_destroy(echo_grid *g);
+
+/* Number of evolved variables nv (system_setup, system.f90:67-128). */
+int echo_nv(const echo_grid *g);
+
+/* Initial state from primitives: the per-cell system_prim2cons followed by
+ * system_cons2prim of the reference initialisers (init.f90:2937-2958).
+ * v: nv*nx*ny*nz doubles (Fortran layout), host memory. */
+int echo_init_from_prim(echo_grid *g, const double *v);
+
+/* Upload / download primitive and conserved state (host memory, Fortran
+ * layout).  Replaces the reference's direct access to module arrays v, u
+ * (common.f90:203-214). */
+int echo_upload_state(echo_grid *g, const double *v, const double *u);
+int echo_download_state(echo_grid *g, double *v, double *u);
+
+/* nsteps full time steps: the body of the reference main loop echo.f90:
+ * 185-241 (vnewstep/vold bookkeeping, evolve_main for irk = 1..nrk, time
+ * and metric update), all on the device.  Returns 0 or the error code of a
+ * crashed step (the reference's errcode). */
+int echo_steps(echo_grid *g, int32_t nsteps);
+
+/* One RK stage: evolve_main(irk, dt), evolve.f90:114-298.  Stages must be
+ * called in order 1..nrk; echo_end_step then does the bookkeeping of
+ * echo.f90:192-241.  echo_steps is the fused equivalent. */
+int echo_rk_stage(echo_grid *g, int32_t irk);
+int echo_end_step(echo_grid *g);
+
+/* Scalars of the run (reads device state):
+ * s[0] t  s[1] dt  s[2] timeold  s[3] timeinterval  s[4] g_cov_3_old
+ * s[5] glm_ch_old  s[6] tout  s[7] iterations  s[8] run_crashed  s[9] errcode */
+int echo_get_scalars(echo_grid *g, double *s);
+
+/* Next output time tout (echo.f90:172, 243-247); the dt clamps of
+ * evolve.f90:179-188 use it. */
+int echo_set_tout(echo_grid *g, double tout);
+
+/* Batched con2prim: system_cons2prim (system.f90:423) on n independent
+ * states at the grid's current metric.  u: nv*n conserved (variable-major,
+ * u[k*n + i]); v: nv*n primitives in/out (in: guess, out: solution);
+ * status: n error codes (0 = converged).  Host pointers. */
+int echo_cons2prim_batch(echo_grid *g, const double *u, double *v, int32_t *status,
+                         int64_t n);
+
+/* Kernel timing of the last echo_steps call, in milliseconds per launch,
+ * measured with HIP events on the grid's stream: t[0] flux x, t[1] flux y,
+ * t[2] flux z, t[3] cell update.  Only filled when timing was enabled with
+ * echo_set_timing(g, 1). */
+int echo_set_timing(echo_grid *g, int32_t on);
+int echo_get_timing(echo_grid *g, double *t);
+
+const char *echo_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
