@@ -145,6 +145,9 @@ def main():
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain", action="store_true",
+                    help="time with plain kernel launches instead of graph replays "
+                         "(for counter collection with rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU sharding "
@@ -158,6 +161,8 @@ def main():
     g.init_from_prim(v0)
     nx, ny, nz, nv = args.nx, args.ny, args.nz, g.nv
     N = nx * ny * nz
+    if args.plain:
+        g.set_timing(True)
     g.steps(args.warmup)
     t0 = time.perf_counter()
     g.steps(args.steps)          # returns after the device has finished

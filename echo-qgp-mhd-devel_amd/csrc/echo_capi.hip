@@ -47,6 +47,7 @@ struct echo_grid {
   double* VS[2] = {nullptr, nullptr};
   double* partial = nullptr;
   double* tab = nullptr;
+  int* tabix = nullptr;
   int n_tab = 0;
   DevScalars* S = nullptr;
   int* faillist = nullptr;
@@ -70,6 +71,11 @@ static void rk_coeffs(int nrk, int irk, double& c1, double& c2) {
 }
 
 static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// block count padded to a multiple of the XCD count (see xcd_block)
+static inline unsigned nblocks_xcd(long long n, int bs) {
+  unsigned nb = nblocks(n, bs);
+  return (nb + NXCD - 1) / NXCD * NXCD;
+}
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
@@ -113,7 +119,7 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   const bool last = (irk == nrk);
   double* Ucur = (irk == 1) ? g->U[0] : g->U[1];
   double* Uout = last ? g->U[0] : g->U[1];
-  k_update<VISC, ZZ><<<nblocks(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+  k_update<VISC, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
       g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
       g->VS[0], g->VS[1], g->faillist, c1, c2, last ? 1 : 0);
   if (g->timing) hipEventRecord(g->ev[5], g->stream);
@@ -169,7 +175,7 @@ static void release(echo_grid* g) {
   if (g->graph) hipGraphDestroy(g->graph);
   hipFree(g->V); hipFree(g->U[0]); hipFree(g->U[1]);
   for (int d = 0; d < 3; d++) { hipFree(g->F[d]); hipFree(g->L[d]); }
-  hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->partial); hipFree(g->tab);
+  hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->partial); hipFree(g->tab); hipFree(g->tabix);
   hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
   for (auto& e : g->ev) if (e) hipEventDestroy(e);
   if (g->stream) hipStreamDestroy(g->stream);
@@ -209,7 +215,7 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
   g->nf[2] = (long long)c.nx * c.ny * (c.nz + 1);
   g->active[0] = c.nx > 1; g->active[1] = c.ny > 1; g->active[2] = c.nz > 1;
-  for (int d = 0; d < 3; d++) g->nb[d] = (int)nblocks(g->nf[d], FLUX_BLOCK);
+  for (int d = 0; d < 3; d++) g->nb[d] = (int)nblocks_xcd(g->nf[d], FLUX_BLOCK);
   const size_t szs = sizeof(double) * (size_t)g->nv * g->N;
 #define ALLOC(p, bytes)                                                           \
   do {                                                                            \
@@ -254,12 +260,42 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
       tab[2 * n_rows + i] = p / hc3 * pow(T, 4.0);
       tab[3 * n_rows + i] = s;
     }
+    for (int i = 1; i < n_rows; i++) {
+      if (!(tab[n_rows + i] > tab[n_rows + i - 1]) || !(tab[2 * n_rows + i] > tab[2 * n_rows + i - 1])) {
+        release(g);
+        return fail(ECHO_E_ARG, "EoS table: energy density and pressure must increase strictly");
+      }
+    }
     ALLOC(g->tab, sizeof(double) * tab.size());
     if (hipMemcpy(g->tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
       release(g);
       return fail(ECHO_E_HIP, "table upload failed");
     }
     E.n = n_rows;
+    // bin starts of the bracket accelerator (phys.hpp, tab_bracket_fast)
+    std::vector<int> ix[2];
+    int base[2], nb[2];
+    for (int w = 0; w < 2; w++) {
+      const double* a = tab.data() + (w == 0 ? n_rows : 2 * n_rows);
+      float f1 = (float)a[1], fl = (float)a[n_rows - 1];
+      unsigned u1, ul;
+      memcpy(&u1, &f1, 4);
+      memcpy(&ul, &fl, 4);
+      base[w] = (int)(u1 >> TAB_SHIFT) - 1;
+      nb[w] = (int)(ul >> TAB_SHIFT) - base[w] + 1;
+      ix[w].assign(nb[w] + 1, 0);
+      // start[b] = #{j : bin(a[j]) < b}
+      std::vector<int> cnt(nb[w] + 1, 0);
+      for (int j = 0; j < n_rows; j++) cnt[tab_bin(a[j], base[w], nb[w]) + 1]++;
+      for (int b = 1; b <= nb[w]; b++) ix[w][b] = ix[w][b - 1] + cnt[b];
+    }
+    ALLOC(g->tabix, sizeof(int) * (ix[0].size() + ix[1].size()));
+    if (hipMemcpy(g->tabix, ix[0].data(), sizeof(int) * ix[0].size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->tabix + ix[0].size(), ix[1].data(), sizeof(int) * ix[1].size(), hipMemcpyHostToDevice) != hipSuccess) {
+      release(g);
+      return fail(ECHO_E_HIP, "table index upload failed");
+    }
+    E.base_e = base[0]; E.nb_e = nb[0]; E.base_p = base[1]; E.nb_p = nb[1];
   }
 #undef ALLOC
   // find_przero (eos.f90:192-218), evaluated on the host table copy
@@ -285,6 +321,8 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   }
   if (c.eqstate == 3) {
     E.tt = g->tab; E.te = g->tab + n_rows; E.tp = g->tab + 2 * n_rows; E.tc = g->tab + 3 * n_rows;
+    E.ixe = g->tabix;
+    E.ixp = g->tabix + (E.nb_e + 1);
   }
   GridArgs& a = g->ga;
   a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.coordinates = c.coordinates; a.bc = c.boundary_cond;

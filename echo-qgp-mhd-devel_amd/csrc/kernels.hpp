@@ -37,6 +37,18 @@ struct GridArgs {
 constexpr int FLUX_BLOCK = 256;
 constexpr int UPDATE_BLOCK = 256;
 constexpr int FAIL_CAP = 4096;
+constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
+
+// XCD-aware block order.  Workgroups are dispatched round-robin over the
+// XCDs (block b runs on XCD b % 8), so neighbouring blocks, which share
+// stencil rows, would sit in different L2 caches.  Launch grids are padded
+// to a multiple of NXCD and block b takes the logical tile
+// (b % NXCD) * (nb / NXCD) + b / NXCD: each XCD then walks one contiguous
+// eighth of the index space.  A bijection on [0, nb).
+__device__ __forceinline__ long long xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb / NXCD;
+  return (long long)(b % NXCD) * per + b / NXCD;
+}
 
 // cell index along a direction of length m for a stencil position c0
 // (0-based, may lie in the ghost range -3..m+2): boundary conditions of
@@ -63,7 +75,7 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
   const int m = DIR == 1 ? nx : (DIR == 2 ? ny : nz);
   const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
                      : (DIR == 2 ? (long long)nx * (ny + 1) * nz : (long long)nx * ny * (nz + 1));
-  const long long f = (long long)blockIdx.x * FLUX_BLOCK + threadIdx.x;
+  const long long f = xcd_block(blockIdx.x, gridDim.x) * FLUX_BLOCK + threadIdx.x;
   double amax = 0.;
   if (f < nf) {
     int x, y, z, i;
@@ -146,9 +158,9 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
         }
       }
     }
-    double ul[NV], ur[NV], fl[NV], fr[NV], vfl[2], vfr[2];
-    int e = sys_flux<DIR, VISC, ZZ>(a.eos, M, wl, ul, fl, vfl);
-    if (e == 0) e = sys_flux<DIR, VISC, ZZ>(a.eos, M, wr, ur, fr, vfr);
+    SideState sl, sr;
+    int e = side_state<DIR, VISC, ZZ>(a.eos, M, wl, sl);
+    if (e == 0) e = side_state<DIR, VISC, ZZ>(a.eos, M, wr, sr);
     if (e != 0) {
       // the reference stops the run (run_crashed) on a flux error
       atomicMax(&a.S->errcode, e);
@@ -156,13 +168,22 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
 #pragma unroll
       for (int k = 0; k < NV; k++) F[(long long)k * nf + f] = 0.;
     } else {
-      double ap = dmax(dmax(0., vfl[0]), vfr[0]);
-      double am = dmax(dmax(0., -vfl[1]), -vfr[1]);
+      // HLL, work.f90:195-229
+      double vfl0, vfl1, vfr0, vfr1;
+      side_speeds<DIR>(M, sl, vfl0, vfl1);
+      side_speeds<DIR>(M, sr, vfr0, vfr1);
+      double ap = dmax(dmax(0., vfl0), vfr0);
+      double am = dmax(dmax(0., -vfl1), -vfr1);
       amax = dmax(ap, am);
       double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
 #pragma unroll
-      for (int k = 0; k < NV; k++)
-        F[(long long)k * nf + f] = a1 * (ap * fl[k] + am * fr[k] - ap * am * (ur[k] - ul[k]));
+      for (int k = 0; k < NV; k++) {
+        double ulk = u_comp<VISC>(k, M, sl, wl) * M.gp;
+        double urk = u_comp<VISC>(k, M, sr, wr) * M.gp;
+        double flk = f_comp<DIR, VISC>(k, M, sl, wl) * M.gp;
+        double frk = f_comp<DIR, VISC>(k, M, sr, wr) * M.gp;
+        F[(long long)k * nf + f] = a1 * (ap * flk + am * frk - ap * am * (urk - ulk));
+      }
     }
   }
   // block max of the interface speeds -> partial[blockIdx.x]
@@ -294,7 +315,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     const double* __restrict__ LY, const double* __restrict__ LZ, double* __restrict__ VS0,
     double* __restrict__ VS1, int* __restrict__ faillist, double c1, double c2, int last_stage) {
   constexpr int NV = VISC ? 12 : 5;
-  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
   if (c >= a.N) return;
   DevScalars* S = a.S;
   if (S->run_crashed) return;

@@ -28,6 +28,11 @@ struct Eos {
   const double* __restrict__ te;   // energy density
   const double* __restrict__ tp;   // pressure
   const double* __restrict__ tc;   // c_s^2
+  // bracket accelerator (see tab_bracket_fast); null pointers select the
+  // plain bisection
+  const int* __restrict__ ixe;     // bin starts over te
+  const int* __restrict__ ixp;     // bin starts over tp
+  int base_e, nb_e, base_p, nb_p;
 };
 
 struct Metric {
@@ -61,6 +66,49 @@ __host__ __device__ __forceinline__ int tab_bracket(const double* __restrict__ a
   return lo;
 }
 
+// Bracket accelerator.  For a strictly increasing table a[0..n-1] and
+// a[0] <= x <= a[n-1], the bisection above returns max(1, c) with
+// c = #{j : a[j] < x}, clamped to n-1, whatever path it takes.  c is found
+// here by a short lower-bound search inside one bin of a monotone bin
+// function: the bit pattern of the float nearest to x, shifted right by
+// TAB_SHIFT (256 bins per octave).  Because the bin function is
+// monotone, entries in lower bins are < x and entries in higher bins are
+// > x, so only the entries of x's own bin (at most 13 for qcdIEOS0.dat) are
+// searched.  The result is the same index as tab_bracket.
+constexpr int TAB_SHIFT = 15;
+
+__host__ __device__ __forceinline__ int tab_bin(double x, int base, int nb) {
+  float f = (float)x;
+  unsigned u;
+#ifdef __HIP_DEVICE_COMPILE__
+  u = __float_as_uint(f);
+#else
+  __builtin_memcpy(&u, &f, 4);
+#endif
+  if (u >> 31) return 0;   // negative values, -0 included
+  int b = (int)(u >> TAB_SHIFT) - base;
+  return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+__host__ __device__ __forceinline__ int tab_bracket_fast(const double* __restrict__ a,
+                                                         const int* __restrict__ start, int base,
+                                                         int nb, int n, double x) {
+  int b = tab_bin(x, base, nb);
+  int lo = start[b], hi = start[b + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+}
+
+__host__ __device__ __forceinline__ int bracket_e(const Eos& E, double x) {
+  return E.ixe ? tab_bracket_fast(E.te, E.ixe, E.base_e, E.nb_e, E.n, x) : tab_bracket(E.te, E.n, x);
+}
+__host__ __device__ __forceinline__ int bracket_p(const Eos& E, double x) {
+  return E.ixp ? tab_bracket_fast(E.tp, E.ixp, E.base_p, E.nb_p, E.n, x) : tab_bracket(E.tp, E.n, x);
+}
+
 // eos_pressure_ana (eos.f90:888-925) / get_pressure (eos.f90:399-464)
 __host__ __device__ __forceinline__ int eos_pressure(const Eos& E, double rh, double en, double& pr,
                                             double& dprdrh, double& dprden) {
@@ -71,7 +119,7 @@ __host__ __device__ __forceinline__ int eos_pressure(const Eos& E, double rh, do
   }
   if (en < E.te[0]) return 123;
   if (en > E.te[E.n - 1]) return 124;
-  int l = tab_bracket(E.te, E.n, en);
+  int l = bracket_e(E, en);
   double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
   double t2 = E.tt[l], t1 = E.tt[l - 1];
   pr = p1 + (p2 - p1) * (en - e1) / (e2 - e1);
@@ -91,7 +139,7 @@ __host__ __device__ __forceinline__ int eos_energy(const Eos& E, double rh, doub
   }
   if (pr < E.tp[0]) return 125;
   if (pr > E.tp[E.n - 1]) return 126;
-  int l = tab_bracket(E.tp, E.n, pr);
+  int l = bracket_p(E, pr);
   double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
   en = e1 + (e2 - e1) * (pr - p1) / (p2 - p1);
   return 0;
@@ -109,7 +157,7 @@ __host__ __device__ __forceinline__ int eos_sound(const Eos& E, double rh, doubl
   }
   if (en < E.te[0]) return 131;
   if (en > E.te[E.n - 1]) return 132;
-  int l = tab_bracket(E.te, E.n, en);
+  int l = bracket_e(E, en);
   double s2 = E.tc[l], s1 = E.tc[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
   cs2 = s1 + (s2 - s1) * (en - e1) / (e2 - e1);
   return 0;
@@ -126,7 +174,7 @@ __host__ __device__ __forceinline__ int get_derived_data(const Eos& E, double rh
   }
   if (pr < E.tp[0]) return 135;
   if (pr > E.tp[E.n - 1]) return 136;
-  int l = tab_bracket(E.tp, E.n, pr);
+  int l = bracket_p(E, pr);
   double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
   double t2 = E.tt[l], t1 = E.tt[l - 1];
   double f = (pr - p1) / (p2 - p1);
@@ -182,22 +230,26 @@ __device__ __forceinline__ double mp5(double f, double f1, double f2, double f3,
   return r;
 }
 
-// one WENO5 side, holib.f90:206-217
+// One WENO5 side (holib.f90:206-217).  The reference's weights
+// alpha_k = d_k / (beta_k/12 + eps)^2, normalised by their sum, are
+// evaluated as  w_k = d_k * prod_{j != k} Q_j  with Q_j = (beta_j + 12 eps)^2:
+// the same ratios (the factor 144 cancels), with one division per side
+// instead of seven.  Results agree with the reference form to rounding.
 __device__ __forceinline__ double weno5_side(double f1, double f2, double f3, double f4, double f5) {
-  const double d1 = 1 / 16., d2 = 10 / 16., d3 = 5 / 16., eps = 1.e-6;
+  const double d1 = 1 / 16., d2 = 10 / 16., d3 = 5 / 16., eps12 = 12. * 1.e-6;
   double t, s;
   t = f3 - 2 * f2 + f1; s = 3. * f3 - 4. * f2 + f1;
-  double beta1 = 13. * (t * t) + 3. * (s * s);
+  double q1 = 13. * (t * t) + 3. * (s * s) + eps12;
   t = f4 - 2 * f3 + f2; s = f4 - f2;
-  double beta2 = 13. * (t * t) + 3. * (s * s);
+  double q2 = 13. * (t * t) + 3. * (s * s) + eps12;
   t = f5 - 2 * f4 + f3; s = f5 - 4. * f4 + 3. * f3;
-  double beta3 = 13. * (t * t) + 3. * (s * s);
-  double q1 = beta1 / 12. + eps, q2 = beta2 / 12. + eps, q3 = beta3 / 12. + eps;
-  double a1 = d1 / (q1 * q1), a2 = d2 / (q2 * q2), a3 = d3 / (q3 * q3);
+  double q3 = 13. * (t * t) + 3. * (s * s) + eps12;
+  double Q1 = q1 * q1, Q2 = q2 * q2, Q3 = q3 * q3;
+  double w1 = d1 * (Q2 * Q3), w2 = d2 * (Q1 * Q3), w3 = d3 * (Q1 * Q2);
   double p1 = 3 / 8. * f1 + -10 / 8. * f2 + 15 / 8. * f3;
   double p2 = -1 / 8. * f2 + 6 / 8. * f3 + 3 / 8. * f4;
   double p3 = 3 / 8. * f3 + 6 / 8. * f4 + -1 / 8. * f5;
-  return (a1 * p1 + a2 * p2 + a3 * p3) / (a1 + a2 + a3);
+  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
 }
 
 __device__ __forceinline__ double weno3_side(double f1, double f2, double f3) {
@@ -421,6 +473,91 @@ __device__ __forceinline__ int sys_flux(const Eos& E, const Metric& M, double* v
   vf[0] = vf1 + vf2;
   vf[1] = vf1 - vf2;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// system_flux split for the HLL solver: side_state evaluates the scalar part
+// of system_flux (system.f90:1336-1400 and the fast speeds of 1557-1580) for
+// one reconstructed state; u_comp / f_comp then give conserved component k
+// and its flux along DIR with the same operations as sys_flux, before the
+// final sqrt(-g) factor.  Keeping only these scalars live (instead of four
+// nv-vectors) is what keeps the flux kernel's register use low.
+// ---------------------------------------------------------------------------
+struct SideState {
+  double rh, glf, ww, pr, v2, cs2, vdir, wwdir;
+  PiDer d;
+};
+
+template <int DIR, bool VISC, bool ZZ>
+__device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double* v, SideState& s) {
+  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
+  int err = eos_energy(E, rh, pr, en);
+  if (err > 0) return err;
+  double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
+  if (v2 > 1) return 333;
+  double glf = 1. / sqrt(1. - v2);
+  double w = VISC ? en + pr + v[KPIBU] : en + pr;
+  s.ww = w * (glf * glf);
+  s.rh = rh; s.glf = glf; s.pr = pr; s.v2 = v2;
+  s.d = PiDer{0., 0., 0., 0.};
+  if constexpr (VISC) s.d = derived_pi<ZZ>(v, M);
+  s.vdir = DIR == 1 ? vx : (DIR == 2 ? vy : vz);
+  s.wwdir = s.ww * s.vdir;
+  return eos_sound(E, rh, en, pr, s.cs2);
+}
+
+// fast speeds lambda+ (vf0) and lambda- (vf1), system.f90:1557-1580 (ca2 = 0)
+template <int DIR>
+__device__ __forceinline__ void side_speeds(const Metric& M, const SideState& s, double& lp, double& lm) {
+  const double gdir = DIR == 1 ? M.g1 : (DIR == 2 ? M.g2 : M.g3);
+  const double ca2 = 0.;
+  double a2 = s.cs2 + ca2 - s.cs2 * ca2;
+  double den = 1. / (1. - s.v2 * a2);
+  double vf1 = den * (1. - a2) * s.vdir;
+  double vf2 = den * sqrt(a2 * (1. - s.v2) * ((1. - s.v2 * a2) / gdir - (1. - a2) * (s.vdir * s.vdir)));
+  lp = vf1 + vf2;
+  lm = vf1 - vf2;
+}
+
+template <bool VISC>
+__device__ __forceinline__ double u_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  switch (k) {
+    case KRH: return s.rh * s.glf;
+    case KVX: return VISC ? s.ww * v[KVX] + s.d.pitx : s.ww * v[KVX];
+    case KVY: return VISC ? s.ww * v[KVY] + s.d.pity : s.ww * v[KVY];
+    case KVZ: return VISC ? (s.ww * v[KVZ] + s.d.pitz) * M.g3 : s.ww * v[KVZ] * M.g3;
+    case KPR: return VISC ? s.ww - s.pr - v[KPIBU] - s.d.pitt * M.g0 : s.ww - s.pr;
+    default: return v[k] * (s.rh * s.glf);
+  }
+}
+
+template <int DIR, bool VISC>
+__device__ __forceinline__ double f_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  const double wd = s.wwdir;
+  if (k == KRH) return s.rh * s.glf * s.vdir;
+  if (k >= 5) return (v[k] * (s.rh * s.glf)) * s.vdir;
+  if constexpr (DIR == 1) {
+    switch (k) {
+      case KVX: return VISC ? wd * v[KVX] * M.g1 + s.pr + v[KPIBU] + v[KPIXX] * M.g1 : wd * v[KVX] * M.g1 + s.pr;
+      case KVY: return VISC ? wd * v[KVY] * M.g2 + v[KPIXY] * M.g2 : wd * v[KVY] * M.g2;
+      case KVZ: return VISC ? wd * v[KVZ] * M.g3 + v[KPIXZ] * M.g3 : wd * v[KVZ] * M.g3;
+      default: return VISC ? wd + s.d.pitx : wd;
+    }
+  } else if constexpr (DIR == 2) {
+    switch (k) {
+      case KVX: return VISC ? wd * v[KVX] * M.g1 + v[KPIXY] * M.g1 : wd * v[KVX] * M.g1;
+      case KVY: return VISC ? wd * v[KVY] * M.g2 + s.pr + v[KPIBU] + v[KPIYY] * M.g2 : wd * v[KVY] * M.g2 + s.pr;
+      case KVZ: return VISC ? wd * v[KVZ] * M.g3 + v[KPIYZ] * M.g3 : wd * v[KVZ] * M.g3;
+      default: return VISC ? wd + s.d.pity : wd;
+    }
+  } else {
+    switch (k) {
+      case KVX: return VISC ? wd * v[KVX] * M.g1 + v[KPIXZ] * M.g1 : wd * v[KVX] * M.g1;
+      case KVY: return VISC ? wd * v[KVY] * M.g2 + v[KPIYZ] * M.g2 : wd * v[KVY] * M.g2;
+      case KVZ: return VISC ? wd * v[KVZ] * M.g3 + s.pr + v[KPIBU] + v[KPIZZ] * M.g3 : wd * v[KVZ] * M.g3 + s.pr;
+      default: return VISC ? wd + s.d.pitz : wd;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

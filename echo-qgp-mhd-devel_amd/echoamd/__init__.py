@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libechoamd.so")
+# ECHO_AMD_LIB selects another build of the library (e.g. an experiment)
+LIB_PATH = os.environ.get("ECHO_AMD_LIB", os.path.join(PKG_DIR, "libechoamd.so"))
 EOS_DIR = os.path.join(PKG_DIR, "eos_data")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
