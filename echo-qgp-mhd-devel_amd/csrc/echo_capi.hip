@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/echo_amd.h"
@@ -80,34 +81,34 @@ static inline unsigned nblocks_xcd(long long n, int bs) {
 // ---------------------------------------------------------------------------
 // kernel dispatch
 // ---------------------------------------------------------------------------
-template <int DIR, bool VISC, bool ZZ>
+template <int DIR, int SYS, bool ZZ>
 static void launch_flux_dir(echo_grid* g, double* partial) {
   dim3 grid(g->nb[DIR - 1]), block(FLUX_BLOCK);
   double* F = g->F[DIR - 1];
   double* L = g->L[DIR - 1];
   switch (g->cfg.recal) {
-    case REC_TVD2: k_flux<DIR, REC_TVD2, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_CENO3: k_flux<DIR, REC_CENO3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_WENO3: k_flux<DIR, REC_WENO3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_PPM4: k_flux<DIR, REC_PPM4, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_MPE3: k_flux<DIR, REC_MPE3, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_MPE5: k_flux<DIR, REC_MPE5, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    default: k_flux<DIR, REC_WENO5, VISC, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_TVD2: k_flux<DIR, REC_TVD2, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_CENO3: k_flux<DIR, REC_CENO3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_WENO3: k_flux<DIR, REC_WENO3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_PPM4: k_flux<DIR, REC_PPM4, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_MPE3: k_flux<DIR, REC_MPE3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_MPE5: k_flux<DIR, REC_MPE5, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    default: k_flux<DIR, REC_WENO5, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
   }
 }
 
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 static void enqueue_stage_t(echo_grid* g, int irk) {
   const int nrk = g->cfg.nrk;
   double c1, c2;
   rk_coeffs(nrk, irk, c1, c2);
   double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
   if (g->timing) hipEventRecord(g->ev[0], g->stream);
-  if (g->active[0]) launch_flux_dir<1, VISC, ZZ>(g, part[0]);
+  if (g->active[0]) launch_flux_dir<1, SYS, ZZ>(g, part[0]);
   if (g->timing) hipEventRecord(g->ev[1], g->stream);
-  if (g->active[1]) launch_flux_dir<2, VISC, ZZ>(g, part[1]);
+  if (g->active[1]) launch_flux_dir<2, SYS, ZZ>(g, part[1]);
   if (g->timing) hipEventRecord(g->ev[2], g->stream);
-  if (g->active[2]) launch_flux_dir<3, VISC, ZZ>(g, part[2]);
+  if (g->active[2]) launch_flux_dir<3, SYS, ZZ>(g, part[2]);
   if (g->timing) hipEventRecord(g->ev[3], g->stream);
   if (irk == 1) {
     k_cfl<<<1, 1024, 0, g->stream>>>(g->ga, g->partial, g->nb[0], g->nb[1], g->nb[2]);
@@ -119,20 +120,33 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   const bool last = (irk == nrk);
   double* Ucur = (irk == 1) ? g->U[0] : g->U[1];
   double* Uout = last ? g->U[0] : g->U[1];
-  k_update<VISC, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+  k_update<SYS, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
       g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
-      g->VS[0], g->VS[1], g->faillist, c1, c2, last ? 1 : 0);
+      g->VS[0], g->VS[1], g->faillist, c1, c2, irk == 1 ? 1 : 0, last ? 1 : 0);
   if (g->timing) hipEventRecord(g->ev[5], g->stream);
   // con2prim repair of the cells k_update could not invert (rare); it
   // writes the next step's vnewstep too, before k_end_step flips the parity
-  k_fixup<VISC, ZZ><<<1, 64, 0, g->stream>>>(g->ga, g->V, Uout, g->faillist, g->VS[0],
-                                              g->VS[1], last ? 1 : 0);
+  k_fixup<SYS, ZZ><<<1, 64, 0, g->stream>>>(g->ga, g->V, Uout, g->faillist, g->VS[0],
+                                             g->VS[1], last ? 1 : 0);
+}
+
+// the system of echo_config: hydro, viscous hydro ('no'/'zz'), ideal MHD
+// without / with GLM
+template <typename Fn>
+static void dispatch_sys(const echo_config& c, Fn&& fn) {
+  if (c.mhd) {
+    if (c.divclean) fn(std::integral_constant<int, SYS_MHDGLM>(), std::false_type());
+    else fn(std::integral_constant<int, SYS_MHD>(), std::false_type());
+  } else if (c.viscosity) {
+    if (c.obtained_zz) fn(std::integral_constant<int, SYS_VISC>(), std::true_type());
+    else fn(std::integral_constant<int, SYS_VISC>(), std::false_type());
+  } else {
+    fn(std::integral_constant<int, SYS_HD>(), std::false_type());
+  }
 }
 
 static void enqueue_stage(echo_grid* g, int irk) {
-  if (!g->cfg.viscosity) enqueue_stage_t<false, false>(g, irk);
-  else if (g->cfg.obtained_zz) enqueue_stage_t<true, true>(g, irk);
-  else enqueue_stage_t<true, false>(g, irk);
+  dispatch_sys(g->cfg, [&](auto sys, auto zz) { enqueue_stage_t<decltype(sys)::value, decltype(zz)::value>(g, irk); });
 }
 
 static void enqueue_end_step(echo_grid* g) {
@@ -192,7 +206,13 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   *out = nullptr;
   const echo_config& c = *cfg;
   if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(ECHO_E_ARG, "grid sizes must be >= 1");
-  if (c.mhd != 0) return fail(ECHO_E_ARG, "MHD=%d is not supported by this build", c.mhd);
+  if (c.mhd == 2) return fail(ECHO_E_ARG, "resistive MHD (MHD=2) is not supported by this build");
+  if (c.mhd != 0 && c.mhd != 1) return fail(ECHO_E_ARG, "MHD must be 0 or 1");
+  if (c.mhd && c.viscosity) return fail(ECHO_E_ARG, "viscosity together with MHD is not supported");
+  if (c.mhd && !((c.algo_solver == 4 && c.eqstate == 2) || (c.algo_solver == 5 && c.eqstate == 1) ||
+                 (c.algo_solver == 6 && c.eqstate == 1 && c.coordinates == 1)))
+    return fail(ECHO_E_ARG, "MHD needs ALG_SOLV 4 with EQ_STATE 2, or 5 (or 6 in Minkowski) with "
+                            "EQ_STATE 1 (got %d, %d)", c.algo_solver, c.eqstate);
   if (c.coordinates != 1 && c.coordinates != 2) return fail(ECHO_E_ARG, "COORD must be 1 or 2");
   if (c.nrk != 2 && c.nrk != 3) return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2) or RK3 (3)");
   if (c.boundary_cond != 0 && c.boundary_cond != 2)
@@ -209,7 +229,7 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   g->dev = device;
   hipError_t he = hipSetDevice(device);
   if (he != hipSuccess) { delete g; return fail(ECHO_E_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
-  g->nv = c.viscosity ? 12 : 5;
+  g->nv = c.mhd ? (c.divclean ? 9 : 8) : (c.viscosity ? 12 : 5);
   g->N = (long long)c.nx * c.ny * c.nz;
   g->nf[0] = (long long)(c.nx + 1) * c.ny * c.nz;
   g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
@@ -313,6 +333,7 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
     S0.t = c.tmin; S0.timeold = c.tmin; S0.timeinterval = 0.; S0.g_cov_3_old = 0.;
     S0.dt = c.maxdt; S0.tout = c.tmin + c.dtout; S0.dtout = c.dtout; S0.tmax = c.tmax;
     S0.glm_ch_old = 0.5;
+    S0.glm_ch = 0.5;
     S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? smooth_threshold(Eh, c.smooth_temp) : 0.;
     if (hipMemcpy(g->S, &S0, sizeof S0, hipMemcpyHostToDevice) != hipSuccess) {
       release(g);
@@ -336,6 +357,9 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   a.vp.eta_over_s = c.eta_over_s; a.vp.tau_pi_coeff = c.tau_pi_coeff;
   a.vp.bulkvis = c.viscosity && c.bulk; a.vp.init_type = c.init_type;
   a.enable_smooth = c.viscosity && c.smooth_temp > 0.;
+  a.algo_solver = c.algo_solver;
+  a.divclean = c.mhd && c.divclean;
+  a.glm_alpha = c.glm_alpha;
   a.S = g->S;
   if (hipMemset(g->faillist, 0, sizeof(int) * FAIL_CAP) != hipSuccess) {
     release(g);
@@ -385,12 +409,10 @@ int echo_init_from_prim(echo_grid* g, const double* v) {
   HIPCHECK(hipMemcpyAsync(g->status, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHECK(hipMemcpyAsync(&g->S->vs_parity, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
   dim3 grid(nblocks(g->N, UPDATE_BLOCK));
-  if (!g->cfg.viscosity)
-    k_init<false, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
-  else if (g->cfg.obtained_zz)
-    k_init<true, true><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
-  else
-    k_init<true, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
+  dispatch_sys(g->cfg, [&](auto sys, auto zz) {
+    k_init<decltype(sys)::value, decltype(zz)::value>
+        <<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], g->VS[0], g->status);
+  });
   HIPCHECK(hipGetLastError());
   int st = 0;
   HIPCHECK(hipMemcpyAsync(&st, g->status, sizeof(int), hipMemcpyDeviceToHost, g->stream));
@@ -499,9 +521,10 @@ int echo_cons2prim_batch(echo_grid* g, const double* u, double* v, int32_t* stat
   HIPCHECK(hipMemcpyAsync(du, u, sz, hipMemcpyHostToDevice, g->stream));
   HIPCHECK(hipMemcpyAsync(dv, v, sz, hipMemcpyHostToDevice, g->stream));
   dim3 grid(nblocks(n, UPDATE_BLOCK));
-  if (!g->cfg.viscosity) k_c2p_batch<false, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
-  else if (g->cfg.obtained_zz) k_c2p_batch<true, true><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
-  else k_c2p_batch<true, false><<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
+  dispatch_sys(g->cfg, [&](auto sys, auto zz) {
+    k_c2p_batch<decltype(sys)::value, decltype(zz)::value>
+        <<<grid, UPDATE_BLOCK, 0, g->stream>>>(g->ga, du, dv, ds, n);
+  });
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(v, dv, sz, hipMemcpyDeviceToHost, g->stream));
   HIPCHECK(hipMemcpyAsync(status, ds, sizeof(int) * n, hipMemcpyDeviceToHost, g->stream));

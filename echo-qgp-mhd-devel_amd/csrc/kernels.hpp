@@ -20,6 +20,7 @@ namespace echo {
 struct DevScalars {
   double t, dt, timeold, timeinterval, g_cov_3_old, tout, glm_ch, glm_ch_old;
   double smooth_pressure, dtout, tmax, pad;
+  double aflux[3], pad1;   // max(ap, am) at the last interface of each sweep
   int iter, run_crashed, errcode, nfail;
   int vs_parity, pad2, pad3, pad4;
 };
@@ -31,6 +32,9 @@ struct GridArgs {
   Eos eos;
   ViscParams vp;
   int enable_smooth;
+  int algo_solver;      // MHD con2prim algorithm (4, 5 or 6)
+  int divclean;         // Dedner GLM on
+  double glm_alpha;     // GLM damping parameter
   DevScalars* S;
 };
 
@@ -66,11 +70,12 @@ __device__ __forceinline__ int bc_map(int c0, int m, int bc) {
 // system.f90:1316-1583) and writes it, plus a per-block max of the fast
 // speeds for the CFL condition (evolve.f90:723-764).
 // ---------------------------------------------------------------------------
-template <int DIR, int REC, bool VISC, bool ZZ>
+template <int DIR, int REC, int SYS, bool ZZ>
 __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
                                                      double* __restrict__ partial) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int m = DIR == 1 ? nx : (DIR == 2 ? ny : nz);
   const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
@@ -158,9 +163,20 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
         }
       }
     }
+    if constexpr (SysT<SYS>::GLM) {
+      // work_glm{x,y,z}, work.f90:67-139: Dedner interface values of psi and
+      // the normal B.  The reference sets wr(psi) = wl(psi) before taking
+      // their difference, so the B update reduces to the average.
+      constexpr int KB = DIR == 1 ? KBX : (DIR == 2 ? KBY : KBZ);
+      const double ch = S->glm_ch;
+      wl[KGLM] = 0.5 * (wl[KGLM] + wr[KGLM]) - (wr[KB] - wl[KB]) * ch / 2;
+      wr[KGLM] = wl[KGLM];
+      wl[KB] = 0.5 * (wl[KB] + wr[KB]) - (wr[KGLM] - wl[KGLM]) / (2 * ch);
+      wr[KB] = wl[KB];
+    }
     SideState sl, sr;
-    int e = side_state<DIR, VISC, ZZ>(a.eos, M, wl, sl);
-    if (e == 0) e = side_state<DIR, VISC, ZZ>(a.eos, M, wr, sr);
+    int e = side_state<DIR, SYS, ZZ>(a.eos, M, wl, sl);
+    if (e == 0) e = side_state<DIR, SYS, ZZ>(a.eos, M, wr, sr);
     if (e != 0) {
       // the reference stops the run (run_crashed) on a flux error
       atomicMax(&a.S->errcode, e);
@@ -175,13 +191,15 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
       double ap = dmax(dmax(0., vfl0), vfr0);
       double am = dmax(dmax(0., -vfl1), -vfr1);
       amax = dmax(ap, am);
+      // aflux(dir) keeps the last interface of the sweep (work.f90:217,310,401)
+      if (f == nf - 1) a.S->aflux[DIR - 1] = amax;
       double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
 #pragma unroll
       for (int k = 0; k < NV; k++) {
-        double ulk = u_comp<VISC>(k, M, sl, wl) * M.gp;
-        double urk = u_comp<VISC>(k, M, sr, wr) * M.gp;
-        double flk = f_comp<DIR, VISC>(k, M, sl, wl) * M.gp;
-        double frk = f_comp<DIR, VISC>(k, M, sr, wr) * M.gp;
+        double ulk = u_comp<SYS>(k, M, sl, wl) * M.gp;
+        double urk = u_comp<SYS>(k, M, sr, wr) * M.gp;
+        double flk = f_comp<DIR, SYS>(k, M, sl, wl) * M.gp;
+        double frk = f_comp<DIR, SYS>(k, M, sr, wr) * M.gp;
         F[(long long)k * nf + f] = a1 * (ap * flk + am * frk - ap * am * (urk - ulk));
       }
     }
@@ -237,7 +255,9 @@ __global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* __restri
     if (a.nx > 1) amax = dmax(amax, a.ddx * mm[0]);
     if (a.ny > 1) amax = dmax(amax, a.ddy * mm[1]);
     if (a.nz > 1) amax = dmax(amax, a.ddz * mm[2]);
-    if (amax == 0) {
+    if (a.divclean) {
+      amax = 1.;   // evolve.f90:734-736
+    } else if (amax == 0) {
       if (a.nx > 1) amax = dmax(amax, a.ddx);
       if (a.ny > 1) amax = dmax(amax, a.ddy);
       if (a.nz > 1) amax = dmax(amax, a.ddz);
@@ -307,14 +327,15 @@ __device__ __forceinline__ Deriv cell_deriv(const GridArgs& a, const Metric& M, 
 // smoothing (evolve_smooth_viscosity, evolve.f90:1159-1207).  Cells whose
 // con2prim fails are appended to a list for k_fixup.
 // ---------------------------------------------------------------------------
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
-    GridArgs a, double* __restrict__ V, const double* __restrict__ U0,
-    const double* __restrict__ Ucur, double* __restrict__ Uout, const double* __restrict__ FX,
-    const double* __restrict__ FY, const double* __restrict__ FZ, const double* __restrict__ LX,
-    const double* __restrict__ LY, const double* __restrict__ LZ, double* __restrict__ VS0,
-    double* __restrict__ VS1, int* __restrict__ faillist, double c1, double c2, int last_stage) {
-  constexpr int NV = VISC ? 12 : 5;
+    GridArgs a, double* __restrict__ V, double* U0, const double* Ucur, double* Uout,
+    const double* __restrict__ FX, const double* __restrict__ FY, const double* __restrict__ FZ,
+    const double* __restrict__ LX, const double* __restrict__ LY, const double* __restrict__ LZ,
+    double* __restrict__ VS0, double* __restrict__ VS1, int* __restrict__ faillist, double c1,
+    double c2, int first_stage, int last_stage) {
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
   const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
   if (c >= a.N) return;
   DevScalars* S = a.S;
@@ -356,16 +377,36 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     double vv[NV], src[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) vv[k] = v[k];
-    int e = sys_source<VISC, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
+    int e = sys_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
     if (e > 0) { atomicMax(&S->errcode, e); atomicExch(&S->run_crashed, 1); return; }
 #pragma unroll
     for (int k = 0; k < NV; k++) du[k] = du[k] - src[k];
   }
   const double dt = S->dt;
+  double u0[NV], uc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) { u0[k] = U0[k * N + c]; uc[k] = Ucur[k * N + c]; }
+  if constexpr (SysT<SYS>::GLM) {
+    if (first_stage) {
+      // evolve_glm_source(dt/2) before u0 = u (evolve.f90:196): the damped
+      // psi is also the RK base of the later stages
+      const double ch = S->glm_ch;
+      double ef = exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
+      u0[KGLM] = u0[KGLM] * ef;
+      uc[KGLM] = u0[KGLM];
+      U0[KGLM * N + c] = u0[KGLM];
+    }
+  }
   double un[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) un[k] = c1 * U0[k * N + c] + c2 * (Ucur[k * N + c] - dt * du[k]);
-  int e = cons2prim<VISC, ZZ>(a.eos, M, un, v);
+  for (int k = 0; k < NV; k++) un[k] = c1 * u0[k] + c2 * (uc[k] - dt * du[k]);
+  if constexpr (SysT<SYS>::GLM) {
+    if (last_stage) {   // evolve.f90:201
+      const double ch = S->glm_ch;
+      un[KGLM] = un[KGLM] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
+    }
+  }
+  int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);
   if (e != 0) {
     int slot = atomicAdd(&S->nfail, 1);
     if (slot < FAIL_CAP) faillist[slot] = (int)c;
@@ -376,7 +417,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 #pragma unroll
       for (int k = KPIBU; k <= KPIZZ; k++) v[k] = v[k] * (q * q);
       if constexpr (ZZ) derived_pi<true>(v, M);
-      prim2cons<VISC, ZZ>(a.eos, M, v, un);
+      prim2cons<SYS, ZZ>(a.eos, M, v, un);
     }
   }
 #pragma unroll
@@ -394,13 +435,18 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 // neighbour averages read the six face neighbours, clamped to the grid
 // (the reference reads outside its arrays on boundary cells).
 // ---------------------------------------------------------------------------
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
                         int* __restrict__ faillist, double* __restrict__ VS0,
                         double* __restrict__ VS1, int last_stage) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevScalars* S = a.S;
+  // glm_ch_old = maxval(aflux) of this stage; the next stage starts with
+  // glm_ch = glm_ch_old (evolve.f90:157-163)
+  S->glm_ch_old = dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
+  S->glm_ch = S->glm_ch_old;
   int n = S->nfail;
   if (n == 0) return;
   S->nfail = 0;
@@ -428,7 +474,7 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
       uu[k] = (Uk[nb[0]] + Uk[nb[1]] + Uk[nb[2]] + Uk[nb[3]] + Uk[nb[4]] + Uk[nb[5]]) / 6.;
       vv[k] = V[k * N + c];
     }
-    cons2prim<VISC, ZZ>(a.eos, M, uu, vv);
+    cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
     double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
     if (v2 > 1) {
       double s = a.maxspeed / sqrt(v2);
@@ -438,8 +484,8 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
       const double* P = V + KPR * N;
       vv[KPR] = (P[nb[0]] + P[nb[1]] + P[nb[2]] + P[nb[3]] + P[nb[4]] + P[nb[5]]) / 6.;
     }
-    prim2cons<VISC, ZZ>(a.eos, M, vv, uu);
-    cons2prim<VISC, ZZ>(a.eos, M, uu, vv);
+    prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
+    cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
     v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
     if (!((vv[KPR] > 0) && (v2 < 1))) {
       S->errcode = 1; S->run_crashed = 1; return;
@@ -449,7 +495,7 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
 #pragma unroll
       for (int k = KPIBU; k <= KPIZZ; k++) vv[k] = vv[k] * (q * q);
       if constexpr (ZZ) derived_pi<true>(vv, M);
-      prim2cons<VISC, ZZ>(a.eos, M, vv, uu);
+      prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
     }
 #pragma unroll
     for (int k = 0; k < NV; k++) { U[k * N + c] = uu[k]; V[k * N + c] = vv[k]; }
@@ -478,11 +524,12 @@ __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
 }
 
 // initial state: per-cell prim2cons then cons2prim (init.f90:2937-2958)
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_init(GridArgs a, double* __restrict__ V,
                                                        double* __restrict__ U, double* __restrict__ VS0,
                                                        int* __restrict__ status) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
   const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
   if (c >= a.N) return;
   const long long N = a.N;
@@ -490,8 +537,8 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_init(GridArgs a, double* __res
   double v[NV], u[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = V[k * N + c];
-  int e = prim2cons<VISC, ZZ>(a.eos, M, v, u);
-  if (e == 0 && !VISC) e = cons2prim<VISC, ZZ>(a.eos, M, u, v);
+  int e = prim2cons<SYS, ZZ>(a.eos, M, v, u);
+  if (e == 0 && !VISC) e = cons2prim<SYS, ZZ>(a.eos, M, u, v, a.algo_solver);
   if (e) atomicMax(status, e);
 #pragma unroll
   for (int k = 0; k < NV; k++) { V[k * N + c] = v[k]; U[k * N + c] = u[k]; }
@@ -499,18 +546,18 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_init(GridArgs a, double* __res
 }
 
 // batched con2prim over n independent states (system_cons2prim)
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_c2p_batch(GridArgs a, const double* __restrict__ Ub,
                                                             double* __restrict__ Vb,
                                                             int* __restrict__ status, long long n) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr int NV = SysT<SYS>::NV;
   const long long i = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
   if (i >= n) return;
   const Metric M = make_metric(a.coordinates, a.S->t);
   double u[NV], v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) { u[k] = Ub[k * n + i]; v[k] = Vb[k * n + i]; }
-  status[i] = cons2prim<VISC, ZZ>(a.eos, M, u, v);
+  status[i] = cons2prim<SYS, ZZ>(a.eos, M, u, v, a.algo_solver);
 #pragma unroll
   for (int k = 0; k < NV; k++) Vb[k * n + i] = v[k];
 }

@@ -16,6 +16,19 @@ enum { KRH = 0, KVX = 1, KVY = 2, KVZ = 3, KPR = 4, KPIBU = 5, KPIXY = 6,
 enum { MINKOWSKI = 1, BJORKEN = 2 };
 enum { SHEAR_VISCOUS_1D = 2, GUBSER_VISCOUS = 5 };
 
+// Evolved systems (system_setup, system.f90:67-128): hydro, viscous hydro,
+// ideal MHD, ideal MHD with Dedner GLM divergence cleaning.
+enum { SYS_HD = 0, SYS_VISC = 1, SYS_MHD = 2, SYS_MHDGLM = 3 };
+template <int SYS>
+struct SysT {
+  static constexpr bool VISC = SYS == SYS_VISC;
+  static constexpr bool MHD = SYS == SYS_MHD || SYS == SYS_MHDGLM;
+  static constexpr bool GLM = SYS == SYS_MHDGLM;
+  static constexpr int NV = SYS == SYS_HD ? 5 : (SYS == SYS_VISC ? 12 : (SYS == SYS_MHD ? 8 : 9));
+};
+// magnetic field and GLM slots of the MHD systems (kbx..kbz, kglm)
+enum { KBX = 5, KBY = 6, KBZ = 7, KGLM = 8 };
+
 // Reconstruction codes (same numbering as the C-ABI echo_config.recal).
 enum { REC_TVD2 = 1, REC_CENO3 = 2, REC_WENO3 = 3, REC_WENO5 = 4, REC_PPM4 = 5,
        REC_MPE3 = 6, REC_MPE5 = 7 };
@@ -343,9 +356,24 @@ __device__ __forceinline__ PiDer derived_pi(double* p, const Metric& M) {
 // ---------------------------------------------------------------------------
 // prim2cons, system.f90:164-263
 // ---------------------------------------------------------------------------
-template <bool VISC, bool ZZ>
+// system_compute_E_in_ideal_MHD (system.f90:1882-1897): covariant E = -v x B
+// times sqrt(-g); system_EXB (1846-1862): E x B
+__device__ __forceinline__ void ideal_E(const Metric& M, const double* v, double* E) {
+  E[0] = -M.gp * (v[KVY] * v[KBZ] - v[KVZ] * v[KBY]);
+  E[1] = -M.gp * (v[KVZ] * v[KBX] - v[KVX] * v[KBZ]);
+  E[2] = -M.gp * (v[KVX] * v[KBY] - v[KVY] * v[KBX]);
+}
+__device__ __forceinline__ void e_cross_b(const Metric& M, const double* E, const double* v, double* X) {
+  double c0 = E[0], c1 = E[1], c2 = E[2] / M.g3;
+  X[0] = M.gp * (c1 * v[KBZ] - c2 * v[KBY]);
+  X[1] = M.gp * (c2 * v[KBX] - c0 * v[KBZ]);
+  X[2] = M.gp * (c0 * v[KBY] - c1 * v[KBX]);
+}
+
+template <int SYS, bool ZZ>
 __device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* v, double* u) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
   int err = eos_energy(E, rh, pr, en);
   if (err > 0) return err;
@@ -369,6 +397,19 @@ __device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* 
     u[KPIYZ] = v[KPIYZ] * u[KRH];
     u[KPIXX] = v[KPIXX] * u[KRH];
     u[KPIYY] = v[KPIYY] * u[KRH];
+  } else if constexpr (SysT<SYS>::MHD) {
+    double Ef[3], X[3];
+    ideal_E(M, v, Ef);
+    double E2 = Ef[0] * Ef[0] + Ef[1] * Ef[1] + Ef[2] * Ef[2] / M.g3;
+    e_cross_b(M, Ef, v, X);
+    double B2 = v[KBX] * v[KBX] + v[KBY] * v[KBY] + v[KBZ] * v[KBZ] * M.g3;
+    double EEM = 0.5 * (E2 + B2);
+    u[KVX] = ww * vx + X[0];
+    u[KVY] = ww * vy + X[1];
+    u[KVZ] = ww * vz * M.g3 + X[2];
+    u[KPR] = ww - pr + EEM;
+    u[KBX] = v[KBX]; u[KBY] = v[KBY]; u[KBZ] = v[KBZ];
+    if constexpr (SysT<SYS>::GLM) u[KGLM] = v[KGLM] / M.gp;
   } else {
     u[KVX] = ww * vx;
     u[KVY] = ww * vy;
@@ -381,101 +422,6 @@ __device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* 
 }
 
 // ---------------------------------------------------------------------------
-// system_flux, system.f90:1316-1583.  DIR = 1,2,3.  Returns u, f (nv each)
-// and the two fast speeds vf[0] = lambda+, vf[1] = lambda-.
-// ---------------------------------------------------------------------------
-template <int DIR, bool VISC, bool ZZ>
-__device__ __forceinline__ int sys_flux(const Eos& E, const Metric& M, double* v, double* u,
-                                        double* f, double* vf) {
-  constexpr int NV = VISC ? 12 : 5;
-  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
-  int err = eos_energy(E, rh, pr, en);
-  if (err > 0) return err;
-  double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
-  if (v2 > 1) return 333;
-  double glf = 1. / sqrt(1. - v2);
-  double w = VISC ? en + pr + v[KPIBU] : en + pr;
-  double ww = w * (glf * glf);
-  PiDer d{0., 0., 0., 0.};
-  u[KRH] = rh * glf;
-  if constexpr (VISC) {
-    d = derived_pi<ZZ>(v, M);
-#pragma unroll
-    for (int k = KPIBU; k < NV; k++) u[k] = v[k] * u[KRH];
-    u[KVX] = ww * vx + d.pitx;
-    u[KVY] = ww * vy + d.pity;
-    u[KVZ] = (ww * vz + d.pitz) * M.g3;
-    u[KPR] = ww - pr - v[KPIBU] - d.pitt * M.g0;
-  } else {
-    u[KVX] = ww * vx;
-    u[KVY] = ww * vy;
-    u[KVZ] = ww * vz * M.g3;
-    u[KPR] = ww - pr;
-  }
-  double wwdir, vdir, gdir;
-  if constexpr (DIR == 1) {
-    wwdir = ww * vx; vdir = vx; gdir = M.g1;
-    f[KRH] = rh * glf * vx;
-    if constexpr (VISC) {
-      f[KVX] = wwdir * vx * M.g1 + pr + v[KPIBU] + v[KPIXX] * M.g1;
-      f[KVY] = wwdir * vy * M.g2 + v[KPIXY] * M.g2;
-      f[KVZ] = wwdir * vz * M.g3 + v[KPIXZ] * M.g3;
-      f[KPR] = wwdir + d.pitx;
-    } else {
-      f[KVX] = wwdir * vx * M.g1 + pr;
-      f[KVY] = wwdir * vy * M.g2;
-      f[KVZ] = wwdir * vz * M.g3;
-      f[KPR] = wwdir;
-    }
-  } else if constexpr (DIR == 2) {
-    wwdir = ww * vy; vdir = vy; gdir = M.g2;
-    f[KRH] = rh * glf * vy;
-    if constexpr (VISC) {
-      f[KVX] = wwdir * vx * M.g1 + v[KPIXY] * M.g1;
-      f[KVY] = wwdir * vy * M.g2 + pr + v[KPIBU] + v[KPIYY] * M.g2;
-      f[KVZ] = wwdir * vz * M.g3 + v[KPIYZ] * M.g3;
-      f[KPR] = wwdir + d.pity;
-    } else {
-      f[KVX] = wwdir * vx * M.g1;
-      f[KVY] = wwdir * vy * M.g2 + pr;
-      f[KVZ] = wwdir * vz * M.g3;
-      f[KPR] = wwdir;
-    }
-  } else {
-    wwdir = ww * vz; vdir = vz; gdir = M.g3;
-    f[KRH] = rh * glf * vz;
-    if constexpr (VISC) {
-      f[KVX] = wwdir * vx * M.g1 + v[KPIXZ] * M.g1;
-      f[KVY] = wwdir * vy * M.g2 + v[KPIYZ] * M.g2;
-      f[KVZ] = wwdir * vz * M.g3 + pr + v[KPIBU] + v[KPIZZ] * M.g3;
-      f[KPR] = wwdir + d.pitz;
-    } else {
-      f[KVX] = wwdir * vx * M.g1;
-      f[KVY] = wwdir * vy * M.g2;
-      f[KVZ] = wwdir * vz * M.g3 + pr;
-      f[KPR] = wwdir;
-    }
-  }
-  if constexpr (VISC) {
-#pragma unroll
-    for (int k = 5; k < NV; k++) f[k] = u[k] * vdir;
-  }
-#pragma unroll
-  for (int k = 0; k < NV; k++) { u[k] = u[k] * M.gp; f[k] = f[k] * M.gp; }
-  double cs2;
-  err = eos_sound(E, rh, en, pr, cs2);
-  if (err > 0) return err;
-  const double ca2 = 0.;
-  double a2 = cs2 + ca2 - cs2 * ca2;
-  double den = 1. / (1. - v2 * a2);
-  double vf1 = den * (1. - a2) * vdir;
-  double vf2 = den * sqrt(a2 * (1. - v2) * ((1. - v2 * a2) / gdir - (1. - a2) * (vdir * vdir)));
-  vf[0] = vf1 + vf2;
-  vf[1] = vf1 - vf2;
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
 // system_flux split for the HLL solver: side_state evaluates the scalar part
 // of system_flux (system.f90:1336-1400 and the fast speeds of 1557-1580) for
 // one reconstructed state; u_comp / f_comp then give conserved component k
@@ -484,12 +430,14 @@ __device__ __forceinline__ int sys_flux(const Eos& E, const Metric& M, double* v
 // nv-vectors) is what keeps the flux kernel's register use low.
 // ---------------------------------------------------------------------------
 struct SideState {
-  double rh, glf, ww, pr, v2, cs2, vdir, wwdir;
+  double rh, glf, ww, pr, v2, cs2, ca2, vdir, wwdir;
   PiDer d;
+  double E[3], X[3], EEM;   // MHD: covariant E, E x B, field energy
 };
 
-template <int DIR, bool VISC, bool ZZ>
+template <int DIR, int SYS, bool ZZ>
 __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double* v, SideState& s) {
+  constexpr bool VISC = SysT<SYS>::VISC;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
   int err = eos_energy(E, rh, pr, en);
   if (err > 0) return err;
@@ -501,16 +449,26 @@ __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double*
   s.rh = rh; s.glf = glf; s.pr = pr; s.v2 = v2;
   s.d = PiDer{0., 0., 0., 0.};
   if constexpr (VISC) s.d = derived_pi<ZZ>(v, M);
+  s.ca2 = 0.;
+  if constexpr (SysT<SYS>::MHD) {
+    ideal_E(M, v, s.E);
+    double E2 = s.E[0] * s.E[0] + s.E[1] * s.E[1] + s.E[2] * s.E[2] / M.g3;
+    e_cross_b(M, s.E, v, s.X);
+    double B2 = v[KBX] * v[KBX] + v[KBY] * v[KBY] + v[KBZ] * v[KBZ] * M.g3;
+    s.EEM = 0.5 * (E2 + B2);
+    double bb2 = B2 - E2;
+    s.ca2 = bb2 / (bb2 + (en + pr));
+  }
   s.vdir = DIR == 1 ? vx : (DIR == 2 ? vy : vz);
   s.wwdir = s.ww * s.vdir;
   return eos_sound(E, rh, en, pr, s.cs2);
 }
 
-// fast speeds lambda+ (vf0) and lambda- (vf1), system.f90:1557-1580 (ca2 = 0)
+// fast speeds lambda+ (lp) and lambda- (lm), system.f90:1557-1580
 template <int DIR>
 __device__ __forceinline__ void side_speeds(const Metric& M, const SideState& s, double& lp, double& lm) {
   const double gdir = DIR == 1 ? M.g1 : (DIR == 2 ? M.g2 : M.g3);
-  const double ca2 = 0.;
+  const double ca2 = s.ca2;
   double a2 = s.cs2 + ca2 - s.cs2 * ca2;
   double den = 1. / (1. - s.v2 * a2);
   double vf1 = den * (1. - a2) * s.vdir;
@@ -519,8 +477,20 @@ __device__ __forceinline__ void side_speeds(const Metric& M, const SideState& s,
   lm = vf1 - vf2;
 }
 
-template <bool VISC>
+template <int SYS>
 __device__ __forceinline__ double u_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  constexpr bool VISC = SysT<SYS>::VISC;
+  if constexpr (SysT<SYS>::MHD) {
+    switch (k) {
+      case KRH: return s.rh * s.glf;
+      case KVX: return s.ww * v[KVX] + s.X[0];
+      case KVY: return s.ww * v[KVY] + s.X[1];
+      case KVZ: return s.ww * v[KVZ] * M.g3 + s.X[2];
+      case KPR: return s.ww - s.pr + s.EEM;
+      case KGLM: return v[KGLM] / M.gp;
+      default: return v[k];   // B
+    }
+  }
   switch (k) {
     case KRH: return s.rh * s.glf;
     case KVX: return VISC ? s.ww * v[KVX] + s.d.pitx : s.ww * v[KVX];
@@ -531,10 +501,50 @@ __device__ __forceinline__ double u_comp(int k, const Metric& M, const SideState
   }
 }
 
-template <int DIR, bool VISC>
+template <int DIR, int SYS>
 __device__ __forceinline__ double f_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr bool GLM = SysT<SYS>::GLM;
   const double wd = s.wwdir;
   if (k == KRH) return s.rh * s.glf * s.vdir;
+  if constexpr (SysT<SYS>::MHD) {
+    const double* E = s.E;
+    const double Bx = v[KBX], By = v[KBY], Bz = v[KBZ];
+    if constexpr (DIR == 1) {
+      switch (k) {
+        case KVX: return wd * v[KVX] - E[0] * E[0] - Bx * Bx + s.pr + s.EEM;
+        case KVY: return wd * v[KVY] - E[0] * E[1] - Bx * By;
+        case KVZ: return wd * v[KVZ] * M.g3 - E[0] * E[2] - Bx * Bz * M.g3;
+        case KPR: return wd + s.X[0];
+        case KBX: return GLM ? v[KGLM] : 0.;
+        case KBY: return -E[2] * M.gm;
+        case KBZ: return +E[1] * M.gm;
+        default: return v[KBX] / M.gp;
+      }
+    } else if constexpr (DIR == 2) {
+      switch (k) {
+        case KVX: return wd * v[KVX] - E[1] * E[0] - By * Bx;
+        case KVY: return wd * v[KVY] - E[1] * E[1] - By * By + s.pr + s.EEM;
+        case KVZ: return wd * v[KVZ] * M.g3 - E[1] * E[2] - By * Bz * M.g3;
+        case KPR: return wd + s.X[1];
+        case KBX: return E[2] * M.gm;
+        case KBY: return GLM ? v[KGLM] : 0.;
+        case KBZ: return -E[0] * M.gm;
+        default: return v[KBY] / M.gp;
+      }
+    } else {
+      switch (k) {
+        case KVX: return wd * v[KVX] - E[2] / M.g3 * E[0] - Bz * Bx;
+        case KVY: return wd * v[KVY] - E[2] / M.g3 * E[1] - Bz * By;
+        case KVZ: return wd * v[KVZ] * M.g3 - E[2] * E[2] / M.g3 - Bz * Bz * M.g3 + s.pr + s.EEM;
+        case KPR: return wd + s.X[2] / M.g3;
+        case KBX: return -E[1] * M.gm;
+        case KBY: return E[0] * M.gm;
+        case KBZ: return GLM ? v[KGLM] / M.g3 : 0.;
+        default: return v[KBZ] / M.gp;
+      }
+    }
+  }
   if (k >= 5) return (v[k] * (s.rh * s.glf)) * s.vdir;
   if constexpr (DIR == 1) {
     switch (k) {
@@ -564,9 +574,219 @@ __device__ __forceinline__ double f_comp(int k, const Metric& M, const SideState
 // cons2prim, system.f90:511-665 (ideal Newton on p; viscous nested
 // iteration).  v holds the guess on entry (v[KPR], velocities for viscous).
 // ---------------------------------------------------------------------------
-template <bool VISC, bool ZZ>
-__device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const double* u, double* v) {
-  constexpr int NV = VISC ? 12 : 5;
+// ---------------------------------------------------------------------------
+// Ideal-MHD con2prim: algorithms 4 (ideal gas), 5 (p = e/3) and 6 (quartic,
+// e = 3p, Minkowski norms) of system_cons2prim (system.f90:880-1158) with
+// rtsafe (2242-2290) and the cubic solvers solver7 / solver8 (2102-2201).
+// ---------------------------------------------------------------------------
+struct MhdPar { double B2, SB2, S2, UU, D; };
+
+template <bool IDEALGAS>
+__device__ __forceinline__ void mhd_cubic(const MhdPar& P, double x, double& f, double& df, double& wout) {
+  const double third = 1.0 / 3.0;
+  const double gam = IDEALGAS ? (GAMMA_AD - 1) / GAMMA_AD : 0.25;
+  double v2 = x;
+  double rho = IDEALGAS ? P.D * sqrt(1.0 - v2) : 0.;
+  double c3 = 1.0 - gam * (1.0 - v2);
+  double c2 = IDEALGAS ? gam * rho + .5 * P.B2 * (1.0 + v2) - P.UU : .5 * P.B2 * (1.0 + v2) - P.UU;
+  double c0 = -0.5 * P.SB2;
+  double w;
+  if (!IDEALGAS && c0 == 0.0) {
+    w = -c2 / c3;
+  } else {
+    w = dmax(-c2 / c3, pow(-c0 / c3, third));
+    const int nit = IDEALGAS ? 50 : 100;
+    const double tol = IDEALGAS ? 1.e-10 : 1.e-12;
+    for (int iter = 1; iter <= nit; iter++) {
+      double dw = -((c3 * w + c2) * (w * w) + c0) / ((3 * c3 * w + 2 * c2) * w);
+      if (fabs(dw / w) < tol) break;
+      w = w + dw;
+    }
+  }
+  double dc3 = gam;
+  double dc2 = IDEALGAS ? .5 * (P.B2 - gam * rho / (1. - v2)) : .5 * P.B2;
+  double dlogw = -(dc3 * w + dc2) / (3 * c3 * w + 2 * c2);
+  double wb = w + P.B2;
+  double vb2 = P.SB2 / (w * w);
+  f = (wb * wb) * v2 - (2 * w + P.B2) * vb2 - P.S2;
+  df = wb * (wb + 2 * dlogw * (w * v2 + vb2));
+  wout = w;
+}
+
+template <bool IDEALGAS>
+__device__ __forceinline__ double mhd_rtsafe(const MhdPar& P, double x0, double x1, double x2,
+                                             double xacc, int& ierr, double& w) {
+  double f, df, xl = x1, xh = x2, r = x0, dxold = fabs(x2 - x1), dx = dxold, temp;
+  ierr = 0;
+  mhd_cubic<IDEALGAS>(P, r, f, df, w);
+  for (int j = 1; j <= 100; j++) {
+    if (((r - xh) * df - f) * ((r - xl) * df - f) > 0. || fabs(2. * f) > fabs(dxold * df)) {
+      dxold = dx;
+      dx = 0.5 * (xh - xl);
+      r = xl + dx;
+      if (xl == r) { mhd_cubic<IDEALGAS>(P, r, f, df, w); return r; }
+    } else {
+      dxold = dx;
+      dx = f / df;
+      temp = r;
+      r = r - dx;
+      if (temp == r) { mhd_cubic<IDEALGAS>(P, r, f, df, w); return r; }
+    }
+    if (fabs(dx) < xacc) return r;
+    mhd_cubic<IDEALGAS>(P, r, f, df, w);
+    if (f < 0.) xl = r; else xh = r;
+  }
+  ierr = 1;
+  return r;
+}
+
+__device__ __forceinline__ double get_x9(double y, double s2, double b2, double sb2) {
+  return (s2 * (y * y) + b2 * sb2 + 2 * sb2 * y) / ((y * y) * ((b2 * b2) + 2 * b2 * y + (y * y)));
+}
+
+template <int SYS>
+__device__ __forceinline__ int mhd_cons2prim(int algo, const Metric& M, const double* u, double* v) {
+  constexpr int NV = SysT<SYS>::NV;
+  const double gm = M.gm, g3 = M.g3;
+  double uin[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) uin[k] = gm * u[k];
+  const double D = uin[KRH], sx = uin[KVX], sy = uin[KVY], sz = uin[KVZ], UU = uin[KPR];
+  const double bx = uin[KBX], by = uin[KBY], bz = uin[KBZ];
+  const double SB = (sx * bx + sy * by + sz * bz);
+  const double SB2 = SB * SB;
+  if (algo == 4 || algo == 5) {
+    MhdPar P;
+    P.S2 = sx * sx + sy * sy + sz * sz / g3;
+    P.B2 = bx * bx + by * by + bz * bz * g3;
+    P.SB2 = SB2; P.UU = UU; P.D = D;
+    double v2, w;
+    int ierr;
+    if (algo == 4) {
+      double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ];
+      v2 = mhd_rtsafe<true>(P, v2_old, 0.0, 1.0 - 1.e-11, 1.e-12, ierr, w);
+      if (ierr > 0) return 0;   // the reference returns with errcode 0, v untouched
+#pragma unroll
+      for (int k = 0; k < NV; k++) v[k] = uin[k];
+      if constexpr (SysT<SYS>::GLM) v[KGLM] = u[KGLM];
+      double rh = D * sqrt(1. - v2);
+      double pg = (GAMMA_AD - 1) / GAMMA_AD * (w * (1. - v2) - rh);
+      v[KRH] = rh;
+      v[KPR] = dmax(pg, 1.e-12);
+    } else {
+      double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ] * g3;
+      v2 = mhd_rtsafe<false>(P, v2_old, 0.0, 1.0 - 1.e-11, 1.e-12, ierr, w);
+      if ((v2 >= 1) || (ierr > 0)) return 778;
+#pragma unroll
+      for (int k = 0; k < NV; k++) v[k] = uin[k];
+      if constexpr (SysT<SYS>::GLM) v[KGLM] = u[KGLM];
+      v[KPR] = 0.25 * (1 - v2) * w;
+      v[KRH] = D * sqrt(1. - v2);
+    }
+    double vb = SB / w;
+    v[KVX] = (sx + vb * bx) / (w + P.B2);
+    v[KVY] = (sy + vb * by) / (w + P.B2);
+    v[KVZ] = (sz / g3 + vb * bz) / (w + P.B2);
+    return 0;
+  }
+  if (algo == 6) {
+    // exact quartic for e = 3p (system.f90:1000-1158)
+    double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ];
+    double oldsol9 = 4 * v[KPR] / (1 - v2_old);
+    double S2 = sx * sx + sy * sy + sz * sz;
+    double B2 = bx * bx + by * by + bz * bz;
+    double b9 = 8 * B2 - 4 * UU;
+    double c9 = 7 * (B2 * B2) - 8 * B2 * UU + S2;
+    double d9 = 2 * (B2 * B2 * B2) - 4 * (B2 * B2) * UU + 2 * B2 * S2;
+    double e9 = SB2 * B2;
+    double p9 = (24 * c9 - 3 * (b9 * b9)) / 72;
+    double q9 = ((b9 * b9 * b9) - 12 * b9 * c9 + 72 * d9) / 216;
+    double dnull9 = (c9 * c9) - 3 * b9 * d9 + 36 * e9;
+    double deinz9 = 2 * (c9 * c9 * c9) - 9 * b9 * c9 * d9 + 27 * (b9 * b9) * e9 + 81 * (d9 * d9) - 216 * c9 * e9;
+    double discr9 = (deinz9 * deinz9) - 4 * (dnull9 * dnull9 * dnull9);
+    double sbig9 = 0, sol9 = -1, v2 = 0;
+    bool exact9 = true;
+    if (discr9 > 0) {
+      double droot9 = sqrt(discr9);
+      double qbig9 = pow(0.5 * (deinz9 + droot9), 1. / 3.);
+      double drad9 = -2 * p9 / 3 + (qbig9 + dnull9 / qbig9) / 9;
+      if (drad9 <= 0) return 12;
+      sbig9 = 0.5 * sqrt(drad9);
+    } else if (dnull9 > 0) {
+      double phi9 = acos(deinz9 / (2 * sqrt(dnull9 * dnull9 * dnull9)));
+      sbig9 = 0.5 * sqrt(-2. * p9 / 3 + 2. / 9. * sqrt(dnull9) * cos(phi9 / 3.));
+    } else {
+      // solver10_approximate, system.f90:2205-2223
+      double xx = oldsol9;
+      int iter;
+      for (iter = 1; iter <= 100; iter++) {
+        double dx = -(12 * (xx * xx * xx) + 3 * b9 * (xx * xx) + 2 * c9 * xx + d9);
+        if (fabs(dx / xx) < 1.e-12) break;
+        xx = xx + dx;
+      }
+      if (iter == 100) return 12;
+      sol9 = xx;
+      exact9 = false;
+      v2 = get_x9(sol9, S2, B2, SB2);
+      if ((v2 < 0) || (v2 > 1) || (sol9 < 0)) return 12;
+    }
+    if (exact9) {
+      double rad129 = -4 * (sbig9 * sbig9) - 2 * p9 + q9 / sbig9;
+      double rad349 = -4 * (sbig9 * sbig9) - 2 * p9 - q9 / sbig9;
+      double rapp9 = -b9 / 12;
+      double sol1 = -1, sol2 = -1, sol3 = -1, sol4 = -1, dist9 = 1.e100, v9;
+      if (rad129 >= 0) { double r = 0.5 * sqrt(rad129); sol1 = rapp9 - sbig9 - r; sol2 = rapp9 - sbig9 + r; }
+      if (rad349 >= 0) { double r = 0.5 * sqrt(rad349); sol3 = rapp9 + sbig9 - r; sol4 = rapp9 + sbig9 + r; }
+      sol9 = -1.;
+      if (sol2 > 0) {
+        v9 = get_x9(sol2, S2, B2, SB2);
+        if ((v9 >= 0) && (v9 < 1)) {
+          sol9 = sol2; v2 = v9; dist9 = fabs(oldsol9 - sol9);
+          if (sol1 > 0) {
+            v9 = get_x9(sol1, S2, B2, SB2);
+            if ((v9 >= 0) && (v9 < 1)) {
+              double d19 = fabs(oldsol9 - sol1);
+              if (d19 <= dist9) { sol9 = sol1; dist9 = d19; v2 = v9; }
+            }
+          }
+        }
+      }
+      if (sol4 > 0) {
+        v9 = get_x9(sol4, S2, B2, SB2);
+        if ((v9 >= 0) && (v9 < 1)) {
+          double d49 = fabs(oldsol9 - sol4);
+          if (d49 <= dist9) { sol9 = sol4; dist9 = d49; v2 = v9; }
+        }
+        if (sol3 > 0) {
+          v9 = get_x9(sol3, S2, B2, SB2);
+          if ((v9 >= 0) && (v9 < 1)) {
+            double d39 = fabs(oldsol9 - sol3);
+            if (d39 <= dist9) { sol9 = sol3; dist9 = d39; v2 = v9; }
+          }
+        }
+      }
+      if (sol9 < 0) return 12;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = uin[k];
+    v[KRH] = D * sqrt(1. - v2);
+    v[KPR] = sol9 / 4 * (1 - v2);
+    double vb = SB / sol9;
+    v[KVX] = (sx + vb * bx) / (sol9 + B2);
+    v[KVY] = (sy + vb * by) / (sol9 + B2);
+    v[KVZ] = (sz + vb * bz) / (sol9 + B2);
+    if constexpr (SysT<SYS>::GLM) v[KGLM] = u[KGLM];
+    return 0;
+  }
+  return 11;
+}
+
+template <int SYS, bool ZZ>
+__device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const double* u, double* v,
+                                         int algo = 5) {
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
+  if constexpr (SysT<SYS>::MHD) return mhd_cons2prim<SYS>(algo, M, u, v);
   const double tol = 1.e-12, eps = 1.e-12;
   const double gm = M.gm;
   int errcode = 0;

@@ -82,11 +82,12 @@ __device__ __forceinline__ int get_viscous_parameters(const Eos& E, const ViscPa
 
 // system_source, system.f90:1587-1842 for HD (VISC=false) and viscous HD.
 // v: the cell's primitives (a private copy: pi^zz may be rewritten).
-template <bool VISC, bool ZZ>
+template <int SYS, bool ZZ>
 __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coordinates,
                                           const ViscParams& P, double* v, const Deriv& D,
                                           double* src) {
-  constexpr int NV = VISC ? 12 : 5;
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
 #pragma unroll
   for (int k = 0; k < NV; k++) src[k] = 0.;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
@@ -99,7 +100,18 @@ __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coo
   if constexpr (!VISC) {
     if (coordinates == BJORKEN) {
       double ww = (en + pr) * (glf * glf);
-      src[KPR] = -t * (ww * vz * vz + pr / M.g3);
+      if constexpr (SysT<SYS>::MHD) {
+        // system.f90:1643-1652
+        double Ef[3];
+        ideal_E(M, v, Ef);
+        double B2 = v[KBX] * v[KBX] + v[KBY] * v[KBY] + v[KBZ] * v[KBZ] * M.g3;
+        double E2 = Ef[0] * Ef[0] + Ef[1] * Ef[1] + Ef[2] * Ef[2] / M.g3;
+        double EEM = 0.5 * (E2 + B2);
+        double e3 = Ef[2] / M.g3;
+        src[KPR] = -t * (ww * vz * vz - e3 * e3 - v[KBZ] * v[KBZ] + (EEM + pr) / M.g3);
+      } else {
+        src[KPR] = -t * (ww * vz * vz + pr / M.g3);
+      }
     }
   } else {
     const double lambda0 = 1., lambda1 = 1., lambda2 = 0.;

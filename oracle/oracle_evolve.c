@@ -27,10 +27,16 @@ void orc_free(void) {
 
 int orc_nv(void) { return G.nv; }
 
-/* system_setup, system.f90:67-128 (HD / viscous) */
+/* system_setup, system.f90:67-128 (HD / viscous / ideal MHD with GLM) */
 static void system_setup(void) {
   G.nov = 5; G.nou = 5;
   G.nv = G.viscosity ? 12 : 5;
+  G.kbx = G.kby = G.kbz = G.kglm = -1;
+  if (G.mhd) {
+    G.nov += 3; G.nou += 3; G.nv += 3;
+    G.kbx = G.nv - 3; G.kby = G.nv - 2; G.kbz = G.nv - 1;
+    if (G.divclean) { G.nov++; G.nou++; G.nv++; G.kglm = G.nv - 1; }
+  }
   G.nu = G.nv;
 }
 
@@ -68,7 +74,8 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.dtout = rc[16]; G.smooth_temp = rc[17]; G.tmax = rc[18];
   G.enable_smooth_viscosity = G.smooth_temp > 0.;
   G.glm_ch_old = 0.5;
-  if (G.mhd) return -11;   /* MHD is not restated by this oracle */
+  G.aflux[0] = G.aflux[1] = G.aflux[2] = 0.;
+  if (G.rmhd || (G.mhd && G.viscosity)) return -11;   /* as the reference: no such system */
   system_setup();
   common_grid();
   if (G.eqstate == 3) {
@@ -281,6 +288,20 @@ static int sweep(int dir) {
           }
         }
       }
+      /* work_glm{x,y,z}, work.f90:67-139: Dedner interface values of psi and
+       * the normal B.  The reference sets wr(psi) = wl(psi) before using
+       * their difference, so the B update reduces to the average. */
+      if (G.divclean) {
+        const int kb = dir == 1 ? G.kbx : dir == 2 ? G.kby : G.kbz, kg = G.kglm;
+        for (int i = 0; i <= m; i++) {
+          double *gl = &wl[kg * (m + 1) + i], *gr = &wr[kg * (m + 1) + i];
+          double *bl = &wl[kb * (m + 1) + i], *br = &wr[kb * (m + 1) + i];
+          *gl = 0.5 * (*gl + *gr) - (*br - *bl) * G.glm_ch / 2;
+          *gr = *gl;
+          *bl = 0.5 * (*bl + *br) - (*gr - *gl) / (2 * G.glm_ch);
+          *br = *bl;
+        }
+      }
       /* work_fup? HLL, work.f90:195-229 */
       for (int i = 0; i <= m; i++) {
         for (int k = 0; k < nv; k++) { fl[k] = wl[k * (m + 1) + i]; fr[k] = wr[k * (m + 1) + i]; }
@@ -353,8 +374,17 @@ static int evolve_src(void) {
   return 0;
 }
 
+/* evolve_glm_source, evolve.f90:83-110 */
+static void evolve_glm_source(double dt_glm) {
+  double expfact = exp(-(G.glm_ch * G.glm_ch) * G.glm_alpha * dt_glm);
+  size_t nc = ncell();
+  double *a = &G.u[(size_t)G.kglm * nc];
+  for (size_t c = 0; c < nc; c++) a[c] = a[c] * expfact;
+}
+
 /* evolve_dt, evolve.f90:723-764 */
 static double evolve_dt(void) {
+  if (G.divclean) return G.cfl / 1.;
   double amax = 0.;
   const int nx = G.nx, ny = G.ny, nz = G.nz;
   for (int iz = 0; iz < nz; iz++)
@@ -513,9 +543,11 @@ static int evolve_main(int irk) {
     if (fabs(G.t + G.dt - G.tout) < 1.e-9) dt_local = G.tout - G.t;
     else if ((G.t + 2. * dt_local - G.tout) > 1.e-9) dt_local = (G.tout - G.t) / 2.;
     G.dt = dt_local;
+    if (G.divclean) evolve_glm_source(G.dt / 2.);
     memcpy(G.u0, G.u, n * sizeof(double));
   }
   for (size_t i = 0; i < n; i++) G.u[i] = c1 * G.u0[i] + c2 * (G.u[i] - G.dt * G.du[i]);
+  if (G.divclean && irk == nrk) evolve_glm_source(G.dt / 2.);
   if (evolve_u2v()) return G.errcode;
   if (G.viscosity && G.enable_smooth_viscosity) {
     if (evolve_smooth_viscosity()) return G.errcode;

@@ -63,7 +63,23 @@ static void derived_pi(double *v, double *pitt, double *pitx, double *pity, doub
   }
 }
 
-/* system_prim2cons, system.f90:164-263 (HD and viscous branches) */
+/* system_compute_E_in_ideal_MHD, system.f90:1882-1897: covariant E = -v x B
+ * (times sqrt(-g)) from contravariant v and B */
+static void compute_E_ideal(const double *V, const double *B, double *E) {
+  E[0] = -G.gp * (V[1] * B[2] - V[2] * B[1]);
+  E[1] = -G.gp * (V[2] * B[0] - V[0] * B[2]);
+  E[2] = -G.gp * (V[0] * B[1] - V[1] * B[0]);
+}
+
+/* system_EXB, system.f90:1846-1862: E x B from covariant E, contravariant B */
+static void system_EXB(const double *E, const double *B, double *X) {
+  double c0 = E[0], c1 = E[1], c2 = E[2] / G.g_cov[2];
+  X[0] = G.gp * (c1 * B[2] - c2 * B[1]);
+  X[1] = G.gp * (c2 * B[0] - c0 * B[2]);
+  X[2] = G.gp * (c0 * B[1] - c1 * B[0]);
+}
+
+/* system_prim2cons, system.f90:164-263 (HD, viscous and ideal-MHD branches) */
 int sys_prim2cons(double *v, double *u) {
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
   int err = eos_energy(rh, pr, &en);
@@ -90,6 +106,19 @@ int sys_prim2cons(double *v, double *u) {
     u[KPIYZ] = v[KPIYZ] * u[KRH];
     u[KPIXX] = v[KPIXX] * u[KRH];
     u[KPIYY] = v[KPIYY] * u[KRH];
+  } else if (G.mhd) {
+    double E[3], EXB[3];
+    compute_E_ideal(&v[KVX], &v[G.kbx], E);
+    double E2 = E[0] * E[0] + E[1] * E[1] + E[2] * E[2] / g[2];
+    system_EXB(E, &v[G.kbx], EXB);
+    double B2 = v[G.kbx] * v[G.kbx] + v[G.kby] * v[G.kby] + v[G.kbz] * v[G.kbz] * g[2];
+    double EEM = 0.5 * (E2 + B2);
+    u[KVX] = ww * vx + EXB[0];
+    u[KVY] = ww * vy + EXB[1];
+    u[KVZ] = ww * vz * g[2] + EXB[2];
+    u[KPR] = ww - pr + EEM;
+    u[G.kbx] = v[G.kbx]; u[G.kby] = v[G.kby]; u[G.kbz] = v[G.kbz];
+    if (G.divclean) u[G.kglm] = v[G.kglm] / G.gp;
   } else {
     u[KVX] = ww * vx;
     u[KVY] = ww * vy;
@@ -125,6 +154,260 @@ int sys_prim2cons_0(double *v, double *u) {
   u[KPIYY] = v[KPIYY] * u[KRH];
   for (int k = 0; k < G.nu; k++) u[k] = u[k] * G.gp;
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Ideal-MHD con2prim: algorithms 4, 5 and 6 of system_cons2prim
+ * (system.f90:880-1158) with rtsafe (2242-2290) and the cubic solvers
+ * solver7 / solver8 (2102-2201).  solver_par = (B2, SB2, S2, U, D) as the
+ * reference's solver_parameters; w_solver is the W of the last cubic solve
+ * (the reference's w_algo_solver7 / w_algo_solver8_e_3p). */
+static double solver_par[5];
+static double w_solver;
+
+/* solver7, system.f90:2102-2148 (ideal gas, Gamma = 4/3) */
+static void solver7(double x, double *f, double *df) {
+  const double third = 1.0 / 3.0;
+  const double gamma_ad = 4.0 / 3.0;
+  double gam = (gamma_ad - 1) / gamma_ad;
+  double B2 = solver_par[0], SB2 = solver_par[1], S2 = solver_par[2], UU = solver_par[3], D = solver_par[4];
+  double v2 = x;
+  double rho = D * sqrt(1.0 - v2);
+  double c3 = 1.0 - gam * (1.0 - v2);
+  double c2 = gam * rho + .5 * B2 * (1.0 + v2) - UU;
+  double c0 = -0.5 * SB2;
+  double w = fmax2(-c2 / c3, pow(-c0 / c3, third));
+  for (int iter = 1; iter <= 50; iter++) {
+    double dw = -((c3 * w + c2) * (w * w) + c0) / ((3 * c3 * w + 2 * c2) * w);
+    if (fabs(dw / w) < 1.e-10) break;
+    w = w + dw;
+  }
+  double dc3 = gam;
+  double dc2 = .5 * (B2 - gam * rho / (1. - v2));
+  double dlogw = -(dc3 * w + dc2) / (3 * c3 * w + 2 * c2);
+  double wb = w + B2;
+  double vb2 = SB2 / (w * w);
+  *f = (wb * wb) * v2 - (2 * w + B2) * vb2 - S2;
+  *df = wb * (wb + 2 * dlogw * (w * v2 + vb2));
+  w_solver = w;
+}
+
+/* solver8, system.f90:2152-2201 (p = e/3) */
+static void solver8(double x, double *f, double *df) {
+  const double third = 1.0 / 3.0;
+  const double gam = 0.25;
+  double B2 = solver_par[0], SB2 = solver_par[1], S2 = solver_par[2], UU = solver_par[3];
+  double v2 = x;
+  double c3 = 1.0 - gam * (1.0 - v2);
+  double c2 = .5 * B2 * (1.0 + v2) - UU;
+  double c0 = -0.5 * SB2;
+  double w;
+  if (c0 == 0.0) {
+    w = -c2 / c3;
+  } else {
+    w = fmax2(-c2 / c3, pow(-c0 / c3, third));
+    for (int iter = 1; iter <= 100; iter++) {
+      double dw = -((c3 * w + c2) * (w * w) + c0) / ((3 * c3 * w + 2 * c2) * w);
+      if (fabs(dw / w) < 1.e-12) break;
+      w = w + dw;
+    }
+  }
+  double dc3 = gam;
+  double dc2 = .5 * B2;
+  double dlogw = -(dc3 * w + dc2) / (3 * c3 * w + 2 * c2);
+  double wb = w + B2;
+  double vb2 = SB2 / (w * w);
+  *f = (wb * wb) * v2 - (2 * w + B2) * vb2 - S2;
+  *df = wb * (wb + 2 * dlogw * (w * v2 + vb2));
+  w_solver = w;
+}
+
+/* rtsafe, system.f90:2242-2290 (Newton safeguarded by bisection) */
+static double rtsafe(void (*func)(double, double *, double *), double x0, double x1, double x2,
+                     double xacc, int *ierr) {
+  const int maxit = 100;
+  double f, df, xl = x1, xh = x2, r = x0, dxold = fabs(x2 - x1), dx = dxold, temp;
+  *ierr = 0;
+  func(r, &f, &df);
+  for (int j = 1; j <= maxit; j++) {
+    if (((r - xh) * df - f) * ((r - xl) * df - f) > 0. || fabs(2. * f) > fabs(dxold * df)) {
+      dxold = dx;
+      dx = 0.5 * (xh - xl);
+      r = xl + dx;
+      if (xl == r) { func(r, &f, &df); return r; }
+    } else {
+      dxold = dx;
+      dx = f / df;
+      temp = r;
+      r = r - dx;
+      if (temp == r) { func(r, &f, &df); return r; }
+    }
+    if (fabs(dx) < xacc) return r;
+    func(r, &f, &df);
+    if (f < 0.) xl = r; else xh = r;
+  }
+  *ierr = 1;
+  return r;
+}
+
+/* get_x9, system.f90:2294-2301 */
+static double get_x9(double y, double s2, double b2, double sb2) {
+  return (s2 * (y * y) + b2 * sb2 + 2 * sb2 * y) / ((y * y) * ((b2 * b2) + 2 * b2 * y + (y * y)));
+}
+
+/* solver10_approximate, system.f90:2205-2223 */
+static int solver10_approximate(double b, double c, double d, double oldsol, double *x) {
+  double xx = oldsol;
+  int iter;
+  for (iter = 1; iter <= 100; iter++) {
+    double dx = -(12 * (xx * xx * xx) + 3 * b * (xx * xx) + 2 * c * xx + d);
+    if (fabs(dx / xx) < 1.e-12) break;
+    xx = xx + dx;
+  }
+  *x = xx;
+  return iter == 100 ? 12 : 0;
+}
+
+static int mhd_cons2prim(double *u, double *v) {
+  const double eps = 1.e-12;
+  const double *g = G.g_cov;
+  const double gm = G.gm;
+  const int kbx = G.kbx, kby = G.kby, kbz = G.kbz;
+  double uin[MAXNV];
+  int ierr = 0;
+  if (G.algo_solver == 4 || G.algo_solver == 5) {
+    /* system.f90:880-998 */
+    for (int k = 0; k < G.nv; k++) uin[k] = gm * u[k];
+    double D = uin[KRH], sx = uin[KVX], sy = uin[KVY], sz = uin[KVZ], UU = uin[KPR];
+    double bx = uin[kbx], by = uin[kby], bz = uin[kbz];
+    double S2 = sx * sx + sy * sy + sz * sz / g[2];
+    double B2 = bx * bx + by * by + bz * bz * g[2];
+    double SB = (sx * bx + sy * by + sz * bz);
+    double SB2 = SB * SB;
+    solver_par[0] = B2; solver_par[1] = SB2; solver_par[2] = S2; solver_par[3] = UU; solver_par[4] = D;
+    double v2, w;
+    if (G.algo_solver == 4) {
+      double gam = (4.0 / 3.0 - 1) / (4.0 / 3.0);
+      double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ];
+      v2 = rtsafe(solver7, v2_old, 0.0, 1.0 - 1.e-11, 1.e-12, &ierr);
+      if (ierr > 0) return 0;   /* the reference returns with errcode 0, v untouched */
+      for (int k = 0; k < G.nv; k++) v[k] = uin[k];
+      if (G.divclean) v[G.kglm] = u[G.kglm];
+      w = w_solver;
+      double rh = D * sqrt(1. - v2);
+      double pg = gam * (w * (1. - v2) - rh);
+      v[KRH] = rh;
+      v[KPR] = fmax2(pg, eps);
+    } else {
+      double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ] * g[2];
+      v2 = rtsafe(solver8, v2_old, 0.0, 1.0 - 1.e-11, 1.e-12, &ierr);
+      if ((v2 >= 1) || (ierr > 0)) return 778;
+      for (int k = 0; k < G.nv; k++) v[k] = uin[k];
+      if (G.divclean) v[G.kglm] = u[G.kglm];
+      w = w_solver;
+      double rh = D * sqrt(1. - v2);
+      v[KPR] = 0.25 * (1 - v2) * w;
+      v[KRH] = rh;
+    }
+    double vb = SB / w;
+    v[KVX] = (sx + vb * bx) / (w + B2);
+    v[KVY] = (sy + vb * by) / (w + B2);
+    v[KVZ] = (sz / g[2] + vb * bz) / (w + B2);
+    return 0;
+  }
+  if (G.algo_solver == 6) {
+    /* exact quartic for e = 3p, system.f90:1000-1158 (Minkowski norms) */
+    double v2_old = v[KVX] * v[KVX] + v[KVY] * v[KVY] + v[KVZ] * v[KVZ];
+    double oldsol9 = 4 * v[KPR] / (1 - v2_old);
+    for (int k = 0; k < G.nv; k++) uin[k] = gm * u[k];
+    double DD = uin[KRH], sx = uin[KVX], sy = uin[KVY], sz = uin[KVZ], UU = uin[KPR];
+    double bx = uin[kbx], by = uin[kby], bz = uin[kbz];
+    double S2 = sx * sx + sy * sy + sz * sz;
+    double B2 = bx * bx + by * by + bz * bz;
+    double SB = (sx * bx + sy * by + sz * bz);
+    double SB2 = SB * SB;
+    double b9 = 8 * B2 - 4 * UU;
+    double c9 = 7 * (B2 * B2) - 8 * B2 * UU + S2;
+    double d9 = 2 * (B2 * B2 * B2) - 4 * (B2 * B2) * UU + 2 * B2 * S2;
+    double e9 = SB2 * B2;
+    double p9 = (24 * c9 - 3 * (b9 * b9)) / 72;
+    double q9 = ((b9 * b9 * b9) - 12 * b9 * c9 + 72 * d9) / 216;
+    double dnull9 = (c9 * c9) - 3 * b9 * d9 + 36 * e9;
+    double deinz9 = 2 * (c9 * c9 * c9) - 9 * b9 * c9 * d9 + 27 * (b9 * b9) * e9 + 81 * (d9 * d9) - 216 * c9 * e9;
+    double discr9 = (deinz9 * deinz9) - 4 * (dnull9 * dnull9 * dnull9);
+    double sbig9 = 0, sol9 = -1, v2 = 0;
+    int exact9 = 1;
+    if (discr9 > 0) {
+      double droot9 = sqrt(discr9);
+      double qbig9 = pow(0.5 * (deinz9 + droot9), 1. / 3.);
+      double drad9 = -2 * p9 / 3 + (qbig9 + dnull9 / qbig9) / 9;
+      if (drad9 <= 0) return 12;
+      sbig9 = 0.5 * sqrt(drad9);
+    } else if (dnull9 > 0) {
+      double phi9 = acos(deinz9 / (2 * sqrt(dnull9 * dnull9 * dnull9)));
+      sbig9 = 0.5 * sqrt(-2. * p9 / 3 + 2. / 9. * sqrt(dnull9) * cos(phi9 / 3.));
+    } else {
+      if (solver10_approximate(b9, c9, d9, oldsol9, &sol9)) return 12;
+      exact9 = 0;
+      v2 = get_x9(sol9, S2, B2, SB2);
+      if ((v2 < 0) || (v2 > 1) || (sol9 < 0)) return 12;
+    }
+    if (exact9) {
+      double rad129 = -4 * (sbig9 * sbig9) - 2 * p9 + q9 / sbig9;
+      double rad349 = -4 * (sbig9 * sbig9) - 2 * p9 - q9 / sbig9;
+      double rapp9 = -b9 / 12;
+      double sol1 = -1, sol2 = -1, sol3 = -1, sol4 = -1, dist9 = 1.e100, v9;
+      if (rad129 >= 0) {
+        double root129 = 0.5 * sqrt(rad129);
+        sol1 = rapp9 - sbig9 - root129;
+        sol2 = rapp9 - sbig9 + root129;
+      }
+      if (rad349 >= 0) {
+        double root349 = 0.5 * sqrt(rad349);
+        sol3 = rapp9 + sbig9 - root349;
+        sol4 = rapp9 + sbig9 + root349;
+      }
+      sol9 = -1.;
+      if (sol2 > 0) {
+        v9 = get_x9(sol2, S2, B2, SB2);
+        if ((v9 >= 0) && (v9 < 1)) {
+          sol9 = sol2; v2 = v9; dist9 = fabs(oldsol9 - sol9);
+          if (sol1 > 0) {
+            v9 = get_x9(sol1, S2, B2, SB2);
+            if ((v9 >= 0) && (v9 < 1)) {
+              double d19 = fabs(oldsol9 - sol1);
+              if (d19 <= dist9) { sol9 = sol1; dist9 = d19; v2 = v9; }
+            }
+          }
+        }
+      }
+      if (sol4 > 0) {
+        v9 = get_x9(sol4, S2, B2, SB2);
+        if ((v9 >= 0) && (v9 < 1)) {
+          double d49 = fabs(oldsol9 - sol4);
+          if (d49 <= dist9) { sol9 = sol4; dist9 = d49; v2 = v9; }
+        }
+        if (sol3 > 0) {
+          v9 = get_x9(sol3, S2, B2, SB2);
+          if ((v9 >= 0) && (v9 < 1)) {
+            double d39 = fabs(oldsol9 - sol3);
+            if (d39 <= dist9) { sol9 = sol3; dist9 = d39; v2 = v9; }
+          }
+        }
+      }
+      if (sol9 < 0) return 12;
+    }
+    for (int k = 0; k < G.nv; k++) v[k] = uin[k];
+    v[KRH] = DD * sqrt(1. - v2);
+    v[KPR] = sol9 / 4 * (1 - v2);
+    double vb = SB / sol9;
+    v[KVX] = (sx + vb * bx) / (sol9 + B2);
+    v[KVY] = (sy + vb * by) / (sol9 + B2);
+    v[KVZ] = (sz + vb * bz) / (sol9 + B2);
+    if (G.divclean) v[G.kglm] = u[G.kglm];
+    return 0;
+  }
+  return 11;   /* algorithms 1-3 (BRENTM / LU-Newton) are not restated */
 }
 
 /* system_cons2prim, system.f90:423-665: ideal HD Newton on p (511-562) and
@@ -221,7 +504,7 @@ int sys_cons2prim(double *u, double *v) {
     }
     return errcode;
   }
-  return 11;   /* MHD not restated in this oracle: exit(11) branch, system.f90:1160 */
+  return mhd_cons2prim(u, v);
 }
 
 /* system_flux, system.f90:1316-1583 (HD and viscous branches) */
@@ -234,12 +517,30 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
   if (v2 > 1) return 333;
   double glf = 1. / sqrt(1. - v2);
   double w;
-  if (!G.viscosity) w = en + pr;
-  else w = en + pr + v[KPIBU];
+  double E[3] = {0, 0, 0}, EXB[3] = {0, 0, 0}, E2 = 0, B2 = 0, EEM = 0;
+  if (!G.viscosity) {
+    w = en + pr;
+    if (G.mhd) {
+      compute_E_ideal(&v[KVX], &v[G.kbx], E);
+      E2 = E[0] * E[0] + E[1] * E[1] + E[2] * E[2] / g[2];
+      system_EXB(E, &v[G.kbx], EXB);
+      B2 = v[G.kbx] * v[G.kbx] + v[G.kby] * v[G.kby] + v[G.kbz] * v[G.kbz] * g[2];
+      EEM = 0.5 * (E2 + B2);
+    }
+  } else {
+    w = en + pr + v[KPIBU];
+  }
   double ww = w * (glf * glf);
   double pitt = 0, pitx = 0, pity = 0, pitz = 0;
   u[KRH] = rh * glf;
-  if (G.viscosity) {
+  if (!G.viscosity && G.mhd) {
+    u[KVX] = ww * vx + EXB[0];
+    u[KVY] = ww * vy + EXB[1];
+    u[KVZ] = ww * vz * g[2] + EXB[2];
+    u[KPR] = ww - pr + EEM;
+    u[G.kbx] = v[G.kbx]; u[G.kby] = v[G.kby]; u[G.kbz] = v[G.kbz];
+    if (G.divclean) u[G.kglm] = v[G.kglm] / G.gp;
+  } else if (G.viscosity) {
     derived_pi(v, &pitt, &pitx, &pity, &pitz);
     for (int k = KPIBU; k < G.nv; k++) u[k] = v[k] * u[KRH];
     u[KVX] = ww * vx + pitx;
@@ -253,10 +554,20 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
     u[KPR] = ww - pr;
   }
   double wwdir, vdir, gcov_dir;
+  const double Bx = G.mhd ? v[G.kbx] : 0, By = G.mhd ? v[G.kby] : 0, Bz = G.mhd ? v[G.kbz] : 0;
   if (dir == 1) {
     wwdir = ww * vx; vdir = vx; gcov_dir = g[0];
     f[KRH] = rh * glf * vx;
-    if (G.viscosity) {
+    if (!G.viscosity && G.mhd) {
+      f[KVX] = wwdir * vx - E[0] * E[0] - Bx * Bx + pr + EEM;
+      f[KVY] = wwdir * vy - E[0] * E[1] - Bx * By;
+      f[KVZ] = wwdir * vz * g[2] - E[0] * E[2] - Bx * Bz * g[2];
+      f[KPR] = wwdir + EXB[0];
+      f[G.kbx] = 0.;
+      f[G.kby] = -E[2] * G.gm;
+      f[G.kbz] = +E[1] * G.gm;
+      if (G.divclean) { f[G.kbx] = v[G.kglm]; f[G.kglm] = v[G.kbx] / G.gp; }
+    } else if (G.viscosity) {
       f[KVX] = wwdir * vx * g[0] + pr + v[KPIBU] + v[KPIXX] * g[0];
       f[KVY] = wwdir * vy * g[1] + v[KPIXY] * g[1];
       f[KVZ] = wwdir * vz * g[2] + v[KPIXZ] * g[2];
@@ -271,7 +582,16 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
   } else if (dir == 2) {
     wwdir = ww * vy; vdir = vy; gcov_dir = g[1];
     f[KRH] = rh * glf * vy;
-    if (G.viscosity) {
+    if (!G.viscosity && G.mhd) {
+      f[KVX] = wwdir * vx - E[1] * E[0] - By * Bx;
+      f[KVY] = wwdir * vy - E[1] * E[1] - By * By + pr + EEM;
+      f[KVZ] = wwdir * vz * g[2] - E[1] * E[2] - By * Bz * g[2];
+      f[KPR] = wwdir + EXB[1];
+      f[G.kbx] = E[2] * G.gm;
+      f[G.kby] = 0.;
+      f[G.kbz] = -E[0] * G.gm;
+      if (G.divclean) { f[G.kby] = v[G.kglm]; f[G.kglm] = v[G.kby] / G.gp; }
+    } else if (G.viscosity) {
       f[KVX] = wwdir * vx * g[0] + v[KPIXY] * g[0];
       f[KVY] = wwdir * vy * g[1] + pr + v[KPIBU] + v[KPIYY] * g[1];
       f[KVZ] = wwdir * vz * g[2] + v[KPIYZ] * g[2];
@@ -286,7 +606,16 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
   } else {
     wwdir = ww * vz; vdir = vz; gcov_dir = g[2];
     f[KRH] = rh * glf * vz;
-    if (G.viscosity) {
+    if (!G.viscosity && G.mhd) {
+      f[KVX] = wwdir * vx - E[2] / g[2] * E[0] - Bz * Bx;
+      f[KVY] = wwdir * vy - E[2] / g[2] * E[1] - Bz * By;
+      f[KVZ] = wwdir * vz * g[2] - E[2] * E[2] / g[2] - Bz * Bz * g[2] + pr + EEM;
+      f[KPR] = wwdir + EXB[2] / g[2];
+      f[G.kbx] = -E[1] * G.gm;
+      f[G.kby] = E[0] * G.gm;
+      f[G.kbz] = 0.;
+      if (G.divclean) { f[G.kbz] = v[G.kglm] / g[2]; f[G.kglm] = v[G.kbz] / G.gp; }
+    } else if (G.viscosity) {
       f[KVX] = wwdir * vx * g[0] + v[KPIXZ] * g[0];
       f[KVY] = wwdir * vy * g[1] + v[KPIYZ] * g[1];
       f[KVZ] = wwdir * vz * g[2] + pr + v[KPIBU] + v[KPIZZ] * g[2];
@@ -301,6 +630,10 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
   }
   for (int k = 0; k < G.nu; k++) { u[k] = u[k] * G.gp; f[k] = f[k] * G.gp; }
   double ca2 = 0., cs2;
+  if (G.mhd && !G.viscosity) {
+    double bb2 = B2 - E2;
+    ca2 = bb2 / (bb2 + (en + pr));
+  }
   err = eos_sound(rh, en, pr, &cs2);
   if (err > 0) return err;
   double a2 = cs2 + ca2 - cs2 * ca2;
@@ -396,7 +729,17 @@ int sys_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff
     if (G.coordinates == BJORKEN) {
       double w = en + pr;
       double ww = w * (glf * glf);
-      src[KPR] = -t * (ww * vz * vz + pr / g[2]);
+      if (G.mhd) {
+        double E[3];
+        compute_E_ideal(&v[KVX], &v[G.kbx], E);
+        double B2 = v[G.kbx] * v[G.kbx] + v[G.kby] * v[G.kby] + v[G.kbz] * v[G.kbz] * g[2];
+        double E2 = E[0] * E[0] + E[1] * E[1] + E[2] * E[2] / g[2];
+        double EEM = 0.5 * (E2 + B2);
+        double e3 = E[2] / g[2];
+        src[KPR] = -t * (ww * vz * vz - e3 * e3 - v[G.kbz] * v[G.kbz] + (EEM + pr) / g[2]);
+      } else {
+        src[KPR] = -t * (ww * vz * vz + pr / g[2]);
+      }
     }
   } else {
     double sigma[6], dd[16], eta, zeta, tau_pi, tau_pi_big;

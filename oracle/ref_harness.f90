@@ -106,6 +106,7 @@ contains
     enable_smooth_viscosity = (smooth_temp > 0.d0)
     tmax = rc(19)
     glm_ch_old = 0.5d0
+    aflux = 0.d0
 
     if (eqstate == 3) then
       eos_tab_name = ''

@@ -80,3 +80,44 @@ def rel_linf(a, b):
         d = np.abs(a[k] - b[k]).max()
         out = max(out, d / scale if scale > 0 else d)
     return out
+
+
+def mhd2d(n=32, eqstate=1, algo=5, divclean=1, coordinates=1, rec="WENO5", nrk=3, bc=0, B0=0.5):
+    """A smooth 2-D MHD field: a pressure bump advected by a rotating flow
+    through an oblique magnetic field with a periodic perturbation."""
+    ic, rc = make_config(nx=n, ny=n, nz=1, coordinates=coordinates, mhd=1, divclean=divclean,
+                         eqstate=eqstate, algo_solver=algo, rec=rec, nrk=nrk, bc=bc, init_type=7,
+                         xlim=(-1.0, 1.0), ylim=(-1.0, 1.0), zlim=(-0.5, 0.5), tmin=1.0,
+                         cfl=0.4, maxdt=0.01, glm_alpha=5.0)
+    nv = 8 + (1 if divclean else 0)
+    x = cell_centres(n, -1, 1)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    r2 = X ** 2 + Y ** 2
+    v = np.zeros((nv, 1, n, n))
+    v[0, 0] = 1.0 + 0.2 * np.exp(-4 * r2)
+    v[1, 0] = -0.3 * Y * np.exp(-2 * r2)
+    v[2, 0] = 0.3 * X * np.exp(-2 * r2)
+    v[4, 0] = 1.0 + 0.5 * np.exp(-8 * r2)
+    v[5, 0] = B0 * (1 + 0.1 * np.sin(np.pi * Y))
+    v[6, 0] = 0.5 * B0 * (1 + 0.1 * np.cos(np.pi * X))
+    v[7, 0] = 0.1 * B0
+    return ic, rc, v
+
+
+def mhd_fireball3d(nx=16, ny=14, nz=10, algo=5, divclean=1, B0=0.3):
+    """3-D Bjorken fireball with a transverse magnetic field, p = e/3."""
+    ic, rc, v5 = fireball3d(nx, ny, nz)
+    ic[6] = 1
+    ic[7] = divclean
+    ic[12] = algo
+    nv = 8 + (1 if divclean else 0)
+    v = np.zeros((nv,) + v5.shape[1:])
+    v[:5] = v5
+    x = cell_centres(nx, -3, 3)
+    y = cell_centres(ny, -3, 3)
+    z = cell_centres(nz, -1, 1)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    v[6] = B0 * np.exp(-(X ** 2 + Y ** 2) / 8)
+    v[5] = 0.2 * B0 * np.sin(Y)
+    v[7] = 0.05 * B0
+    return ic, rc, v

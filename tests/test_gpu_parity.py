@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from libs import Oracle
-from scenarios import fireball3d, gubser2d, rel_linf, shock2d
+from scenarios import fireball3d, gubser2d, mhd2d, mhd_fireball3d, rel_linf, shock2d
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -145,3 +145,30 @@ def test_cons2prim_batch(oracle):
         assert np.max(np.abs(vg[:, i] - vo) / np.maximum(np.abs(vo), 1e-300)) <= 1e-12 or \
             np.max(np.abs(vg[:, i] - vo)) <= 1e-14
     assert np.max(np.abs(vg - v) / np.maximum(np.abs(v), 1e-12)) < 1e-8
+
+
+@pytest.mark.parametrize("algo,eqstate,coords,bc", [(5, 1, 1, 0), (5, 1, 2, 2), (6, 1, 1, 0), (4, 2, 1, 0)])
+@pytest.mark.parametrize("divclean", [1, 0])
+def test_mhd2d(oracle, algo, eqstate, coords, bc, divclean):
+    ic, rc, v0 = mhd2d(32, eqstate=eqstate, algo=algo, divclean=divclean, coordinates=coords, bc=bc)
+    (vg, ug), (vo, uo) = _run_both(oracle, ic, rc, v0, 12)
+    assert rel_linf(vg, vo) <= TOL
+    assert rel_linf(ug, uo) <= TOL
+
+
+@pytest.mark.parametrize("divclean", [1, 0])
+def test_mhd_fireball3d(oracle, divclean):
+    ic, rc, v0 = mhd_fireball3d(divclean=divclean)
+    (vg, ug), (vo, uo) = _run_both(oracle, ic, rc, v0, 6)
+    assert rel_linf(vg, vo) <= TOL
+    assert rel_linf(ug, uo) <= TOL
+    assert abs(oracle.scalars()["glm_ch_old"] - _gpu_scalar_after(ic, rc, v0, 6)) <= 1e-10
+
+
+def _gpu_scalar_after(ic, rc, v0, n):
+    g = _gpu(ic, rc)
+    g.init_from_prim(v0)
+    g.steps(n)
+    s = g.scalars()["glm_ch_old"]
+    g.close()
+    return s
