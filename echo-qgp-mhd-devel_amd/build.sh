@@ -1,15 +1,25 @@
 #!/bin/bash
-# Builds libechoamd.so (HIP kernels for gfx950 + the C-ABI of include/echo_amd.h).
+# Builds libechoamd.so (HIP kernels for gfx950 + the C-ABI of include/echo_amd.h)
+# from two translation units compiled in parallel: the grid update
+# (echo_capi.hip) and the batched / learned con2prim (c2p_mlp.hip).
 # FP64 multiply-adds are contracted to FMA (-ffp-contract=fast); results then
 # agree with the reference to rounding rather than bit for bit.  Packed-FP32
-# instructions are disabled (the kernels are FP64 and do not need them).
+# instructions are disabled.
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 ARCH=${ECHO_ARCH:-gfx950}
-"$HIPCC" --offload-arch="$ARCH" -O3 -std=c++17 -fPIC -shared \
-  -Wno-unused-value -Wno-unused-result -ffp-contract=fast -Xclang -target-feature -Xclang -packed-fp32-ops \
-  -I"$HERE/../include" \
-  -o "$HERE/libechoamd.so" "$HERE/csrc/echo_capi.hip"
+OBJ="$HERE/_obj"
+mkdir -p "$OBJ"
+FLAGS=(--offload-arch="$ARCH" -O3 -std=c++17 -fPIC
+       -Wno-unused-value -Wno-unused-result -ffp-contract=fast
+       -Xclang -target-feature -Xclang -packed-fp32-ops -I"$HERE/../include")
+"$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/echo_capi.o" "$HERE/csrc/echo_capi.hip" &
+P1=$!
+"$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/c2p_mlp.o" "$HERE/csrc/c2p_mlp.hip" &
+P2=$!
+wait $P1
+wait $P2
+"$HIPCC" --offload-arch="$ARCH" -shared -fPIC -o "$HERE/libechoamd.so" "$OBJ/echo_capi.o" "$OBJ/c2p_mlp.o"
 echo "build: $HERE/libechoamd.so"
 "$HIPCC" --offload-arch="$ARCH" -O3 -o "$HERE/copy_calib" "$HERE/csrc/copy_calib.hip"

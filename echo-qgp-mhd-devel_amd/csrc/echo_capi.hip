@@ -10,57 +10,9 @@
 #include <vector>
 
 #include "../../include/echo_amd.h"
-#include "kernels.hpp"
+#include "grid.hpp"
 
 using namespace echo;
-
-static thread_local std::string g_err;
-
-static int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-#define HIPCHECK(x)                                                                     \
-  do {                                                                                  \
-    hipError_t e_ = (x);                                                                \
-    if (e_ != hipSuccess) return fail(ECHO_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
-
-struct echo_grid {
-  echo_config cfg;
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  int nv = 0;
-  long long N = 0;
-  long long nf[3] = {0, 0, 0};
-  int nb[3] = {0, 0, 0};
-  bool active[3] = {false, false, false};
-  double* V = nullptr;
-  double* U[2] = {nullptr, nullptr};
-  double* F[3] = {nullptr, nullptr, nullptr};
-  double* L[3] = {nullptr, nullptr, nullptr};
-  double* VS[2] = {nullptr, nullptr};
-  double* partial = nullptr;
-  double* tab = nullptr;
-  int* tabix = nullptr;
-  int n_tab = 0;
-  DevScalars* S = nullptr;
-  int* faillist = nullptr;
-  int* status = nullptr;
-  GridArgs ga;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  bool timing = false;
-  hipEvent_t ev[8] = {};
-  double tms[4] = {0, 0, 0, 0};
-  int next_stage = 1;
-};
 
 // RK2/RK3 Shu-Osher coefficients crk(1:2, irk, nrk) of evolve.f90:122-125
 static void rk_coeffs(int nrk, int irk, double& c1, double& c2) {
@@ -71,12 +23,6 @@ static void rk_coeffs(int nrk, int irk, double& c1, double& c2) {
   c2 = C[1][irk - 1][nrk - 1];
 }
 
-static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-// block count padded to a multiple of the XCD count (see xcd_block)
-static inline unsigned nblocks_xcd(long long n, int bs) {
-  unsigned nb = nblocks(n, bs);
-  return (nb + NXCD - 1) / NXCD * NXCD;
-}
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
@@ -128,21 +74,6 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   // writes the next step's vnewstep too, before k_end_step flips the parity
   k_fixup<SYS, ZZ><<<1, 64, 0, g->stream>>>(g->ga, g->V, Uout, g->faillist, g->VS[0],
                                              g->VS[1], last ? 1 : 0);
-}
-
-// the system of echo_config: hydro, viscous hydro ('no'/'zz'), ideal MHD
-// without / with GLM
-template <typename Fn>
-static void dispatch_sys(const echo_config& c, Fn&& fn) {
-  if (c.mhd) {
-    if (c.divclean) fn(std::integral_constant<int, SYS_MHDGLM>(), std::false_type());
-    else fn(std::integral_constant<int, SYS_MHD>(), std::false_type());
-  } else if (c.viscosity) {
-    if (c.obtained_zz) fn(std::integral_constant<int, SYS_VISC>(), std::true_type());
-    else fn(std::integral_constant<int, SYS_VISC>(), std::false_type());
-  } else {
-    fn(std::integral_constant<int, SYS_HD>(), std::false_type());
-  }
 }
 
 static void enqueue_stage(echo_grid* g, int irk) {
@@ -198,7 +129,7 @@ static void release(echo_grid* g) {
 
 extern "C" {
 
-const char* echo_last_error(void) { return g_err.c_str(); }
+const char* echo_last_error(void) { return echo::g_err.c_str(); }
 
 int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, int32_t device,
                 echo_grid** out) {

@@ -1,0 +1,93 @@
+// Device-side state of one grid (echo_grid) and host helpers shared by the
+// translation units of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+#include <type_traits>
+
+#include "../../include/echo_amd.h"
+#include "kernels.hpp"
+
+namespace echo {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(x)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(ECHO_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace echo
+
+struct echo_grid {
+  using Eos = echo::Eos;
+  using DevScalars = echo::DevScalars;
+  using GridArgs = echo::GridArgs;
+  echo_config cfg;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int nv = 0;
+  long long N = 0;
+  long long nf[3] = {0, 0, 0};
+  int nb[3] = {0, 0, 0};
+  bool active[3] = {false, false, false};
+  double* V = nullptr;
+  double* U[2] = {nullptr, nullptr};
+  double* F[3] = {nullptr, nullptr, nullptr};
+  double* L[3] = {nullptr, nullptr, nullptr};
+  double* VS[2] = {nullptr, nullptr};
+  double* partial = nullptr;
+  double* tab = nullptr;
+  int* tabix = nullptr;
+  int n_tab = 0;
+  DevScalars* S = nullptr;
+  int* faillist = nullptr;
+  int* status = nullptr;
+  GridArgs ga;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  double tms[4] = {0, 0, 0, 0};
+  int next_stage = 1;
+};
+
+
+namespace echo {
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// block count padded to a multiple of the XCD count (see xcd_block)
+inline unsigned nblocks_xcd(long long n, int bs) {
+  unsigned nb = nblocks(n, bs);
+  return (nb + NXCD - 1) / NXCD * NXCD;
+}
+
+// the system of echo_config: hydro, viscous hydro ('no'/'zz'), ideal MHD
+// without / with GLM
+template <typename Fn>
+inline void dispatch_sys(const echo_config& c, Fn&& fn) {
+  if (c.mhd) {
+    if (c.divclean) fn(std::integral_constant<int, SYS_MHDGLM>(), std::false_type());
+    else fn(std::integral_constant<int, SYS_MHD>(), std::false_type());
+  } else if (c.viscosity) {
+    if (c.obtained_zz) fn(std::integral_constant<int, SYS_VISC>(), std::true_type());
+    else fn(std::integral_constant<int, SYS_VISC>(), std::false_type());
+  } else {
+    fn(std::integral_constant<int, SYS_HD>(), std::false_type());
+  }
+}
+
+}  // namespace echo

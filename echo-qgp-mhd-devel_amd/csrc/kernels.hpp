@@ -225,7 +225,7 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
 // three sweeps, in this order); only the sweeps of active dimensions
 // (n > 1) wrote theirs.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* __restrict__ partial,
+static __global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* __restrict__ partial,
                                               int nb0, int nb1, int nb2) {
   __shared__ double red[3][16];
   double m[3] = {0., 0., 0.};
@@ -507,7 +507,7 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
 
 // end of step bookkeeping, echo.f90:211-247 (time, metric, dts interval,
 // vnewstep -> vold, tout cadence)
-__global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
+static __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (S->run_crashed) return;
   S->timeold = S->t;

@@ -19,6 +19,7 @@ EOS_DIR = os.path.join(PKG_DIR, "eos_data")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 # the exported symbols of include/echo_amd.h
 EXPORTS = (
@@ -26,7 +27,12 @@ EXPORTS = (
     "echo_upload_state", "echo_download_state", "echo_steps", "echo_rk_stage",
     "echo_end_step", "echo_get_scalars", "echo_set_tout", "echo_cons2prim_batch",
     "echo_set_timing", "echo_get_timing", "echo_last_error",
+    "echo_cons2prim_batch_dev", "echo_mlp_create", "echo_mlp_destroy", "echo_mlp_set_params",
+    "echo_mlp_get_params", "echo_mlp_forward", "echo_mlp_grad", "echo_mlp_train",
+    "echo_c2p_surrogate",
 )
+
+MLP_NPARAM = 9221
 
 REC_CODES = {"TVD2": 1, "CENO3": 2, "WENO3": 3, "WENO5": 4, "PPM4": 5, "MPE3": 6, "MPE5": 7}
 
@@ -54,6 +60,13 @@ class EchoConfig(C.Structure):
         rc = np.ascontiguousarray(rc, dtype=np.float64)
         assert ic.size == 16 and rc.size == 19
         return cls.from_buffer_copy(ic.tobytes() + rc.tobytes())
+
+
+class TrainConfig(C.Structure):
+    """Mirror of struct echo_mlp_train_cfg."""
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("lr_final", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float)]
 
 
 _lib = None
@@ -86,6 +99,16 @@ def load_library(path=LIB_PATH):
         "echo_set_timing": (C.c_int, [P, C.c_int32]),
         "echo_get_timing": (C.c_int, [P, _dp]),
         "echo_last_error": (C.c_char_p, []),
+        "echo_cons2prim_batch_dev": (C.c_int, [P, P, P, P, C.c_int64]),
+        "echo_mlp_create": (C.c_int, [C.c_int32, C.POINTER(P)]),
+        "echo_mlp_destroy": (None, [P]),
+        "echo_mlp_set_params": (C.c_int, [P, _fp, C.c_int64]),
+        "echo_mlp_get_params": (C.c_int, [P, _fp, C.c_int64]),
+        "echo_mlp_forward": (C.c_int, [P, P, P, C.c_int64, P]),
+        "echo_mlp_grad": (C.c_int, [P, P, P, P, C.c_int64, _fp, C.POINTER(C.c_double)]),
+        "echo_mlp_train": (C.c_int, [P, P, P, P, C.c_int64, C.POINTER(TrainConfig), P]),
+        "echo_c2p_surrogate": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
+                                         C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -185,10 +208,83 @@ class EchoGrid:
         self._check(self.lib.echo_cons2prim_batch(self.h, u.ravel(), v.ravel(), st, n))
         return v, st
 
+    def cons2prim_batch_dev(self, d_u, d_v, d_status, n):
+        """Plain solver on device pointers (ints): u, v [nv][n] FP64, status [n] int32."""
+        self._check(self.lib.echo_cons2prim_batch_dev(self.h, d_u, d_v, d_status, int(n)))
+
     def set_timing(self, on):
         self._check(self.lib.echo_set_timing(self.h, 1 if on else 0))
 
-    def timing(self):
+    def timing_raw(self):
         t = np.zeros(4)
         self._check(self.lib.echo_get_timing(self.h, t))
+        return t
+
+    def timing(self):
+        t = self.timing_raw()
         return dict(flux_x=t[0], flux_y=t[1], flux_z=t[2], update=t[3])
+
+
+class C2pNet:
+    """The learned con2prim network on one GPU (echo_mlp of the C-ABI).
+
+    Device arrays are passed as integer device pointers (e.g. a torch
+    tensor's data_ptr()); the library synchronises its own stream before
+    returning, and the caller must have finished writing its inputs."""
+
+    def __init__(self, device=0, params=None):
+        self.lib = load_library()
+        h = C.c_void_p()
+        self._check(self.lib.echo_mlp_create(int(device), C.byref(h)))
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise EchoError(rc, self.lib.echo_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.echo_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.echo_mlp_set_params(self.h, p, p.size))
+
+    def params(self):
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._check(self.lib.echo_mlp_get_params(self.h, p, p.size))
+        return p
+
+    def forward(self, grid, d_u, n, d_y):
+        self._check(self.lib.echo_mlp_forward(self.h, grid.h, d_u, int(n), d_y))
+
+    def grad(self, grid, d_u, d_v, n):
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.echo_mlp_grad(self.h, grid.h, d_u, d_v, int(n), g, C.byref(loss)))
+        return loss.value, g
+
+    def train(self, grid, d_u, d_v, n, steps, batch, lr=3e-3, lr_final=3e-5, beta1=0.9,
+              beta2=0.999, eps=1e-8):
+        cfg = TrainConfig(int(steps), int(batch), lr, lr_final, beta1, beta2, eps)
+        log = np.zeros(int(steps), dtype=np.float32)
+        self._check(self.lib.echo_mlp_train(self.h, grid.h, d_u, d_v, int(n), C.byref(cfg),
+                                            log.ctypes.data))
+        return log
+
+    def invert(self, grid, d_u, d_v, d_status, n, iters=2, tol=1e-11):
+        """The learned inversion; returns the number of fallback states."""
+        nf = C.c_int64()
+        self._check(self.lib.echo_c2p_surrogate(grid.h, self.h, d_u, d_v, d_status, int(n),
+                                                int(iters), float(tol), C.byref(nf)))
+        return nf.value

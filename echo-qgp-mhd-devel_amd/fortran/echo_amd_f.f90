@@ -1,0 +1,143 @@
+! Fortran binding of the C-ABI in include/echo_amd.h (iso_c_binding).
+!
+! A Fortran caller such as the reference's echo.f90 uses this module in place
+! of the per-stage calls into evolve.f90: echo_create replaces the set-up of
+! echo.f90:73-109, echo_upload_state hands over the initial v/u arrays
+! (common.f90:203-214, same column-major layout v(ix,iy,iz,k)), echo_steps
+! replaces the body of the main loop echo.f90:185-241 and
+! echo_download_state refreshes v/u on the output cadence (echo.f90:243-260).
+module echo_amd_f
+  use iso_c_binding
+  implicit none
+  private
+  public :: echo_config, echo_create, echo_destroy, echo_nv, echo_init_from_prim, &
+            echo_upload_state, echo_download_state, echo_steps, echo_rk_stage, echo_end_step, &
+            echo_get_scalars, echo_set_tout, echo_cons2prim_batch, echo_set_timing, &
+            echo_get_timing, echo_last_error, echo_error_message
+
+  ! struct echo_config: 16 int32 then 19 double (param.dat keys in brackets)
+  type, bind(C) :: echo_config
+    integer(c_int32_t) :: nx, ny, nz            ! NX, NY, NZ
+    integer(c_int32_t) :: coordinates           ! COORD
+    integer(c_int32_t) :: viscosity             ! VISCOUS
+    integer(c_int32_t) :: bulk                  ! BULK
+    integer(c_int32_t) :: mhd                   ! MHD
+    integer(c_int32_t) :: divclean              ! DIVCLEAN
+    integer(c_int32_t) :: eqstate               ! EOS
+    integer(c_int32_t) :: recal                 ! REC_ALGO (1 TVD2 .. 7 MPE5)
+    integer(c_int32_t) :: nrk                   ! INT_ALGO (2 RK2, 3 RK3)
+    integer(c_int32_t) :: boundary_cond         ! bound_con
+    integer(c_int32_t) :: algo_solver           ! ALG_SOLV
+    integer(c_int32_t) :: obtained_zz           ! TRACE_IMP (1 'zz', 0 'no')
+    integer(c_int32_t) :: init_type             ! INIT_TYPE
+    integer(c_int32_t) :: derivatives_out       ! derivativ
+    real(c_double) :: x1, x2, y1, y2, z1, z2    ! XMIN .. ZMAX
+    real(c_double) :: tmin                      ! TSTART
+    real(c_double) :: cfl                       ! CFL
+    real(c_double) :: maxdt                     ! MAXDT
+    real(c_double) :: maxspeed                  ! MAXSPEED
+    real(c_double) :: eta_over_s                ! ETA_S
+    real(c_double) :: tau_pi_coeff              ! TAU_PI_C
+    real(c_double) :: enezero                   ! ENEZERO
+    real(c_double) :: glm_alpha                 ! GLM_PARAM
+    real(c_double) :: sigma_el                  ! EL_COND
+    real(c_double) :: sigma_chi                 ! CHIR_COND
+    real(c_double) :: dtout                     ! DTOUT
+    real(c_double) :: smooth_temp               ! CUT_TEMP
+    real(c_double) :: tmax                      ! TSTOP
+  end type echo_config
+
+  interface
+    integer(c_int) function echo_create(cfg, eos_rows, n_rows, device, grid) bind(C, name="echo_create")
+      import :: c_int, c_int32_t, c_double, c_ptr, echo_config
+      type(echo_config), intent(in) :: cfg
+      real(c_double), intent(in) :: eos_rows(*)
+      integer(c_int32_t), value :: n_rows, device
+      type(c_ptr), intent(out) :: grid
+    end function
+    subroutine echo_destroy(grid) bind(C, name="echo_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: grid
+    end subroutine
+    integer(c_int) function echo_nv(grid) bind(C, name="echo_nv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: grid
+    end function
+    integer(c_int) function echo_init_from_prim(grid, v) bind(C, name="echo_init_from_prim")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(in) :: v(*)
+    end function
+    integer(c_int) function echo_upload_state(grid, v, u) bind(C, name="echo_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(in) :: v(*), u(*)
+    end function
+    integer(c_int) function echo_download_state(grid, v, u) bind(C, name="echo_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(out) :: v(*), u(*)
+    end function
+    integer(c_int) function echo_steps(grid, nsteps) bind(C, name="echo_steps")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function echo_rk_stage(grid, irk) bind(C, name="echo_rk_stage")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int32_t), value :: irk
+    end function
+    integer(c_int) function echo_end_step(grid) bind(C, name="echo_end_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: grid
+    end function
+    integer(c_int) function echo_get_scalars(grid, s) bind(C, name="echo_get_scalars")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(out) :: s(10)
+    end function
+    integer(c_int) function echo_set_tout(grid, tout) bind(C, name="echo_set_tout")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), value :: tout
+    end function
+    integer(c_int) function echo_cons2prim_batch(grid, u, v, status, n) bind(C, name="echo_cons2prim_batch")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(in) :: u(*)
+      real(c_double), intent(inout) :: v(*)
+      integer(c_int32_t), intent(out) :: status(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function echo_set_timing(grid, on) bind(C, name="echo_set_timing")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function echo_get_timing(grid, t) bind(C, name="echo_get_timing")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(out) :: t(4)
+    end function
+    type(c_ptr) function echo_last_error() bind(C, name="echo_last_error")
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! the message of the last failed call, as a Fortran string
+  function echo_error_message() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(echo_last_error(), s, [512])
+    do i = 1, 512
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function echo_error_message
+
+end module echo_amd_f

@@ -41,14 +41,17 @@ typedef struct echo_config {
   int32_t coordinates;         /* [COORD] 1 Minkowski, 2 Bjorken (Milne) */
   int32_t viscosity;           /* [VISCOUS] 0/1 */
   int32_t bulk;                /* [BULK] 0/1 */
-  int32_t mhd;                 /* [MHD] 0 (1/2 not supported yet: ECHO_E_ARG) */
+  int32_t mhd;                 /* [MHD] 0 hydro, 1 ideal MHD (2 resistive:
+                                  ECHO_E_ARG, non-functional in the reference) */
   int32_t divclean;            /* [DIVCLEAN] */
   int32_t eqstate;             /* [EQ_STATE] 1 p=e/3, 2 ideal gas, 3 table */
   int32_t recal;               /* [REC_ALGO] 1 TVD2 2 CENO3 3 WENO3 4 WENO5
                                   5 PPM4 6 MPE3 7 MPE5 */
   int32_t nrk;                 /* [INT_ALGO] 2 RK2, 3 RK3 */
   int32_t boundary_cond;       /* [BOUND_CON] 0 periodic, 2 outflow */
-  int32_t algo_solver;         /* [ALG_SOLV] (MHD only) */
+  int32_t algo_solver;         /* [ALG_SOLV] MHD con2prim: 4 ideal gas
+                                  (EQ_STATE 2), 5 p=e/3 (EQ_STATE 1), 6 p=e/3
+                                  Minkowski quartic */
   int32_t obtained_zz;         /* [TRACE_IMP] 0 'no', 1 'zz' */
   int32_t init_type;           /* [INIT_TYPE] (selects viscous tau_pi rules) */
   int32_t derivatives_out;     /* [DER_OUT] */
@@ -122,10 +125,59 @@ int echo_set_tout(echo_grid *g, double tout);
 int echo_cons2prim_batch(echo_grid *g, const double *u, double *v, int32_t *status,
                          int64_t n);
 
-/* Kernel timing of the last echo_steps call, in milliseconds per launch,
- * measured with HIP events on the grid's stream: t[0] flux x, t[1] flux y,
- * t[2] flux z, t[3] cell update.  Only filled when timing was enabled with
- * echo_set_timing(g, 1). */
+/* Batched con2prim on device-resident states (device pointers, same layout
+ * as echo_cons2prim_batch): the plain solver of the grid's system. */
+int echo_cons2prim_batch_dev(echo_grid *g, const double *d_u, double *d_v, int32_t *d_status,
+                             int64_t n);
+
+/* ---- learned con2prim (no counterpart in the reference) ----------------
+ * An 8 -> 64 -> 64 -> 64 -> 5 network (bfloat16 MFMA, FP32 accumulation)
+ * supplies the initial guess (v^2, W) of the ideal-MHD p = e/3 inversion
+ * that the reference does with algorithm 5 of system_cons2prim
+ * (system.f90:945-998, rtsafe + solver8); `iters` FP64 Newton iterations on
+ * the two equations of solver8 refine it; a state is accepted when the
+ * Newton correction at the final point is below tol relative to 1 - v^2 and
+ * to W, else the plain solver (from the guess in d_v) handles it.
+ * Parameters: ECHO_MLP_NPARAM floats, W1[64][8] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[5][64] b4[5] (row-major [out][in]). */
+#define ECHO_MLP_NPARAM 9221
+typedef struct echo_mlp echo_mlp;
+
+typedef struct echo_mlp_train_cfg {
+  int32_t steps;         /* optimiser steps */
+  int32_t batch;         /* states per step (consecutive slices of the data) */
+  float lr, lr_final;    /* Adam step size, cosine schedule lr -> lr_final */
+  float beta1, beta2, eps;
+} echo_mlp_train_cfg;
+
+int echo_mlp_create(int32_t device, echo_mlp **out);
+void echo_mlp_destroy(echo_mlp *m);
+int echo_mlp_set_params(echo_mlp *m, const float *p, int64_t n);
+int echo_mlp_get_params(echo_mlp *m, float *p, int64_t n);
+/* Network outputs y[5][n] (device) for n conserved states d_u[8][n]
+ * (Minkowski metric). */
+int echo_mlp_forward(echo_mlp *m, echo_grid *g, const double *d_u, int64_t n, float *d_y);
+/* Training loss and its gradient (host outputs) over n labelled states
+ * (d_u conserved, d_v primitives of the plain solver, [8][n], Minkowski). */
+int echo_mlp_grad(echo_mlp *m, echo_grid *g, const double *d_u, const double *d_v, int64_t n,
+                  float *grad, double *loss);
+/* Adam training on the device; loss_log (host, optional): cfg->steps
+ * per-step losses. */
+int echo_mlp_train(echo_mlp *m, echo_grid *g, const double *d_u, const double *d_v, int64_t n,
+                   const echo_mlp_train_cfg *cfg, float *loss_log);
+/* The learned inversion on n states at the grid's current metric (device
+ * pointers; d_v: guess in, primitives out).  The grid must be configured
+ * for ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1).
+ * *n_fallback: states handed to the plain solver. */
+int echo_c2p_surrogate(echo_grid *g, echo_mlp *m, const double *d_u, double *d_v,
+                       int32_t *d_status, int64_t n, int32_t iters, double tol,
+                       int64_t *n_fallback);
+
+/* Kernel timing in milliseconds, measured with HIP events on the grid's
+ * stream when enabled with echo_set_timing(g, 1).  After echo_steps (per
+ * launch): t[0] flux x, t[1] flux y, t[2] flux z, t[3] cell update.  After
+ * echo_cons2prim_batch_dev: t[0] the solver kernel.  After
+ * echo_c2p_surrogate: t[0] network + refinement kernel, t[1] fallback. */
 int echo_set_timing(echo_grid *g, int32_t on);
 int echo_get_timing(echo_grid *g, double *t);
 

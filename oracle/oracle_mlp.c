@@ -1,0 +1,303 @@
+/*
+ * ORACLE - test infrastructure, not product code.
+ *
+ * CPU restatement of the learned con2prim (the MLP initial guess, its
+ * training gradients and the FP64 Newton refinement with its residual gate)
+ * that the HIP library implements.  The reference has no such component; it
+ * is pinned by mathematics instead:
+ *   - the backward pass by central finite differences of the FP64 forward
+ *     (orc_mlp_loss with round = 0) -- tests/test_mlp_oracle.py;
+ *   - the refinement by the plain solver of the reference (algorithm 5 of
+ *     system_cons2prim, system.f90:945-998, restated in oracle_sys.c and
+ *     bit-exact against the reference): both must reach the same root;
+ *   - the residual functions by prim2cons round trips (a state built from
+ *     primitives has zero residual at its own (v^2, W)).
+ *
+ * Network: 8 -> 64 -> 64 -> 64 -> 5, activation gelu(x) = x * phi(clamp(x,
+ * -3, 3)) with phi an odd degree-9 polynomial fit of the normal CDF (max
+ * error 2.6e-4 on [-3, 3], monotone).  With round = 1 the forward pass
+ * rounds the network input and every hidden activation to bfloat16 (round
+ * to nearest even) and uses bfloat16-rounded weights, as the MFMA kernel
+ * does; sums are accumulated in FP64 here and in FP32 on the GPU.
+ *
+ * Parameter block (float, MLP_NPARAM = 9221), row-major W[out][in]:
+ *   W1[64][8] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[5][64] b4[5]
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "oracle_mlp.h"
+
+#define NI MLP_NIN
+#define NH MLP_NH
+#define NO MLP_NOUT
+
+static const int OFF_W[4] = {0, 576, 4736, 8896};
+static const int OFF_B[4] = {512, 4672, 8832, 9216};
+static const int DIN[4] = {NI, NH, NH, NH};
+static const int DOUT[4] = {NH, NH, NH, NO};
+
+/* loss weights of the five heads */
+static const double LAMBDA[NO] = {1.0, 1.0, 0.25, 0.25, 0.25};
+
+/* phi(t) = 0.5 + t*(C1 + t^2*(C3 + t^2*(C5 + t^2*(C7 + t^2*C9)))) */
+#define GC1 0.39794397f
+#define GC3 -0.063981285f
+#define GC5 0.0081565485f
+#define GC7 -0.00060929223f
+#define GC9 1.9460935e-05f
+
+float orc_bf16(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+/* float evaluation, explicit fma, the same sequence as the HIP kernel */
+float orc_gelu(float x) {
+  float t = fminf(fmaxf(x, -3.0f), 3.0f);
+  float t2 = t * t;
+  float p = fmaf(t2, GC9, GC7);
+  p = fmaf(t2, p, GC5);
+  p = fmaf(t2, p, GC3);
+  p = fmaf(t2, p, GC1);
+  float phi = fmaf(t, p, 0.5f);
+  return x * phi;
+}
+
+/* d gelu / dx (the clamp makes phi constant outside [-3, 3]) */
+float orc_gelu_d(float x) {
+  float t = fminf(fmaxf(x, -3.0f), 3.0f);
+  float t2 = t * t;
+  float p = fmaf(t2, GC9, GC7);
+  p = fmaf(t2, p, GC5);
+  p = fmaf(t2, p, GC3);
+  p = fmaf(t2, p, GC1);
+  float phi = fmaf(t, p, 0.5f);
+  float dp = fmaf(t2, 9.0f * GC9, 7.0f * GC7);
+  dp = fmaf(t2, dp, 5.0f * GC5);
+  dp = fmaf(t2, dp, 3.0f * GC3);
+  dp = fmaf(t2, dp, GC1);
+  return (x > -3.0f && x < 3.0f) ? phi + x * dp : phi;
+}
+
+static double gelu64(double x) {
+  double t = fmin(fmax(x, -3.0), 3.0), t2 = t * t;
+  double p = (((GC9 * t2 + GC7) * t2 + GC5) * t2 + GC3) * t2 + GC1;
+  return x * (0.5 + t * p);
+}
+
+static double gelu64_d(double x) {
+  double t = fmin(fmax(x, -3.0), 3.0), t2 = t * t;
+  double p = (((GC9 * t2 + GC7) * t2 + GC5) * t2 + GC3) * t2 + GC1;
+  double dp = (((9.0 * GC9 * t2 + 7.0 * GC7) * t2 + 5.0 * GC5) * t2 + 3.0 * GC3) * t2 + GC1;
+  return (x > -3.0 && x < 3.0) ? 0.5 + t * p + x * dp : 0.5 + t * p;
+}
+
+/* Invariants of a de-densitised MHD conserved state u = (D, S_i, U, B^i)
+ * (the quantities system_cons2prim forms at system.f90:886-893). */
+static void invariants(const double *u, double g33, double *S2, double *B2, double *SB) {
+  *S2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3] / g33;
+  *B2 = u[5] * u[5] + u[6] * u[6] + u[7] * u[7] * g33;
+  *SB = u[1] * u[5] + u[2] * u[6] + u[3] * u[7];
+}
+
+/* Network input: 8 scale-free features of u (u already divided by sqrt(-g)).
+ * With s2 = S^2/U^2, b2 = B^2/U, q = (S.B)^2/(S^2 B^2), d = D/U. */
+void orc_mlp_features(const double *u, double g33, float *x) {
+  double S2, B2, SB;
+  invariants(u, g33, &S2, &B2, &SB);
+  const double U = u[4];
+  const double s2 = S2 / (U * U), b2 = B2 / U;
+  const double den = S2 * B2;
+  const double q = den > 0.0 ? (SB * SB) / den : 0.0;
+  const double d = u[0] / U;
+  const double oms = fmax(1.0 - s2, 1e-30);
+  x[0] = (float)(log(oms) / 3.0 + 1.0);
+  x[1] = (float)(2.0 * s2 - 1.0);
+  x[2] = (float)(log(fmax(b2, 1e-30)) / 4.0 + 1.0);
+  x[3] = (float)(2.0 * b2 / (1.0 + b2) - 1.0);
+  x[4] = (float)(2.0 * q - 1.0);
+  x[5] = (float)(log(fmax(d, 1e-30)) / 8.0);
+  x[6] = (float)(2.0 * sqrt(q) - 1.0);
+  x[7] = (float)(2.0 * oms / (oms + b2) - 1.0);
+}
+
+/* Training targets (normalised) of a labelled pair (u, v), p = e/3:
+ * t0 = z/3 - 1 with z = -log(1 - v^2); t1 = 2 log(W/U) with W = 4 p gamma^2;
+ * t2 = (log(p/U) + 4)/3; t3 = 2 U/(W + B^2) - 1; t4 = 2 v^2 - 1. */
+void orc_mlp_targets(const double *u, const double *v, double g33, double *t) {
+  double S2, B2, SB;
+  invariants(u, g33, &S2, &B2, &SB);
+  const double U = u[4];
+  const double v2 = v[1] * v[1] + v[2] * v[2] + v[3] * v[3] * g33;
+  const double z = -log(1.0 - v2);
+  const double W = 4.0 * v[4] / (1.0 - v2);
+  t[0] = z / 3.0 - 1.0;
+  t[1] = 2.0 * log(W / U);
+  t[2] = (log(v[4] / U) + 4.0) / 3.0;
+  t[3] = 2.0 * U / (W + B2) - 1.0;
+  t[4] = 2.0 * v2 - 1.0;
+}
+
+/* Forward pass of one state.  round = 1: bfloat16 input, weights and hidden
+ * activations (float activation function), outputs rounded to float, as on
+ * the GPU; round = 0: FP64 throughout.  zs (optional): the 3x64 hidden
+ * pre-activations; hs (optional): the layer inputs h0 (8) then h1..h3 (64
+ * each), as used by the next layer. */
+static void forward(const float *P, const float *x, int round, double *y, double *zs, double *hs) {
+  double h[NH], z[NH];
+  int n_in = NI;
+  for (int i = 0; i < NI; i++) h[i] = round ? orc_bf16(x[i]) : x[i];
+  if (hs) for (int i = 0; i < NI; i++) hs[i] = h[i];
+  for (int l = 0; l < 4; l++) {
+    const float *W = P + OFF_W[l], *b = P + OFF_B[l];
+    for (int o = 0; o < DOUT[l]; o++) {
+      double acc = b[o];
+      for (int i = 0; i < n_in; i++) {
+        const double w = round ? orc_bf16(W[o * n_in + i]) : W[o * n_in + i];
+        acc += w * h[i];
+      }
+      z[o] = acc;
+    }
+    if (l == 3) {
+      for (int o = 0; o < NO; o++) y[o] = round ? (double)(float)z[o] : z[o];
+      return;
+    }
+    for (int o = 0; o < NH; o++) {
+      if (zs) zs[l * NH + o] = z[o];
+      h[o] = round ? orc_bf16(orc_gelu((float)z[o])) : gelu64(z[o]);
+      if (hs) hs[NI + l * NH + o] = h[o];
+    }
+    n_in = NH;
+  }
+}
+
+void orc_mlp_forward(const float *P, const float *x, int round, float *y) {
+  double yy[NO];
+  forward(P, x, round, yy, 0, 0);
+  for (int o = 0; o < NO; o++) y[o] = (float)yy[o];
+}
+
+/* Backward pass of one state: dz holds dL/dy on entry; adds the parameter
+ * gradient to grad.  round = 1 uses the bfloat16 weights of the forward pass
+ * and the float activation derivative (the bfloat16 roundings are passed
+ * through as the identity). */
+static void backward(const float *P, int round, const double *zs, const double *hs, double *dz,
+                     double *grad) {
+  double dh[NH];
+  for (int l = 3; l >= 0; l--) {
+    const float *W = P + OFF_W[l];
+    const int ni = DIN[l], no = DOUT[l];
+    const double *hin = (l == 0) ? hs : hs + NI + (l - 1) * NH;
+    double *gW = grad + OFF_W[l], *gb = grad + OFF_B[l];
+    for (int o = 0; o < no; o++) {
+      gb[o] += dz[o];
+      for (int i = 0; i < ni; i++) gW[o * ni + i] += dz[o] * hin[i];
+    }
+    if (l == 0) return;
+    for (int i = 0; i < ni; i++) {
+      double acc = 0.0;
+      for (int o = 0; o < no; o++)
+        acc += (round ? (double)orc_bf16(W[o * ni + i]) : (double)W[o * ni + i]) * dz[o];
+      dh[i] = acc;
+    }
+    for (int i = 0; i < ni; i++) {
+      const double zz = zs[(l - 1) * NH + i];
+      dz[i] = dh[i] * (round ? (double)orc_gelu_d((float)zz) : gelu64_d(zz));
+    }
+  }
+}
+
+/* Mean weighted squared error over n labelled states (u, v: 8 x n,
+ * variable-major, u de-densitised) and, if grad != NULL, its gradient with
+ * respect to the parameter block (FP64 accumulation). */
+double orc_mlp_loss(const float *P, const double *u, const double *v, long n, double g33,
+                    int round, double *grad) {
+  double loss = 0.0;
+  if (grad) memset(grad, 0, sizeof(double) * MLP_NPARAM);
+  for (long s = 0; s < n; s++) {
+    double us[8], vs[8], t[NO], y[NO], dz[NH], zs[3 * NH], hs[NI + 3 * NH];
+    float x[NI];
+    for (int k = 0; k < 8; k++) { us[k] = u[k * n + s]; vs[k] = v[k * n + s]; }
+    orc_mlp_features(us, g33, x);
+    orc_mlp_targets(us, vs, g33, t);
+    forward(P, x, round, y, zs, hs);
+    for (int o = 0; o < NO; o++) {
+      const double r = y[o] - t[o];
+      loss += LAMBDA[o] * r * r;
+      dz[o] = 2.0 * LAMBDA[o] * r / (double)n;
+    }
+    if (grad) backward(P, round, zs, hs, dz, grad);
+  }
+  return loss / (double)n;
+}
+
+/* ---------------------------------------------------------------------- */
+/* FP64 refinement of a guess (x = v^2, W = (e+p) gamma^2) for ideal MHD with
+ * p = e/3: two-variable Newton on the pair of equations that algorithm 5 of
+ * the reference solves nested (solver8, system.f90:2152-2201: the energy
+ * cubic in W at fixed v^2, and the momentum equation for v^2):
+ *   R1 = c3 W^3 + c2 W^2 + c0,  c3 = (3 + x)/4, c2 = B^2 (1 + x)/2 - U,
+ *        c0 = -(S.B)^2/2
+ *   R2 = (W + B^2)^2 x - (2W + B^2) (S.B)^2 / W^2 - S^2.                    */
+static void mhd_e3p_residual(double x, double W, double S2, double B2, double SB2, double U,
+                             double *R1, double *R2, double *J) {
+  const double c3 = 0.25 * (3.0 + x);
+  const double c2 = 0.5 * B2 * (1.0 + x) - U;
+  const double iw = 1.0 / W;
+  const double wb = W + B2;
+  const double sbw2 = SB2 * iw * iw;
+  *R1 = (c3 * W + c2) * W * W - 0.5 * SB2;
+  *R2 = wb * wb * x - (2.0 * W + B2) * sbw2 - S2;
+  J[0] = W * W * (0.25 * W + 0.5 * B2);
+  J[1] = W * (3.0 * c3 * W + 2.0 * c2);
+  J[2] = wb * wb;
+  J[3] = 2.0 * wb * x - 2.0 * sbw2 + 2.0 * (2.0 * W + B2) * sbw2 * iw;
+}
+
+/* From network outputs y (normalised) to primitives.  Returns 0 when the
+ * state passes the gate (after `iters` Newton iterations the Newton
+ * correction at the final point is below tol relative to 1 - v^2 and to W),
+ * 1 when it must fall back to the plain solver.  u: 8 conserved values
+ * multiplied by sqrt(-g) (as stored on the grid); gm = 1/sqrt(-g). */
+int orc_mlp_refine(const double *u_in, const float *y, double g33, double gm, int iters,
+                   double tol, double *v) {
+  double u[8];
+  for (int k = 0; k < 8; k++) u[k] = gm * u_in[k];
+  double S2, B2, SB;
+  invariants(u, g33, &S2, &B2, &SB);
+  const double SB2 = SB * SB, U = u[4];
+  const double z = 3.0 * ((double)y[0] + 1.0);
+  double x = 1.0 - exp(-fmax(z, 0.0));
+  double W = U * exp(0.5 * (double)y[1]);
+  double R1, R2, J[4];
+  for (int it = 0; it < iters; it++) {
+    mhd_e3p_residual(x, W, S2, B2, SB2, U, &R1, &R2, J);
+    const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
+    x -= (R1 * J[3] - R2 * J[1]) * idet;
+    W -= (J[0] * R2 - J[2] * R1) * idet;
+  }
+  mhd_e3p_residual(x, W, S2, B2, SB2, U, &R1, &R2, J);
+  const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
+  const double dx = (R1 * J[3] - R2 * J[1]) * idet;
+  const double dW = (J[0] * R2 - J[2] * R1) * idet;
+  const int ok = (x >= 0.0) && (x < 1.0) && (W > 0.0) && (fabs(dx) <= tol * (1.0 - x)) &&
+                 (fabs(dW) <= tol * W);
+  if (!ok) return 1;
+  /* primitives as at the end of algorithm 5 (system.f90:990-998) */
+  const double vb = SB / W;
+  v[0] = u[0] * sqrt(1.0 - x);
+  v[4] = 0.25 * (1.0 - x) * W;
+  v[1] = (u[1] + vb * u[5]) / (W + B2);
+  v[2] = (u[2] + vb * u[6]) / (W + B2);
+  v[3] = (u[3] / g33 + vb * u[7]) / (W + B2);
+  v[5] = u[5];
+  v[6] = u[6];
+  v[7] = u[7];
+  return 0;
+}

@@ -9,6 +9,7 @@ Both expose the same methods so a test can run one scenario on each.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -22,27 +23,12 @@ EOS_TABLE = os.path.join(PKG, "eos_data", "qcdIEOS0.dat")
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
-REC_CODES = {"TVD2": 1, "CENO3": 2, "WENO3": 3, "WENO5": 4, "PPM4": 5,
-             "MPE3": 6, "MPE5": 7, "MPE7": 8}
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+from echoamd.initial import (HBAR, REC_CODES, cell_centres, gubser_prim,  # noqa: E402,F401
+                             make_config)
+
 REC_NGC = {1: 2, 2: 3, 3: 2, 4: 3, 5: 2, 6: 3, 7: 3, 8: 4}
-
-
-def make_config(nx=32, ny=32, nz=1, coordinates=2, viscosity=0, bulk=0, mhd=0,
-                divclean=0, eqstate=1, rec="WENO5", nrk=3, bc=2, algo_solver=3,
-                obtained_zz=1, init_type=4, derivatives_out=0,
-                xlim=(-3.2, 3.2), ylim=(-3.2, 3.2), zlim=(-0.5, 0.5), tmin=1.0,
-                cfl=0.4, maxdt=1.0, maxspeed=0.995, eta_over_s=0.08,
-                tau_pi_coeff=5.0, enezero=0.001, glm_alpha=10.0, sigma_el=0.0058,
-                sigma_chi=0.0015, dtout=1.0e6, smooth_temp=0.0, tmax=1.0e6):
-    """Integer / real configuration arrays, laid out as oracle/ref_harness.f90."""
-    ic = np.array([nx, ny, nz, coordinates, viscosity, bulk, mhd, divclean, eqstate,
-                   REC_CODES[rec] if isinstance(rec, str) else rec, nrk, bc,
-                   algo_solver, obtained_zz, init_type, derivatives_out],
-                  dtype=np.int32)
-    rc = np.array([xlim[0], xlim[1], ylim[0], ylim[1], zlim[0], zlim[1], tmin, cfl,
-                   maxdt, maxspeed, eta_over_s, tau_pi_coeff, enezero, glm_alpha,
-                   sigma_el, sigma_chi, dtout, smooth_temp, tmax], dtype=np.float64)
-    return ic, rc
 
 
 class _Lib:
@@ -215,37 +201,78 @@ class Ref(_Lib):
             os.chdir(cwd)
 
 
+class OracleMLP:
+    """ctypes wrapper of the learned-con2prim oracle (oracle/oracle_mlp.c)."""
+
+    NPARAM = 9221
+
+    def __init__(self):
+        lib = C.CDLL(ORACLE_SO)
+        fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        lib.orc_bf16.restype = C.c_float
+        lib.orc_bf16.argtypes = [C.c_float]
+        lib.orc_gelu.restype = C.c_float
+        lib.orc_gelu.argtypes = [C.c_float]
+        lib.orc_gelu_d.restype = C.c_float
+        lib.orc_gelu_d.argtypes = [C.c_float]
+        lib.orc_mlp_features.restype = None
+        lib.orc_mlp_features.argtypes = [dp, C.c_double, fp]
+        lib.orc_mlp_targets.restype = None
+        lib.orc_mlp_targets.argtypes = [dp, dp, C.c_double, dp]
+        lib.orc_mlp_forward.restype = None
+        lib.orc_mlp_forward.argtypes = [fp, fp, C.c_int, fp]
+        lib.orc_mlp_loss.restype = C.c_double
+        lib.orc_mlp_loss.argtypes = [fp, dp, dp, C.c_long, C.c_double, C.c_int, C.c_void_p]
+        lib.orc_mlp_refine.restype = C.c_int
+        lib.orc_mlp_refine.argtypes = [dp, fp, C.c_double, C.c_double, C.c_int, C.c_double, dp]
+        self.lib = lib
+
+    def bf16(self, x):
+        return self.lib.orc_bf16(x)
+
+    def gelu(self, x):
+        return self.lib.orc_gelu(x)
+
+    def gelu_d(self, x):
+        return self.lib.orc_gelu_d(x)
+
+    def features(self, u, g33=1.0):
+        x = np.zeros(8, dtype=np.float32)
+        self.lib.orc_mlp_features(np.ascontiguousarray(u, dtype=np.float64), g33, x)
+        return x
+
+    def targets(self, u, v, g33=1.0):
+        t = np.zeros(5)
+        self.lib.orc_mlp_targets(np.ascontiguousarray(u, dtype=np.float64),
+                                 np.ascontiguousarray(v, dtype=np.float64), g33, t)
+        return t
+
+    def forward(self, P, x, rnd=1):
+        y = np.zeros(5, dtype=np.float32)
+        self.lib.orc_mlp_forward(np.ascontiguousarray(P, dtype=np.float32),
+                                 np.ascontiguousarray(x, dtype=np.float32), rnd, y)
+        return y
+
+    def loss(self, P, u, v, rnd=1, grad=False, g33=1.0):
+        """u, v: (8, n) (u de-densitised).  Returns loss or (loss, grad)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        g = np.zeros(self.NPARAM) if grad else None
+        L = self.lib.orc_mlp_loss(P, u, v, u.shape[1], g33, rnd,
+                                  g.ctypes.data if grad else None)
+        return (L, g) if grad else L
+
+    def refine(self, u, y, g33=1.0, gm=1.0, iters=2, tol=1e-11):
+        v = np.zeros(8)
+        st = self.lib.orc_mlp_refine(np.ascontiguousarray(u, dtype=np.float64),
+                                     np.ascontiguousarray(y, dtype=np.float32), g33, gm,
+                                     int(iters), float(tol), v)
+        return st, v
+
+
 def ref_available():
     return os.path.exists(REF_SO)
 
 
-# ---------------------------------------------------------------------------
-# initial conditions (restated from init.f90, evaluated here in numpy)
-# ---------------------------------------------------------------------------
-HBAR = 0.197326  # common.f90:47
-
-
-def cell_centres(n, lo, hi):
-    d = (hi - lo) / n
-    return lo + (np.arange(1, n + 1) - 0.5) * d
-
-
-def gubser_prim(nx, ny, xlim, ylim, tau, that0=2.0, q=1.0, nv=5, nz=1):
-    """Ideal Gubser flow primitives (rho=1, v^x, v^y, 0, p=e/3) at proper
-    time tau, following init.f90:2926-2935 with T0_idGub -> that0 (fm^-1)."""
-    x = cell_centres(nx, *xlim)
-    y = cell_centres(ny, *ylim)
-    X, Y = np.meshgrid(x, y, indexing="xy")  # shape (ny, nx)
-    r = np.sqrt(X ** 2 + Y ** 2)
-    temp = (HBAR * that0 / tau * (4 * q ** 2 * tau ** 2) ** (1.0 / 3.0)
-            / (1.0 + 2.0 * q ** 2 * (tau ** 2 + r ** 2) + q ** 4 * (tau ** 2 - r ** 2) ** 2) ** (1.0 / 3.0))
-    e = 47.5 * np.pi ** 2 / (30.0 * HBAR ** 3) * temp ** 4
-    k = np.arctanh(2 * q ** 2 * tau * r / (1.0 + (q * tau) ** 2 + (q * r) ** 2))
-    vx = X / r * np.tanh(k)
-    vy = Y / r * np.tanh(k)
-    v = np.zeros((nv, nz, ny, nx))
-    v[0] = 1.0
-    v[1] = vx[None]
-    v[2] = vy[None]
-    v[4] = (e / 3.0)[None]
-    return v

@@ -71,14 +71,30 @@ def shock2d(n=48, rec="WENO5", nrk=3, eqstate=2):
     return ic, rc, v
 
 
+def _groups(nv):
+    """Variables measured on a common scale: the three components of the
+    velocity (momentum) vector, and for MHD the field components together
+    with the GLM scalar (same units)."""
+    groups = [[0], [1, 2, 3], [4]]
+    if nv in (8, 9):
+        groups.append(list(range(5, nv)))
+    else:
+        groups += [[k] for k in range(5, nv)]
+    return groups
+
+
 def rel_linf(a, b):
-    """max over variables of max|a-b| / max|b| (variables with max|b| = 0 use
-    the absolute difference)."""
+    """max over variables of max|a-b| / scale, where the scale of a variable is
+    max|b| over its group (see _groups); an all-zero group uses the absolute
+    difference.  Vector components share the scale of their vector, so a
+    component that is zero up to rounding (e.g. v^z of a 2-D run) is measured
+    against the vector, not against its own rounding noise."""
     out = 0.0
-    for k in range(b.shape[0]):
-        scale = np.abs(b[k]).max()
-        d = np.abs(a[k] - b[k]).max()
-        out = max(out, d / scale if scale > 0 else d)
+    for grp in _groups(b.shape[0]):
+        scale = max(np.abs(b[k]).max() for k in grp)
+        for k in grp:
+            d = np.abs(a[k] - b[k]).max()
+            out = max(out, d / scale if scale > 0 else d)
     return out
 
 

@@ -1,0 +1,198 @@
+"""The learned-con2prim oracle (oracle/oracle_mlp.c) pinned by mathematics.
+
+The reference has no learned inversion, so these checks fix each part of the
+restatement independently of it:
+- backward pass: central finite differences of the FP64 forward pass;
+- Newton refinement + primitive formulas: convergence to the root that the
+  reference's own algorithm 5 (restated in oracle_sys.c, bit-exact against
+  the reference Fortran) reaches, and to the primitives the state was built
+  from;
+- residual gate: rejects an unconverged guess, accepts a converged one;
+- features: invariance under rotations and under the scaling
+  (D, S, U, B) -> (l D, l S, l U, sqrt(l) B) that maps solutions of the
+  p = e/3 inversion onto each other;
+- the host-side data generator's prim2cons against the oracle's.
+"""
+import numpy as np
+import pytest
+
+from libs import OracleMLP, gubser_prim, make_config
+
+
+@pytest.fixture(scope="module")
+def om(oracle):
+    return OracleMLP()
+
+
+def _snap():
+    v = gubser_prim(24, 24, (-2.4, 2.4), (-2.4, 2.4), 1.0)
+    return np.stack([v[0, 0], v[1, 0], v[2, 0], 0 * v[3, 0], v[4, 0]]).reshape(5, -1)
+
+
+def _states(n, seed=3):
+    from echoamd.surrogate import mhd_e3p_states
+    return mhd_e3p_states(n, _snap(), seed=seed)
+
+
+def _scales(v):
+    """Per-component error scales: rho and p their own size, velocity and
+    field components the largest component of their vector."""
+    return np.array([abs(v[0])] + [np.abs(v[1:4]).max()] * 3 + [abs(v[4])] +
+                    [np.abs(v[5:8]).max()] * 3)
+
+
+def _mhd_oracle(oracle, coords=1, t=1.0):
+    ic, rc = make_config(mhd=1, divclean=0, algo_solver=5, eqstate=1, coordinates=coords)
+    oracle.setup(ic, rc)
+    oracle.set_time(t)
+
+
+def test_bf16_rounding(om):
+    # round to nearest even on the 16 dropped bits
+    assert om.bf16(1.0) == 1.0
+    assert om.bf16(1.0 + 2 ** -8) == 1.0                 # tie -> even
+    assert om.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # tie -> even (up)
+    assert om.bf16(1.0 + 2 ** -8 + 2 ** -20) == 1.0 + 2 ** -7
+    assert om.bf16(-3.0) == -3.0
+
+
+def test_gelu_is_close_to_exact_gelu(om):
+    from scipy.special import ndtr
+    x = np.linspace(-6, 6, 241)
+    g = np.array([om.gelu(float(a)) for a in x])
+    exact = x * ndtr(x)
+    inside = np.abs(x) <= 3
+    assert np.max(np.abs(g - exact)[inside] / np.maximum(np.abs(x[inside]), 1.0)) < 3e-4
+    # derivative against central differences (away from the clamp points)
+    for a in np.linspace(-2.9, 2.9, 37):
+        h = 1e-3
+        fd = (om.gelu(a + h) - om.gelu(a - h)) / (2 * h)
+        assert abs(om.gelu_d(a) - fd) < 2e-3, a
+
+
+def test_backward_matches_finite_differences(om):
+    from echoamd.surrogate import init_params
+    u, vt, _ = _states(12)
+    P = init_params(5)
+    rng = np.random.default_rng(0)
+    P[512:576] = rng.uniform(-0.2, 0.2, 64)     # non-zero biases
+    P[4672:4736] = rng.uniform(-0.2, 0.2, 64)
+    L, g = om.loss(P, u, vt, rnd=0, grad=True)
+    assert np.isfinite(L) and L > 0
+    idx = np.concatenate([rng.choice(np.arange(0, 512), 8, replace=False),
+                          rng.choice(np.arange(512, 576), 4, replace=False),
+                          rng.choice(np.arange(576, 4672), 8, replace=False),
+                          rng.choice(np.arange(4736, 8832), 8, replace=False),
+                          rng.choice(np.arange(8896, 9221), 8, replace=False)])
+    h = 2.0 ** -12
+    for i in idx:
+        Pp = P.copy(); Pp[i] += h
+        Pm = P.copy(); Pm[i] -= h
+        fd = (om.loss(Pp, u, vt, rnd=0) - om.loss(Pm, u, vt, rnd=0)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g).max()) + 1e-4 * abs(g[i]), (i, fd, g[i])
+
+
+def test_bf16_gradient_close_to_fp64_gradient(om):
+    from echoamd.surrogate import init_params
+    u, vt, _ = _states(64)
+    P = init_params(6)
+    L0, g0 = om.loss(P, u, vt, rnd=0, grad=True)
+    L1, g1 = om.loss(P, u, vt, rnd=1, grad=True)
+    assert abs(L1 - L0) < 2e-2 * L0
+    assert np.linalg.norm(g1 - g0) < 5e-2 * np.linalg.norm(g0)
+
+
+def test_targets_encode_the_solution(om):
+    u, vt, _ = _states(50)
+    for s in range(50):
+        t = om.targets(u[:, s], vt[:, s])
+        v2 = (vt[1:4, s] ** 2).sum()
+        assert abs(1 - np.exp(-3 * (t[0] + 1)) - v2) < 1e-13
+        W = 4 * vt[4, s] / (1 - v2)
+        assert abs(u[4, s] * np.exp(0.5 * t[1]) / W - 1) < 1e-13
+        assert abs(t[4] - (2 * v2 - 1)) < 1e-15
+
+
+@pytest.mark.parametrize("coords,t", [(1, 1.0), (2, 1.7)])
+def test_refine_reaches_the_reference_root(oracle, om, coords, t):
+    """From a guess 1e-4 off in (v^2, W), two Newton iterations reach the
+    primitives the state was built from to 1e-11 relative, on a Minkowski and
+    a Bjorken metric; the reference's algorithm 5 reaches the same root only
+    to its own tolerance (v^2 to about 1e-12 absolute, so p to ~1e-12 gamma^2
+    relative, up to ~1e-9 at gamma = 20)."""
+    _mhd_oracle(oracle, coords, t)
+    u, vt, vw = _states(200, seed=11)
+    g33 = t * t if coords == 2 else 1.0
+    gp = t if coords == 2 else 1.0
+    rng = np.random.default_rng(1)
+    nacc = 0
+    for s in range(200):
+        vp = vt[:, s].copy()
+        if coords == 2:
+            vp[3] /= t          # contravariant v^eta
+            vp[7] /= t          # contravariant B^eta
+        e, us, _ = oracle.prim2cons(vp)
+        assert e == 0
+        tg = om.targets(us / gp, vp, g33)
+        y = (tg[:5] * (1 + 1e-4 * rng.standard_normal(5))).astype(np.float32)
+        st, vr = om.refine(us, y, g33=g33, gm=1.0 / gp, iters=2, tol=1e-11)
+        guess = vw[:, s].copy()
+        if coords == 2:
+            guess[3] /= t; guess[7] /= t
+        e, vo, _ = oracle.cons2prim(us, guess)
+        assert e == 0
+        if st == 0:
+            nacc += 1
+            sc = _scales(vp)
+            assert np.all(np.abs(vr - vp) <= 1e-11 * sc), (s, vr, vp)
+            assert np.all(np.abs(vo - vp) <= 2e-9 * sc), (s, vo, vp)
+    assert nacc >= 190
+
+
+def test_gate_rejects_unconverged_and_accepts_converged(om):
+    u, vt, _ = _states(100, seed=12)
+    rej = acc = 0
+    for s in range(100):
+        tg = om.targets(u[:, s], vt[:, s])
+        y = (tg + np.array([3e-3, 3e-3, 0, 0, 0])).astype(np.float32)
+        st0, _ = om.refine(u[:, s], y, iters=0, tol=1e-11)
+        st3, _ = om.refine(u[:, s], y, iters=4, tol=1e-11)
+        rej += st0 == 1
+        acc += st3 == 0
+    assert rej == 100 and acc >= 98
+
+
+def test_features_invariances(om):
+    u, _, _ = _states(40, seed=13)
+    rng = np.random.default_rng(4)
+    for s in range(40):
+        x0 = om.features(u[:, s])
+        lam = np.exp(rng.uniform(-3, 3))
+        us = u[:, s].copy()
+        us[:5] *= lam
+        us[5:8] *= np.sqrt(lam)
+        assert np.allclose(om.features(us), x0, rtol=0, atol=2e-6)
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        ur = u[:, s].copy()
+        ur[1:4] = q @ ur[1:4]
+        ur[5:8] = q @ ur[5:8]
+        assert np.allclose(om.features(ur), x0, rtol=0, atol=2e-6)
+
+
+def test_generator_prim2cons_matches_oracle(oracle):
+    _mhd_oracle(oracle)
+    u, vt, _ = _states(100, seed=14)
+    for s in range(100):
+        e, uo, _ = oracle.prim2cons(vt[:, s].copy())
+        assert e == 0
+        assert np.allclose(u[:, s], uo, rtol=1e-14, atol=1e-14 * np.abs(uo).max())
+
+
+def test_generator_ranges():
+    from echoamd.surrogate import mhd_e3p_states
+    u, vt, vw = mhd_e3p_states(20000, _snap(), seed=2)
+    gam = 1 / np.sqrt(1 - (vt[1:4] ** 2).sum(0))
+    sig = (vt[5:8] ** 2).sum(0) / (4 * vt[4])
+    assert gam.min() >= 1.0 and 18 < gam.max() < 22.5
+    assert 0.7e-4 < sig.min() < 2e-4 and 0.7 < sig.max() < 1.5
+    assert np.all(np.isfinite(u))
