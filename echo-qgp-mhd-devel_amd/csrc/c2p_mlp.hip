@@ -319,4 +319,37 @@ int echo_c2p_surrogate(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v
   return 0;
 }
 
+// ---- device memory for callers without a runtime of their own ------------
+int echo_dev_alloc(int32_t device, int64_t bytes, void** ptr) {
+  if (!ptr || bytes < 0) return fail(ECHO_E_ARG, "invalid argument");
+  *ptr = nullptr;
+  HIPCHECK(hipSetDevice(device));
+  if (bytes == 0) return 0;
+  if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+    *ptr = nullptr;
+    return fail(ECHO_E_NOMEM, "hipMalloc of %lld bytes failed", (long long)bytes);
+  }
+  return 0;
+}
+
+int echo_dev_free(int32_t device, void* ptr) {
+  HIPCHECK(hipSetDevice(device));
+  if (ptr) HIPCHECK(hipFree(ptr));
+  return 0;
+}
+
+int echo_dev_upload(int32_t device, void* dst, const void* src, int64_t bytes) {
+  if ((!dst || !src) && bytes > 0) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(device));
+  if (bytes > 0) HIPCHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int echo_dev_download(int32_t device, void* dst, const void* src, int64_t bytes) {
+  if ((!dst || !src) && bytes > 0) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(device));
+  if (bytes > 0) HIPCHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 }  // extern "C"

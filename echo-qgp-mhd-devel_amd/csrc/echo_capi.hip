@@ -7,6 +7,7 @@
 #include <string.h>
 #include <string>
 #include <type_traits>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/echo_amd.h"
@@ -27,21 +28,47 @@ static void rk_coeffs(int nrk, int irk, double& c1, double& c2) {
 // ---------------------------------------------------------------------------
 // kernel dispatch
 // ---------------------------------------------------------------------------
-template <int DIR, int SYS, bool ZZ>
-static void launch_flux_dir(echo_grid* g, double* partial) {
+// flux launch of one sweep with the grid's tile shape; the tiled variants
+// are instantiated for the WENO5 and PPM4 reconstructions
+template <int DIR, int REC, int SYS, bool ZZ>
+static void launch_flux_rec(echo_grid* g, double* partial) {
   dim3 grid(g->nb[DIR - 1]), block(FLUX_BLOCK);
   double* F = g->F[DIR - 1];
   double* L = g->L[DIR - 1];
+  if constexpr (DIR > 1 && (REC == REC_WENO5 || REC == REC_PPM4)) {
+    if (g->tile[DIR - 1] == 2) {
+      k_flux<DIR, REC, SYS, ZZ, 2><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial);
+      return;
+    }
+    if (g->tile[DIR - 1] == 4) {
+      k_flux<DIR, REC, SYS, ZZ, 4><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial);
+      return;
+    }
+  }
+  k_flux<DIR, REC, SYS, ZZ, 1><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial);
+}
+
+template <int DIR, int SYS, bool ZZ>
+static void launch_flux_dir(echo_grid* g, double* partial) {
   switch (g->cfg.recal) {
-    case REC_TVD2: k_flux<DIR, REC_TVD2, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_CENO3: k_flux<DIR, REC_CENO3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_WENO3: k_flux<DIR, REC_WENO3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_PPM4: k_flux<DIR, REC_PPM4, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_MPE3: k_flux<DIR, REC_MPE3, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    case REC_MPE5: k_flux<DIR, REC_MPE5, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
-    default: k_flux<DIR, REC_WENO5, SYS, ZZ><<<grid, block, 0, g->stream>>>(g->ga, g->V, F, L, partial); break;
+    case REC_TVD2: launch_flux_rec<DIR, REC_TVD2, SYS, ZZ>(g, partial); break;
+    case REC_CENO3: launch_flux_rec<DIR, REC_CENO3, SYS, ZZ>(g, partial); break;
+    case REC_WENO3: launch_flux_rec<DIR, REC_WENO3, SYS, ZZ>(g, partial); break;
+    case REC_PPM4: launch_flux_rec<DIR, REC_PPM4, SYS, ZZ>(g, partial); break;
+    case REC_MPE3: launch_flux_rec<DIR, REC_MPE3, SYS, ZZ>(g, partial); break;
+    case REC_MPE5: launch_flux_rec<DIR, REC_MPE5, SYS, ZZ>(g, partial); break;
+    default: launch_flux_rec<DIR, REC_WENO5, SYS, ZZ>(g, partial); break;
   }
 }
+
+// launch blocks of a sweep for tile shape ty (padded for xcd_block)
+static unsigned flux_blocks(const echo_config& c, int d, int ty) {
+  long long t = d == 0 ? flux_tiles<1>(c.nx, c.ny, c.nz, 1)
+              : d == 1 ? flux_tiles<2>(c.nx, c.ny, c.nz, ty) : flux_tiles<3>(c.nx, c.ny, c.nz, ty);
+  return (unsigned)((t + NXCD - 1) / NXCD * NXCD);
+}
+
+static bool tiles_supported(const echo_config& c) { return c.recal == REC_WENO5 || c.recal == REC_PPM4; }
 
 template <int SYS, bool ZZ>
 static void enqueue_stage_t(echo_grid* g, int irk) {
@@ -78,6 +105,64 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
 
 static void enqueue_stage(echo_grid* g, int irk) {
   dispatch_sys(g->cfg, [&](auto sys, auto zz) { enqueue_stage_t<decltype(sys)::value, decltype(zz)::value>(g, irk); });
+}
+
+// one flux sweep (direction d = 0, 1, 2) with the current tile shape
+static void enqueue_flux(echo_grid* g, int d) {
+  double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
+  dispatch_sys(g->cfg, [&](auto sys, auto zz) {
+    constexpr int S = decltype(sys)::value;
+    constexpr bool Z = decltype(zz)::value;
+    if (d == 0) launch_flux_dir<1, S, Z>(g, part[0]);
+    else if (d == 1) launch_flux_dir<2, S, Z>(g, part[1]);
+    else launch_flux_dir<3, S, Z>(g, part[2]);
+  });
+}
+
+static void drop_graph(echo_grid* g) {
+  if (g->gexec) hipGraphExecDestroy(g->gexec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+}
+
+static int set_tile(echo_grid* g, int d, int ty) {
+  if (d < 0 || d > 2) return fail(ECHO_E_ARG, "sweep index must be 0, 1 or 2");
+  if (ty != 1 && ty != 2 && ty != 4) return fail(ECHO_E_ARG, "tile rows must be 1, 2 or 4");
+  if (ty != 1 && (d == 0 || !tiles_supported(g->cfg)))
+    return fail(ECHO_E_ARG, "tiled sweeps exist for y and z with WENO5 or PPM4");
+  g->tile[d] = ty;
+  g->nb[d] = (int)flux_blocks(g->cfg, d, ty);
+  drop_graph(g);
+  if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
+// autotuner: times each tile shape of the y and z sweeps on the current
+// state (reps launches each, HIP events on the grid's stream) and keeps the
+// fastest.  The sweeps only write the face buffers, so the state is unchanged.
+static int tune_grid(echo_grid* g, int reps, double* ms_out) {
+  for (int d = 1; d < 3; d++) {
+    if (!g->active[d] || !tiles_supported(g->cfg)) continue;
+    double best = 1e300;
+    int best_ty = 1;
+    for (int ty : {1, 2, 4}) {
+      if (int e = set_tile(g, d, ty)) return e;
+      enqueue_flux(g, d);   // warm-up
+      HIPCHECK(hipEventRecord(g->ev[6], g->stream));
+      for (int r = 0; r < reps; r++) enqueue_flux(g, d);
+      HIPCHECK(hipEventRecord(g->ev[7], g->stream));
+      HIPCHECK(hipEventSynchronize(g->ev[7]));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, g->ev[6], g->ev[7]));
+      const double per = ms / reps;
+      if (ms_out) ms_out[3 * d + (ty == 1 ? 0 : ty == 2 ? 1 : 2)] = per;
+      if (per < best) { best = per; best_ty = ty; }
+    }
+    if (int e = set_tile(g, d, best_ty)) return e;
+  }
+  HIPCHECK(hipGetLastError());
+  return 0;
 }
 
 static void enqueue_end_step(echo_grid* g) {
@@ -119,11 +204,14 @@ static void release(echo_grid* g) {
   if (g->gexec) hipGraphExecDestroy(g->gexec);
   if (g->graph) hipGraphDestroy(g->graph);
   hipFree(g->V); hipFree(g->U[0]); hipFree(g->U[1]);
-  for (int d = 0; d < 3; d++) { hipFree(g->F[d]); hipFree(g->L[d]); }
-  hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->partial); hipFree(g->tab); hipFree(g->tabix);
+  if (g->owns_scratch) {
+    for (int d = 0; d < 3; d++) { hipFree(g->F[d]); hipFree(g->L[d]); }
+    hipFree(g->partial);
+  }
+  hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->tab); hipFree(g->tabix);
   hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
   for (auto& e : g->ev) if (e) hipEventDestroy(e);
-  if (g->stream) hipStreamDestroy(g->stream);
+  if (g->stream && g->owns_stream) hipStreamDestroy(g->stream);
   delete g;
 }
 
@@ -131,8 +219,13 @@ extern "C" {
 
 const char* echo_last_error(void) { return echo::g_err.c_str(); }
 
-int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, int32_t device,
-                echo_grid** out) {
+}  // extern "C"
+
+// Set-up of one grid.  stream: launch stream to use (not owned) or null for
+// a stream of its own; share_scratch: leave F, L and partial unallocated
+// (an ensemble points them at its shared pool).
+static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, int32_t device,
+                       hipStream_t stream, bool share_scratch, echo_grid** out) {
   if (!cfg || !out) return fail(ECHO_E_ARG, "echo_create: null argument");
   *out = nullptr;
   const echo_config& c = *cfg;
@@ -166,7 +259,13 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
   g->nf[2] = (long long)c.nx * c.ny * (c.nz + 1);
   g->active[0] = c.nx > 1; g->active[1] = c.ny > 1; g->active[2] = c.nz > 1;
-  for (int d = 0; d < 3; d++) g->nb[d] = (int)nblocks_xcd(g->nf[d], FLUX_BLOCK);
+  long long nb_cap = 1;
+  for (int d = 0; d < 3; d++) {
+    g->nb[d] = (int)flux_blocks(c, d, 1);
+    unsigned mx = 0;
+    for (int ty : {1, 2, 4}) mx = std::max(mx, flux_blocks(c, d, ty));
+    nb_cap += mx;
+  }
   const size_t szs = sizeof(double) * (size_t)g->nv * g->N;
 #define ALLOC(p, bytes)                                                           \
   do {                                                                            \
@@ -175,10 +274,12 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
       return fail(ECHO_E_NOMEM, "hipMalloc of %zu bytes failed", (size_t)(bytes)); \
     }                                                                             \
   } while (0)
+  g->owns_scratch = !share_scratch;
+  g->partial_cap = nb_cap;
   ALLOC(g->V, szs);
   ALLOC(g->U[0], szs);
   ALLOC(g->U[1], szs);
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < 3 && !share_scratch; d++) {
     if (!g->active[d]) continue;
     ALLOC(g->F[d], sizeof(double) * (size_t)g->nv * g->nf[d]);
     if (c.viscosity) ALLOC(g->L[d], sizeof(double) * 4 * (size_t)g->nf[d]);
@@ -187,11 +288,14 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
     ALLOC(g->VS[0], sizeof(double) * 3 * (size_t)g->N);
     ALLOC(g->VS[1], sizeof(double) * 3 * (size_t)g->N);
   }
-  ALLOC(g->partial, sizeof(double) * (size_t)(g->nb[0] + g->nb[1] + g->nb[2] + 1));
+  if (!share_scratch) ALLOC(g->partial, sizeof(double) * (size_t)nb_cap);
   ALLOC(g->S, sizeof(DevScalars));
   ALLOC(g->faillist, sizeof(int) * FAIL_CAP);
   ALLOC(g->status, sizeof(int));
-  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (stream) {
+    g->stream = stream;
+    g->owns_stream = false;
+  } else if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
     release(g);
     return fail(ECHO_E_HIP, "stream creation failed");
   }
@@ -299,6 +403,46 @@ int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, 
   for (auto& e : g->ev) hipEventCreate(&e);
   *out = g;
   return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// ensemble of independent event grids on one device: one stream, one shared
+// pool for the face buffers (F, L, partial; sized for the largest event),
+// and one captured graph that advances every event by one time step
+// ---------------------------------------------------------------------------
+struct echo_ensemble {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<echo_grid*> g;
+  double* F[3] = {nullptr, nullptr, nullptr};
+  double* L[3] = {nullptr, nullptr, nullptr};
+  double* partial = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true;
+  hipEvent_t ev[2] = {};
+  double last_ms = 0.0;
+};
+
+static void release_ensemble(echo_ensemble* e) {
+  if (!e) return;
+  hipSetDevice(e->dev);
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  for (echo_grid* g : e->g) release(g);
+  for (int d = 0; d < 3; d++) { hipFree(e->F[d]); hipFree(e->L[d]); }
+  hipFree(e->partial);
+  for (auto& v : e->ev) if (v) hipEventDestroy(v);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" {
+
+int echo_create(const echo_config* cfg, const double* eos_rows, int32_t n_rows, int32_t device,
+                echo_grid** out) {
+  return grid_create(cfg, eos_rows, n_rows, device, nullptr, false, out);
 }
 
 void echo_destroy(echo_grid* g) { release(g); }
@@ -464,6 +608,27 @@ int echo_cons2prim_batch(echo_grid* g, const double* u, double* v, int32_t* stat
   return 0;
 }
 
+int echo_set_tile(echo_grid* g, int32_t sweep, int32_t rows) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_tile inside a step");
+  return set_tile(g, sweep, rows);
+}
+
+int echo_get_tile(const echo_grid* g, int32_t* rows) {
+  if (!g || !rows) return fail(ECHO_E_ARG, "null argument");
+  for (int d = 0; d < 3; d++) rows[d] = g->tile[d];
+  return 0;
+}
+
+int echo_tune(echo_grid* g, int32_t reps, double* ms) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_tune inside a step");
+  if (reps < 1) return fail(ECHO_E_ARG, "reps must be >= 1");
+  HIPCHECK(hipSetDevice(g->dev));
+  if (ms) for (int i = 0; i < 9; i++) ms[i] = 0.0;
+  return tune_grid(g, reps, ms);
+}
+
 int echo_set_timing(echo_grid* g, int32_t on) {
   if (!g) return fail(ECHO_E_ARG, "null argument");
   g->timing = on != 0;
@@ -473,6 +638,120 @@ int echo_set_timing(echo_grid* g, int32_t on) {
 int echo_get_timing(echo_grid* g, double* t) {
   if (!g || !t) return fail(ECHO_E_ARG, "null argument");
   for (int i = 0; i < 4; i++) t[i] = g->tms[i];
+  return 0;
+}
+
+int echo_ensemble_create(const echo_config* cfgs, int32_t n_events, const double* eos_rows,
+                         int32_t n_rows, int32_t device, echo_ensemble** out) {
+  if (!cfgs || !out || n_events < 1) return fail(ECHO_E_ARG, "echo_ensemble_create: invalid argument");
+  *out = nullptr;
+  HIPCHECK(hipSetDevice(device));
+  echo_ensemble* e = new echo_ensemble;
+  e->dev = device;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(ECHO_E_HIP, "stream creation failed");
+  }
+  size_t szF[3] = {0, 0, 0}, szL[3] = {0, 0, 0};
+  long long szP = 0;
+  for (int i = 0; i < n_events; i++) {
+    echo_grid* g = nullptr;
+    int rc = grid_create(&cfgs[i], eos_rows, n_rows, device, e->stream, true, &g);
+    if (rc) {
+      release_ensemble(e);
+      std::string msg = g_err;
+      return fail(rc, "event %d: %s", i, msg.c_str());
+    }
+    g->graph_dirty = &e->dirty;
+    e->g.push_back(g);
+    for (int d = 0; d < 3; d++) {
+      if (!g->active[d]) continue;
+      szF[d] = std::max(szF[d], (size_t)((long long)g->nv * g->nf[d]));
+      if (g->cfg.viscosity) szL[d] = std::max(szL[d], (size_t)(4LL * g->nf[d]));
+    }
+    szP = std::max(szP, g->partial_cap);
+  }
+  for (int d = 0; d < 3; d++) {
+    if (szF[d] && hipMalloc(&e->F[d], sizeof(double) * szF[d]) != hipSuccess) {
+      release_ensemble(e);
+      return fail(ECHO_E_NOMEM, "face buffer allocation failed");
+    }
+    if (szL[d] && hipMalloc(&e->L[d], sizeof(double) * szL[d]) != hipSuccess) {
+      release_ensemble(e);
+      return fail(ECHO_E_NOMEM, "face buffer allocation failed");
+    }
+  }
+  if (hipMalloc(&e->partial, sizeof(double) * szP) != hipSuccess) {
+    release_ensemble(e);
+    return fail(ECHO_E_NOMEM, "partial buffer allocation failed");
+  }
+  for (echo_grid* g : e->g) {
+    for (int d = 0; d < 3; d++) {
+      g->F[d] = g->active[d] ? e->F[d] : nullptr;
+      g->L[d] = (g->active[d] && g->cfg.viscosity) ? e->L[d] : nullptr;
+    }
+    g->partial = e->partial;
+  }
+  for (auto& v : e->ev) hipEventCreate(&v);
+  *out = e;
+  return 0;
+}
+
+void echo_ensemble_destroy(echo_ensemble* e) { release_ensemble(e); }
+
+int echo_ensemble_size(const echo_ensemble* e) { return e ? (int)e->g.size() : 0; }
+
+echo_grid* echo_ensemble_grid(echo_ensemble* e, int32_t i) {
+  if (!e || i < 0 || i >= (int)e->g.size()) {
+    fail(ECHO_E_ARG, "event index out of range");
+    return nullptr;
+  }
+  return e->g[i];
+}
+
+int echo_ensemble_tune(echo_ensemble* e, int32_t reps) {
+  if (!e || reps < 1) return fail(ECHO_E_ARG, "invalid argument");
+  HIPCHECK(hipSetDevice(e->dev));
+  for (echo_grid* g : e->g)
+    if (int rc = tune_grid(g, reps, nullptr)) return rc;
+  e->dirty = true;
+  return 0;
+}
+
+int echo_ensemble_steps(echo_ensemble* e, int32_t nsteps, int32_t* n_crashed) {
+  if (!e || nsteps < 0) return fail(ECHO_E_ARG, "invalid argument");
+  HIPCHECK(hipSetDevice(e->dev));
+  for (echo_grid* g : e->g)
+    if (g->next_stage != 1) return fail(ECHO_E_ARG, "an event is inside a step");
+  if (e->dirty || !e->gexec) {
+    if (e->gexec) hipGraphExecDestroy(e->gexec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (echo_grid* g : e->g) enqueue_step(g);
+    hipGraph_t graph;
+    HIPCHECK(hipStreamEndCapture(e->stream, &graph));
+    e->graph = graph;
+    HIPCHECK(hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0));
+    e->dirty = false;
+  }
+  HIPCHECK(hipEventRecord(e->ev[0], e->stream));
+  for (int s = 0; s < nsteps; s++) HIPCHECK(hipGraphLaunch(e->gexec, e->stream));
+  HIPCHECK(hipEventRecord(e->ev[1], e->stream));
+  HIPCHECK(hipEventSynchronize(e->ev[1]));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+  e->last_ms = ms;
+  int nc = 0;
+  for (echo_grid* g : e->g) nc += read_crash(g) != 0;
+  if (n_crashed) *n_crashed = nc;
+  return 0;
+}
+
+int echo_ensemble_last_ms(const echo_ensemble* e, double* ms) {
+  if (!e || !ms) return fail(ECHO_E_ARG, "null argument");
+  *ms = e->last_ms;
   return 0;
 }
 

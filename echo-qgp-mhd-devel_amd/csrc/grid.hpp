@@ -63,6 +63,11 @@ struct echo_grid {
   hipEvent_t ev[8] = {};
   double tms[4] = {0, 0, 0, 0};
   int next_stage = 1;
+  int tile[3] = {1, 1, 1};    // flux tile shape (TY) per sweep
+  bool owns_scratch = true;   // F, L, partial (shared inside an ensemble)
+  bool owns_stream = true;
+  long long partial_cap = 0;  // doubles in partial
+  bool* graph_dirty = nullptr; // set when the launch shape changes (ensemble graph)
 };
 
 

@@ -70,19 +70,54 @@ __device__ __forceinline__ int bc_map(int c0, int m, int bc) {
 // system.f90:1316-1583) and writes it, plus a per-block max of the fast
 // speeds for the CFL condition (evolve.f90:723-764).
 // ---------------------------------------------------------------------------
-template <int DIR, int REC, int SYS, bool ZZ>
+//
+// Tile shape (TY): with TY = 1 a block takes FLUX_BLOCK consecutive faces in
+// the face order (x fastest).  For the y and z sweeps TY > 1 makes the block
+// a tile of FLUX_BLOCK/TY faces along x by TY faces along the sweep, so the
+// six-cell stencils of the block's faces overlap in TY + 5 cell rows instead
+// of 6 * TY; the autotuner picks TY per sweep and grid.
+template <int DIR>
+__host__ __device__ __forceinline__ long long flux_tiles(int nx, int ny, int nz, int ty) {
+  const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
+                     : (DIR == 2 ? (long long)nx * (ny + 1) * nz : (long long)nx * ny * (nz + 1));
+  if (ty == 1 || DIR == 1) return (nf + FLUX_BLOCK - 1) / FLUX_BLOCK;
+  const int tx = FLUX_BLOCK / ty;
+  const long long ntx = (nx + tx - 1) / tx;
+  const long long nti = ((DIR == 2 ? ny : nz) + 1 + ty - 1) / ty;
+  return ntx * nti * (DIR == 2 ? nz : ny);
+}
+
+template <int DIR, int REC, int SYS, bool ZZ, int TY = 1>
 __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
                                                      double* __restrict__ partial) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
+  static_assert(TY == 1 || DIR != 1, "x sweeps use TY = 1");
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int m = DIR == 1 ? nx : (DIR == 2 ? ny : nz);
   const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
                      : (DIR == 2 ? (long long)nx * (ny + 1) * nz : (long long)nx * ny * (nz + 1));
-  const long long f = xcd_block(blockIdx.x, gridDim.x) * FLUX_BLOCK + threadIdx.x;
+  const long long tile = xcd_block(blockIdx.x, gridDim.x);
+  long long f;
+  bool inside;
+  if constexpr (TY == 1) {
+    f = tile * FLUX_BLOCK + threadIdx.x;
+    inside = f < nf;
+  } else {
+    constexpr int TX = FLUX_BLOCK / TY;
+    const long long ntx = (nx + TX - 1) / TX;
+    const long long nti = (m + 1 + TY - 1) / TY;
+    const long long r = tile / ntx;
+    const int x = (int)(tile % ntx) * TX + (int)threadIdx.x % TX;
+    const int i = (int)(r % nti) * TY + (int)threadIdx.x / TX;
+    const long long rest = r / nti;   // z (y sweep) or y (z sweep)
+    inside = x < nx && i <= m && rest < (DIR == 2 ? nz : ny);
+    f = DIR == 2 ? x + (long long)nx * (i + (long long)(ny + 1) * rest)
+                 : x + (long long)nx * (rest + (long long)ny * i);
+  }
   double amax = 0.;
-  if (f < nf) {
+  if (inside) {
     int x, y, z, i;
     long long base, stride;
     if constexpr (DIR == 1) {

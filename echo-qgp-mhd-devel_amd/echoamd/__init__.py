@@ -29,7 +29,10 @@ EXPORTS = (
     "echo_set_timing", "echo_get_timing", "echo_last_error",
     "echo_cons2prim_batch_dev", "echo_mlp_create", "echo_mlp_destroy", "echo_mlp_set_params",
     "echo_mlp_get_params", "echo_mlp_forward", "echo_mlp_grad", "echo_mlp_train",
-    "echo_c2p_surrogate",
+    "echo_c2p_surrogate", "echo_set_tile", "echo_get_tile", "echo_tune",
+    "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
+    "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
+    "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
 )
 
 MLP_NPARAM = 9221
@@ -109,6 +112,21 @@ def load_library(path=LIB_PATH):
         "echo_mlp_train": (C.c_int, [P, P, P, P, C.c_int64, C.POINTER(TrainConfig), P]),
         "echo_c2p_surrogate": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
                                          C.POINTER(C.c_int64)]),
+        "echo_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
+        "echo_get_tile": (C.c_int, [P, _ip]),
+        "echo_tune": (C.c_int, [P, C.c_int32, _dp]),
+        "echo_ensemble_create": (C.c_int, [C.POINTER(EchoConfig), C.c_int32, _dp, C.c_int32,
+                                           C.c_int32, C.POINTER(P)]),
+        "echo_ensemble_destroy": (None, [P]),
+        "echo_ensemble_size": (C.c_int, [P]),
+        "echo_ensemble_grid": (P, [P, C.c_int32]),
+        "echo_ensemble_tune": (C.c_int, [P, C.c_int32]),
+        "echo_ensemble_steps": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32)]),
+        "echo_ensemble_last_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "echo_dev_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
+        "echo_dev_free": (C.c_int, [C.c_int32, P]),
+        "echo_dev_upload": (C.c_int, [C.c_int32, P, P, C.c_int64]),
+        "echo_dev_download": (C.c_int, [C.c_int32, P, P, C.c_int64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -127,11 +145,73 @@ def read_eos_rows(name="qcdIEOS0.dat"):
     return np.ascontiguousarray(rows, dtype=np.float64)
 
 
+def _check(lib, rc):
+    if rc != 0:
+        raise EchoError(rc, lib.echo_last_error().decode())
+    return rc
+
+
+class DeviceArray:
+    """A device buffer allocated through the library (echo_dev_alloc), for
+    the entry points that take device pointers.  ``ptr`` is the address."""
+
+    def __init__(self, shape, dtype=np.float64, device=0):
+        self.lib = load_library()
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(self.lib, self.lib.echo_dev_alloc(device, self.nbytes, C.byref(p)))
+        self._p = p
+
+    @property
+    def ptr(self):
+        return self._p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, device)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _check(self.lib, self.lib.echo_dev_upload(self.device, self._p, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.lib, self.lib.echo_dev_download(self.device, out.ctypes.data, self._p, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "_p", None) is not None and self._p.value:
+            self.lib.echo_dev_free(self.device, self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class EchoGrid:
     """One event's grid on one GPU (echo_grid of the C-ABI)."""
 
-    def __init__(self, ic, rc, device=0, eos_rows=None):
+    def __init__(self, ic, rc, device=0, eos_rows=None, _handle=None):
         self.lib = load_library()
+        if _handle is not None:
+            # a grid owned by an ensemble
+            self.cfg = EchoConfig.from_arrays(ic, rc)
+            self.nx, self.ny, self.nz = self.cfg.nx, self.cfg.ny, self.cfg.nz
+            self.h = C.c_void_p(_handle)
+            self._owned = False
+            self.nv = self.lib.echo_nv(self.h)
+            return
+        self._owned = True
         self.cfg = EchoConfig.from_arrays(ic, rc)
         self.nx, self.ny, self.nz = self.cfg.nx, self.cfg.ny, self.cfg.nz
         if self.cfg.eqstate == 3 and eos_rows is None:
@@ -154,7 +234,8 @@ class EchoGrid:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.echo_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.echo_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -162,6 +243,20 @@ class EchoGrid:
             self.close()
         except Exception:
             pass
+
+    def set_tile(self, sweep, rows):
+        self._check(self.lib.echo_set_tile(self.h, int(sweep), int(rows)))
+
+    def tile(self):
+        t = np.zeros(3, dtype=np.int32)
+        self._check(self.lib.echo_get_tile(self.h, t))
+        return t
+
+    def tune(self, reps=5):
+        """Autotune the flux tile shapes; returns the 3x3 per-launch times (ms)."""
+        ms = np.zeros(9)
+        self._check(self.lib.echo_tune(self.h, int(reps), ms))
+        return ms.reshape(3, 3)
 
     def init_from_prim(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
@@ -228,9 +323,8 @@ class EchoGrid:
 class C2pNet:
     """The learned con2prim network on one GPU (echo_mlp of the C-ABI).
 
-    Device arrays are passed as integer device pointers (e.g. a torch
-    tensor's data_ptr()); the library synchronises its own stream before
-    returning, and the caller must have finished writing its inputs."""
+    Device arrays are passed as integer device pointers (DeviceArray.ptr);
+    every call returns after its device work has finished."""
 
     def __init__(self, device=0, params=None):
         self.lib = load_library()
@@ -288,3 +382,53 @@ class C2pNet:
         self._check(self.lib.echo_c2p_surrogate(grid.h, self.h, d_u, d_v, d_status, int(n),
                                                 int(iters), float(tol), C.byref(nf)))
         return nf.value
+
+
+class Ensemble:
+    """Independent event grids advanced together (echo_ensemble)."""
+
+    def __init__(self, configs, device=0, eos_rows=None):
+        """configs: list of (ic, rc) arrays."""
+        self.lib = load_library()
+        cfgs = (EchoConfig * len(configs))(*[EchoConfig.from_arrays(ic, rc) for ic, rc in configs])
+        need_tab = any(c.eqstate == 3 for c in cfgs)
+        if need_tab and eos_rows is None:
+            eos_rows = read_eos_rows()
+        rows = np.zeros((1, 4)) if eos_rows is None else np.ascontiguousarray(eos_rows, dtype=np.float64)
+        n_rows = 0 if eos_rows is None else rows.shape[0]
+        h = C.c_void_p()
+        _check(self.lib, self.lib.echo_ensemble_create(cfgs, len(configs), rows.ravel(), n_rows,
+                                                       device, C.byref(h)))
+        self.h = h
+        self.grids = [EchoGrid(ic, rc, _handle=self.lib.echo_ensemble_grid(self.h, i))
+                      for i, (ic, rc) in enumerate(configs)]
+
+    def __len__(self):
+        return len(self.grids)
+
+    def tune(self, reps=3):
+        _check(self.lib, self.lib.echo_ensemble_tune(self.h, int(reps)))
+
+    def steps(self, n):
+        """Advance every event by n steps; returns the number of crashed events."""
+        nc = C.c_int32()
+        _check(self.lib, self.lib.echo_ensemble_steps(self.h, int(n), C.byref(nc)))
+        return nc.value
+
+    def last_ms(self):
+        ms = C.c_double()
+        _check(self.lib, self.lib.echo_ensemble_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for g in self.grids:
+                g.h = None
+            self.lib.echo_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -158,13 +158,12 @@ def e3p_grid(device=0, coordinates=1, tmin=1.0, divclean=0):
 def plain_labels(grid, u, guess):
     """Plain-solver primitives of host states u (8, n) from the guess; returns
     (v, status) on the host."""
-    import torch
-    du = torch.from_numpy(np.ascontiguousarray(u)).cuda()
-    dv = torch.from_numpy(np.ascontiguousarray(guess)).cuda()
-    ds = torch.zeros(u.shape[1], dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
-    grid.cons2prim_batch_dev(du.data_ptr(), dv.data_ptr(), ds.data_ptr(), u.shape[1])
-    return dv.cpu().numpy(), ds.cpu().numpy()
+    from . import DeviceArray
+    du = DeviceArray.from_numpy(u)
+    dv = DeviceArray.from_numpy(guess)
+    ds = DeviceArray((u.shape[1],), np.int32)
+    grid.cons2prim_batch_dev(du.ptr, dv.ptr, ds.ptr, u.shape[1])
+    return dv.numpy(), ds.numpy()
 
 
 def train(device=0, n_train=1 << 20, steps=20000, batch=8192, lr=3e-3, lr_final=3e-5, seed=1,
@@ -174,8 +173,7 @@ def train(device=0, n_train=1 << 20, steps=20000, batch=8192, lr=3e-3, lr_final=
     Returns (params, loss_log).  The training states come from
     mhd_e3p_states around Gubser snapshots; their labels are the primitives
     the plain FP64 solver (algorithm 5, warm-started) returns."""
-    import torch
-    from . import C2pNet
+    from . import C2pNet, DeviceArray
     if snapshots is None:
         g1 = gubser_grid(device)
         snapshots = gubser_snapshots(g1)
@@ -188,11 +186,10 @@ def train(device=0, n_train=1 << 20, steps=20000, batch=8192, lr=3e-3, lr_final=
     u = np.ascontiguousarray(u[:, ok])
     v = np.ascontiguousarray(v[:, ok])
     n = u.shape[1]
-    du = torch.from_numpy(u).cuda()
-    dv = torch.from_numpy(v).cuda()
-    torch.cuda.synchronize()
+    du = DeviceArray.from_numpy(u)
+    dv = DeviceArray.from_numpy(v)
     net = C2pNet(device, params if params is not None else init_params(seed))
-    loss = net.train(grid, du.data_ptr(), dv.data_ptr(), n, steps, min(batch, n), lr, lr_final)
+    loss = net.train(grid, du.ptr, dv.ptr, n, steps, min(batch, n), lr, lr_final)
     p = net.params()
     net.close()
     grid.close()

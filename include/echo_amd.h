@@ -125,6 +125,37 @@ int echo_set_tout(echo_grid *g, double tout);
 int echo_cons2prim_batch(echo_grid *g, const double *u, double *v, int32_t *status,
                          int64_t n);
 
+/* Tile shape of the flux sweeps and its autotuner (no counterpart in the
+ * reference, whose sweeps are pencil loops, evolve.f90:302-685).  A y or z
+ * sweep with `rows` = 2 or 4 runs blocks of 256/rows faces along x by `rows`
+ * faces along the sweep (WENO5 and PPM4); results are identical for every
+ * shape.  echo_tune times each shape on the current state (reps launches
+ * each) and keeps the fastest; ms (optional, 9 doubles): per-launch times,
+ * [3*sweep + {0,1,2}] for rows {1,2,4}. */
+int echo_set_tile(echo_grid *g, int32_t sweep, int32_t rows);
+int echo_get_tile(const echo_grid *g, int32_t *rows);
+int echo_tune(echo_grid *g, int32_t reps, double *ms);
+
+/* ---- event-by-event ensemble (no counterpart in the reference, which runs
+ * one event per process, echo.f90:134-280) ---------------------------------
+ * n_events independent grids (configurations may differ in size, range,
+ * physics) on one device, sharing one stream and one pool of face buffers
+ * sized for the largest event.  echo_ensemble_grid gives event i's grid for
+ * the per-grid calls (init, upload, download, scalars; not echo_destroy).
+ * echo_ensemble_steps advances every event by nsteps time steps, each step
+ * one replay of a captured graph holding all events' kernels; *n_crashed
+ * counts crashed events (their scalars give the error codes). */
+typedef struct echo_ensemble echo_ensemble;
+int echo_ensemble_create(const echo_config *cfgs, int32_t n_events, const double *eos_rows,
+                         int32_t n_rows, int32_t device, echo_ensemble **out);
+void echo_ensemble_destroy(echo_ensemble *e);
+int echo_ensemble_size(const echo_ensemble *e);
+echo_grid *echo_ensemble_grid(echo_ensemble *e, int32_t i);
+int echo_ensemble_tune(echo_ensemble *e, int32_t reps);
+int echo_ensemble_steps(echo_ensemble *e, int32_t nsteps, int32_t *n_crashed);
+/* wall time (HIP events) of the last echo_ensemble_steps call, ms */
+int echo_ensemble_last_ms(const echo_ensemble *e, double *ms);
+
 /* Batched con2prim on device-resident states (device pointers, same layout
  * as echo_cons2prim_batch): the plain solver of the grid's system. */
 int echo_cons2prim_batch_dev(echo_grid *g, const double *d_u, double *d_v, int32_t *d_status,
@@ -180,6 +211,14 @@ int echo_c2p_surrogate(echo_grid *g, echo_mlp *m, const double *d_u, double *d_v
  * echo_c2p_surrogate: t[0] network + refinement kernel, t[1] fallback. */
 int echo_set_timing(echo_grid *g, int32_t on);
 int echo_get_timing(echo_grid *g, double *t);
+
+/* Device memory for callers without a HIP runtime of their own (the
+ * _dev and network entry points take such pointers).  Copies are
+ * synchronous. */
+int echo_dev_alloc(int32_t device, int64_t bytes, void **ptr);
+int echo_dev_free(int32_t device, void *ptr);
+int echo_dev_upload(int32_t device, void *dst, const void *src, int64_t bytes);
+int echo_dev_download(int32_t device, void *dst, const void *src, int64_t bytes);
 
 const char *echo_last_error(void);
 
