@@ -74,11 +74,11 @@ def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_
     return e_per_participant * e / (2 * np.pi * smear ** 2), len(part)
 
 
-def workload_state(ic, rc, eos_rows):
+def workload_state(ic, rc, eos_rows, seed=12345, b=7.0):
     nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
     dx = (rc[1] - rc[0]) / nx
     tau0 = rc[6]
-    eT, npart = glauber_energy(nx, ny, dx)
+    eT, npart = glauber_energy(nx, ny, dx, seed=seed, b=b)
     eta = (np.arange(nz) + 0.5) * (rc[5] - rc[4]) / nz + rc[4]
     prof = np.exp(-np.maximum(np.abs(eta) - 2.0, 0.0) ** 2 / (2 * 1.0 ** 2))
     e = eT[None, :, :] * prof[:, None, None] + 0.05
@@ -136,6 +136,116 @@ def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=32, steps=1):
                       + ("" if e == 0 else f", crashed with code {e}")}
 
 
+def read_traffic(kernel, nx, ny, nz):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (profiles/hbm_traffic_<nx>x<ny>x<nz>.txt: lines '<kernel> <bytes>'), or
+    None when no summary covers this workload."""
+    path = os.path.join(REPO, "profiles", f"hbm_traffic_{nx}x{ny}x{nz}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            f = line.split()
+            if len(f) >= 2 and f[0] == kernel:
+                try:
+                    return float(f[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def c2p_leg(n_states, seed=99):
+    """Learned vs plain con2prim on n_states ideal-MHD (p = e/3) states
+    around snapshots of the Gubser-flow run (128x128, RK3, 100 steps):
+    kernel times (HIP events) with the states resident in HBM, and the
+    largest error against the exact primitives the states were built from."""
+    from echoamd import C2pNet, DeviceArray
+    from echoamd.surrogate import (WEIGHTS, e3p_grid, gubser_grid, gubser_snapshots,
+                                   load_weights, mhd_e3p_states)
+    if not os.path.exists(WEIGHTS):
+        return {"skipped": "no trained weights"}
+    g1 = gubser_grid(0)
+    snap = gubser_snapshots(g1)
+    g1.close()
+    u, vt, vw = mhd_e3p_states(n_states, snap, seed=seed)
+    sc = np.empty_like(vt)
+    sc[0] = np.abs(vt[0]); sc[1:4] = np.abs(vt[1:4]).max(0)
+    sc[4] = np.abs(vt[4]); sc[5:8] = np.abs(vt[5:8]).max(0)
+
+    def max_err(v):
+        return float((np.abs(v - vt) / sc).max())
+
+    g = e3p_grid(0)
+    g.set_timing(True)
+    du = DeviceArray.from_numpy(u)
+    dv = DeviceArray(vw.shape, np.float64)
+    ds = DeviceArray((n_states,), np.int32)
+    out = {"states": n_states, "system": "ideal MHD, p = e/3 (algorithm 5), Minkowski",
+           "bytes_per_state": 128}
+    for name, guess in (("plain_cold", np.zeros_like(vw)), ("plain_warm", vw)):
+        ts = []
+        for _ in range(3):
+            dv.upload(guess)
+            g.cons2prim_batch_dev(du.ptr, dv.ptr, ds.ptr, n_states)
+            ts.append(g.timing_raw()[0])
+        st = ds.numpy()
+        out[name] = {"ms": min(ts), "failed": int((st != 0).sum()), "max_err": max_err(dv.numpy())}
+    net = C2pNet(0, load_weights())
+    ts = []
+    for _ in range(3):
+        dv.upload(vw)
+        nfb = net.invert(g, du.ptr, dv.ptr, ds.ptr, n_states, iters=2, tol=1e-11)
+        ts.append(tuple(g.timing_raw()[:2]))
+    best = min(ts, key=lambda t: t[0] + t[1])
+    ms = best[0] + best[1]
+    out["surrogate"] = {"ms": ms, "ms_network_newton": best[0], "ms_fallback": best[1],
+                        "newton_iters": 2, "gate_tol": 1e-11, "fallback": int(nfb),
+                        "max_err": max_err(dv.numpy())}
+    out["speedup_vs_plain_cold"] = out["plain_cold"]["ms"] / ms
+    out["speedup_vs_plain_warm"] = out["plain_warm"]["ms"] / ms
+    out["surrogate_states_per_s"] = n_states / (ms * 1e-3)
+    out["surrogate_hbm_frac"] = 128.0 * n_states / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    net.close()
+    g.close()
+    return out
+
+
+def ensemble_leg(n_events, steps, eos_rows, seed=2026):
+    """Event-by-event ensemble: n_events Au-Au events with impact parameter
+    b ~ U[0, 12] fm, each on a grid sized to its fireball (transverse
+    96^2..192^2 cells of 0.15 fm, 64..128 eta cells of 0.1), viscous hydro as
+    the single-event run, all advanced together (one captured graph per step,
+    autotuned tiles).  Value: total cells x steps / wall time."""
+    from echoamd import Ensemble
+    rng = np.random.default_rng(seed)
+    bs = rng.uniform(0.0, 12.0, n_events)
+    cfgs, states = [], []
+    for k, b in enumerate(bs):
+        f = 1.0 - b / 12.0
+        nt = int(round((96 + 96 * f) / 16)) * 16
+        ne = int(round((64 + 64 * f) / 16)) * 16
+        ic, rc = workload_config(nt, nt, ne)
+        v, _ = workload_state(ic, rc, eos_rows, seed=seed + k, b=b)
+        cfgs.append((ic, rc))
+        states.append(v)
+    ens = Ensemble(cfgs, eos_rows=eos_rows)
+    for g, v in zip(ens.grids, states):
+        g.init_from_prim(v)
+    del states
+    ens.steps(1)
+    ens.tune(reps=2)
+    ens.steps(1)
+    crashed = ens.steps(steps)
+    ms = ens.last_ms()
+    cells = sum(int(ic[0]) * int(ic[1]) * int(ic[2]) for ic, _ in cfgs)
+    tiles = [g.tile().tolist() for g in ens.grids]
+    ens.close()
+    return {"events": n_events, "cells": cells, "steps": steps, "ms": ms,
+            "zone_updates_per_s": cells * steps / (ms * 1e-3), "crashed": crashed,
+            "grid_sizes": sorted({f"{int(ic[0])}x{int(ic[1])}x{int(ic[2])}" for ic, _ in cfgs}),
+            "tiles_chosen": {str(t): tiles.count(t) for t in map(list, {tuple(t) for t in tiles})}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +255,11 @@ def main():
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--c2p-states", type=int, default=1 << 24,
+                    help="states of the learned-con2prim leg (0: skip)")
+    ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--plain", action="store_true",
                     help="time with plain kernel launches instead of graph replays "
                          "(for counter collection with rocprofv3 --pmc)")
@@ -161,6 +276,8 @@ def main():
     g.init_from_prim(v0)
     nx, ny, nz, nv = args.nx, args.ny, args.nz, g.nv
     N = nx * ny * nz
+    g.steps(1)
+    tune_ms = None if args.no_tune else g.tune(reps=3)
     if args.plain:
         g.set_timing(True)
     g.steps(args.warmup)
@@ -184,21 +301,28 @@ def main():
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded MC-Glauber initial state)",
+        "data": "synthetic (seeded MC-Glauber initial states; seeded con2prim states)",
         "config": {"workload": f"single-event Au-Au grid {nx}x{ny}x{nz} (x,y,eta), 3+1D "
                                "second-order viscous hydro (eta/s=0.1), tabulated EoS, "
-                               "WENO5+HLL+RK3; magnetic/resistive terms not in this build",
+                               "WENO5+HLL+RK3; magnetic/resistive terms not in this run",
                    "cells": N, "nv": nv, "stages_per_step": 3, "participants": npart,
-                   "t_end": sc["t"]},
+                   "t_end": sc["t"], "tiles": g.tile().tolist()},
         "roofline": {"kernel": f"k_flux/k_update ({dom})", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(dom, nx, ny, nz),
                      "ms_per_launch": {k: kern[k][0] for k in kern},
                      "algorithmic_bytes_per_launch": {k: kern[k][1] for k in kern}},
     }
+    if tune_ms is not None:
+        out["config"]["tune_ms_per_launch"] = {"y": tune_ms[1].tolist(), "z": tune_ms[2].tolist()}
+    g.close()
+    if args.c2p_states > 0:
+        out["c2p"] = c2p_leg(args.c2p_states)
+    if args.events > 0:
+        out["ensemble"] = ensemble_leg(args.events, args.ensemble_steps, eos_rows)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, rc, eos_rows)
-    g.close()
     print(json.dumps(out), flush=True)
 
 

@@ -1,8 +1,8 @@
 // Learned con2prim for ideal MHD with p = e/3: an 8 -> 64 -> 64 -> 64 -> 5
 // network evaluated on MFMA (bfloat16 operands, FP32 accumulation) gives the
-// initial guess (v^2, W) of the inversion; two-variable FP64 Newton
-// iterations refine it and a residual gate sends every state it does not
-// accept to the plain solver (algorithm 5 of system_cons2prim,
+// initial guess of the inversion (W = (e+p) gamma^2, with v^2 following
+// from W exactly); FP64 Newton iterations refine it and a residual gate
+// sends every state it does not accept to the plain solver (algorithm 5 of system_cons2prim,
 // system.f90:945-998).  The network is trained on the device by the
 // kernels below (forward with saved activations, backward, weight gradient,
 // Adam).
@@ -117,62 +117,73 @@ __device__ __forceinline__ void features(const double* u, double g33, float* x) 
   x[7] = (float)(2.0 * oms / (oms + b2) - 1.0);
 }
 
+// Baseline of the W head: W0 = U' + p0, p0 the conformal hydro closed form
+// (-U' + sqrt(4 U'^2 - 3 S^2))/3 at U' = U - B^2/2 (magnetic energy
+// removed), floored at 1e-3 U'.  The head predicts 25 log(W/W0), a quantity
+// ~45x narrower than log(W/U) over the training states.
+__device__ __forceinline__ double w_baseline(double S2, double B2, double U) {
+  const double Up = dmax(U - 0.5 * B2, 1e-3 * U);
+  const double disc = dmax(4.0 * Up * Up - 3.0 * S2, 0.0);
+  const double p0 = dmax((-Up + sqrt(disc)) / 3.0, 1e-3 * Up);
+  return Up + p0;
+}
+
 // Normalised training targets of a labelled pair (de-densitised u, v).
 __device__ __forceinline__ void targets(const double* u, const double* v, double g33, double* t) {
+  const double S2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3] / g33;
   const double B2 = u[5] * u[5] + u[6] * u[6] + u[7] * u[7] * g33;
   const double U = u[4];
   const double v2 = v[1] * v[1] + v[2] * v[2] + v[3] * v[3] * g33;
   const double z = -log(1.0 - v2);
   const double W = 4.0 * v[4] / (1.0 - v2);
   t[0] = z / 3.0 - 1.0;
-  t[1] = 2.0 * log(W / U);
+  t[1] = 25.0 * log(W / w_baseline(S2, B2, U));
   t[2] = (log(v[4] / U) + 4.0) / 3.0;
   t[3] = 2.0 * U / (W + B2) - 1.0;
   t[4] = 2.0 * v2 - 1.0;
 }
 
-// Residuals of the p = e/3 ideal-MHD inversion in (x = v^2, W) and their
-// Jacobian (row-major): the energy cubic of solver8 (system.f90:2152-2201)
-// and the momentum equation that rtsafe drives to zero there.
-__device__ __forceinline__ void e3p_residual(double x, double W, double S2, double B2, double SB2,
-                                             double U, double& R1, double& R2, double* J) {
-  const double c3 = 0.25 * (3.0 + x);
-  const double c2 = 0.5 * B2 * (1.0 + x) - U;
+// Refinement for ideal MHD with p = e/3.  Algorithm 5 of the reference
+// (solver8, system.f90:2152-2201) solves the energy cubic in W nested inside
+// rtsafe on the momentum equation in x = v^2.  The momentum equation is
+// linear in x, so x(W) = (S^2 + (2W + B^2)(S.B)^2/W^2) / (W + B^2)^2 exactly,
+// and the refinement is Newton on f(W) = c3(x) W^3 + c2(x) W^2 - (S.B)^2/2
+// alone (c3 = (3 + x)/4, c2 = B^2 (1 + x)/2 - U), from the network's W.
+__device__ __forceinline__ void e3p_w(double W, double S2, double B2, double SB2, double U, double& f,
+                                      double& df, double& x, double& dxdw) {
   const double iw = 1.0 / W;
   const double wb = W + B2;
+  const double iwb = 1.0 / wb;
   const double sbw2 = SB2 * iw * iw;
-  R1 = (c3 * W + c2) * W * W - 0.5 * SB2;
-  R2 = wb * wb * x - (2.0 * W + B2) * sbw2 - S2;
-  J[0] = W * W * (0.25 * W + 0.5 * B2);
-  J[1] = W * (3.0 * c3 * W + 2.0 * c2);
-  J[2] = wb * wb;
-  J[3] = 2.0 * wb * x - 2.0 * sbw2 + 2.0 * (2.0 * W + B2) * sbw2 * iw;
+  x = (S2 + (2.0 * W + B2) * sbw2) * iwb * iwb;
+  dxdw = (2.0 * sbw2 - 2.0 * (2.0 * W + B2) * sbw2 * iw) * iwb * iwb - 2.0 * x * iwb;
+  const double c3 = 0.25 * (3.0 + x);
+  const double c2 = 0.5 * B2 * (1.0 + x) - U;
+  f = (c3 * W + c2) * W * W - 0.5 * SB2;
+  df = W * (3.0 * c3 * W + 2.0 * c2) + W * W * (0.25 * W + 0.5 * B2) * dxdw;
 }
 
-// Newton refinement of the guess (y0, y1) and the residual gate.  u: the
-// de-densitised state; writes the 8 MHD primitives on success.
-__device__ __forceinline__ bool refine(const double* u, float y0, float y1, double g33, int iters,
-                                       double tol, double* v) {
+// Newton refinement of the network's W (y1 = 25 log(W/W0)) and the residual
+// gate: accepted when the Newton correction dW at the final W is below
+// tol * W, the change it implies in 1 - v^2 is below tol * (1 - v^2), and
+// 0 <= v^2 < 1.  u: the de-densitised state; writes the 8 primitives.
+__device__ __forceinline__ bool refine(const double* u, float y1, double g33, int iters, double tol,
+                                       double* v) {
   const double S2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3] / g33;
   const double B2 = u[5] * u[5] + u[6] * u[6] + u[7] * u[7] * g33;
   const double SB = u[1] * u[5] + u[2] * u[6] + u[3] * u[7];
   const double SB2 = SB * SB, U = u[4];
-  const double z = 3.0 * ((double)y0 + 1.0);
-  double x = 1.0 - exp(-dmax(z, 0.0));
-  double W = U * exp(0.5 * (double)y1);
-  double R1, R2, J[4];
+  double W = w_baseline(S2, B2, U) * exp((double)y1 / 25.0);
+  double f, df, x, dxdw;
   for (int it = 0; it < iters; it++) {
-    e3p_residual(x, W, S2, B2, SB2, U, R1, R2, J);
-    const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
-    x -= (R1 * J[3] - R2 * J[1]) * idet;
-    W -= (J[0] * R2 - J[2] * R1) * idet;
+    e3p_w(W, S2, B2, SB2, U, f, df, x, dxdw);
+    W -= f / df;
   }
-  e3p_residual(x, W, S2, B2, SB2, U, R1, R2, J);
-  const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
-  const double dx = (R1 * J[3] - R2 * J[1]) * idet;
-  const double dW = (J[0] * R2 - J[2] * R1) * idet;
-  const bool ok = (x >= 0.0) && (x < 1.0) && (W > 0.0) && (fabs(dx) <= tol * (1.0 - x)) &&
-                  (fabs(dW) <= tol * W);
+  e3p_w(W, S2, B2, SB2, U, f, df, x, dxdw);
+  const double dW = f / df;
+  // the correction's effect on W and on 1 - v^2 (p and rho scale with it)
+  const bool ok = (W > 0.0) && (x >= 0.0) && (x < 1.0) && (fabs(dW) <= tol * W) &&
+                  (fabs(dxdw * dW) <= tol * (1.0 - x));
   if (!ok) return false;
   // primitives as at the end of algorithm 5 (system.f90:990-998)
   const double vb = SB / W;
@@ -379,9 +390,9 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
     for (int q = 0; q < 4; q++)
       if (g == 0) sY[w][16 * q + c] = make_float2(y4[q][0], y4[q][1]);
     __syncthreads();
-    const float2 yy = sY[w][lane];
+    const float y1 = sY[w][lane].y;
     double v[8];
-    const bool ok = valid && refine(u, yy.x, yy.y, g33, ca.iters, ca.tol, v);
+    const bool ok = valid && refine(u, y1, g33, ca.iters, ca.tol, v);
     if (ok) {
 #pragma unroll
       for (int k = 0; k < 8; k++) Vb[k * stride + off + s] = v[k];

@@ -206,7 +206,11 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
     a = ap.parse_args(argv)
+    import time
+    t0 = time.perf_counter()
     p, loss = train(n_train=a.n_train, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed)
+    print(f"trained {a.steps} steps of {a.batch} states in {time.perf_counter() - t0:.1f} s "
+          "(data generation and labelling included)")
     k = max(1, len(loss) // 20)
     for i in range(0, len(loss), k):
         print(f"step {i:6d} loss {loss[i:i + k].mean():.3e}", flush=True)

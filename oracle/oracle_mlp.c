@@ -9,9 +9,9 @@
  *     (orc_mlp_loss with round = 0) -- tests/test_mlp_oracle.py;
  *   - the refinement by the plain solver of the reference (algorithm 5 of
  *     system_cons2prim, system.f90:945-998, restated in oracle_sys.c and
- *     bit-exact against the reference): both must reach the same root;
- *   - the residual functions by prim2cons round trips (a state built from
- *     primitives has zero residual at its own (v^2, W)).
+ *     bit-exact against the reference): both must reach the same root, and
+ *     by prim2cons round trips (a state built from primitives is recovered);
+ *   - the derivative of the refinement's residual by finite differences.
  *
  * Network: 8 -> 64 -> 64 -> 64 -> 5, activation gelu(x) = x * phi(clamp(x,
  * -3, 3)) with phi an odd degree-9 polynomial fit of the normal CDF (max
@@ -127,9 +127,22 @@ void orc_mlp_features(const double *u, double g33, float *x) {
   x[7] = (float)(2.0 * oms / (oms + b2) - 1.0);
 }
 
+/* Baseline of the W head: W0 = U' + p0 with p0 the conformal hydro closed
+ * form p0 = (-U' + sqrt(4 U'^2 - 3 S^2))/3 at U' = U - B^2/2 (the state with
+ * its magnetic energy removed), floored at 1e-3 U'. */
+double orc_mlp_w0(const double *u, double g33) {
+  double S2, B2, SB;
+  invariants(u, g33, &S2, &B2, &SB);
+  const double Up = fmax(u[4] - 0.5 * B2, 1e-3 * u[4]);
+  const double disc = fmax(4.0 * Up * Up - 3.0 * S2, 0.0);
+  const double p0 = fmax((-Up + sqrt(disc)) / 3.0, 1e-3 * Up);
+  return Up + p0;
+}
+
 /* Training targets (normalised) of a labelled pair (u, v), p = e/3:
- * t0 = z/3 - 1 with z = -log(1 - v^2); t1 = 2 log(W/U) with W = 4 p gamma^2;
- * t2 = (log(p/U) + 4)/3; t3 = 2 U/(W + B^2) - 1; t4 = 2 v^2 - 1. */
+ * t0 = z/3 - 1 with z = -log(1 - v^2); t1 = 25 log(W/W0) with
+ * W = 4 p gamma^2 (the head the refinement uses); t2 = (log(p/U) + 4)/3;
+ * t3 = 2 U/(W + B^2) - 1; t4 = 2 v^2 - 1. */
 void orc_mlp_targets(const double *u, const double *v, double g33, double *t) {
   double S2, B2, SB;
   invariants(u, g33, &S2, &B2, &SB);
@@ -138,7 +151,7 @@ void orc_mlp_targets(const double *u, const double *v, double g33, double *t) {
   const double z = -log(1.0 - v2);
   const double W = 4.0 * v[4] / (1.0 - v2);
   t[0] = z / 3.0 - 1.0;
-  t[1] = 2.0 * log(W / U);
+  t[1] = 25.0 * log(W / orc_mlp_w0(u, g33));
   t[2] = (log(v[4] / U) + 4.0) / 3.0;
   t[3] = 2.0 * U / (W + B2) - 1.0;
   t[4] = 2.0 * v2 - 1.0;
@@ -238,31 +251,41 @@ double orc_mlp_loss(const float *P, const double *u, const double *v, long n, do
 }
 
 /* ---------------------------------------------------------------------- */
-/* FP64 refinement of a guess (x = v^2, W = (e+p) gamma^2) for ideal MHD with
- * p = e/3: two-variable Newton on the pair of equations that algorithm 5 of
- * the reference solves nested (solver8, system.f90:2152-2201: the energy
- * cubic in W at fixed v^2, and the momentum equation for v^2):
+/* FP64 refinement for ideal MHD with p = e/3.  Algorithm 5 of the reference
+ * (solver8, system.f90:2152-2201) solves two equations in (x = v^2,
+ * W = (e+p) gamma^2): the energy cubic
  *   R1 = c3 W^3 + c2 W^2 + c0,  c3 = (3 + x)/4, c2 = B^2 (1 + x)/2 - U,
- *        c0 = -(S.B)^2/2
- *   R2 = (W + B^2)^2 x - (2W + B^2) (S.B)^2 / W^2 - S^2.                    */
-static void mhd_e3p_residual(double x, double W, double S2, double B2, double SB2, double U,
-                             double *R1, double *R2, double *J) {
-  const double c3 = 0.25 * (3.0 + x);
-  const double c2 = 0.5 * B2 * (1.0 + x) - U;
+ *        c0 = -(S.B)^2/2,
+ * nested inside rtsafe on the momentum equation
+ *   R2 = (W + B^2)^2 x - (2W + B^2) (S.B)^2 / W^2 - S^2.
+ * R2 is linear in x, so x(W) = (S^2 + (2W + B^2)(S.B)^2/W^2) / (W + B^2)^2
+ * exactly, and the refinement is Newton on f(W) = R1(x(W), W) alone, started
+ * from the network's W. */
+static void e3p_w(double W, double S2, double B2, double SB2, double U, double *f, double *df,
+                  double *xo, double *dxo) {
   const double iw = 1.0 / W;
   const double wb = W + B2;
+  const double iwb = 1.0 / wb;
   const double sbw2 = SB2 * iw * iw;
-  *R1 = (c3 * W + c2) * W * W - 0.5 * SB2;
-  *R2 = wb * wb * x - (2.0 * W + B2) * sbw2 - S2;
-  J[0] = W * W * (0.25 * W + 0.5 * B2);
-  J[1] = W * (3.0 * c3 * W + 2.0 * c2);
-  J[2] = wb * wb;
-  J[3] = 2.0 * wb * x - 2.0 * sbw2 + 2.0 * (2.0 * W + B2) * sbw2 * iw;
+  const double x = (S2 + (2.0 * W + B2) * sbw2) * iwb * iwb;
+  const double dxdw = (2.0 * sbw2 - 2.0 * (2.0 * W + B2) * sbw2 * iw) * iwb * iwb - 2.0 * x * iwb;
+  const double c3 = 0.25 * (3.0 + x);
+  const double c2 = 0.5 * B2 * (1.0 + x) - U;
+  *f = (c3 * W + c2) * W * W - 0.5 * SB2;
+  *df = W * (3.0 * c3 * W + 2.0 * c2) + W * W * (0.25 * W + 0.5 * B2) * dxdw;
+  *xo = x;
+  *dxo = dxdw;
 }
 
-/* From network outputs y (normalised) to primitives.  Returns 0 when the
- * state passes the gate (after `iters` Newton iterations the Newton
- * correction at the final point is below tol relative to 1 - v^2 and to W),
+void orc_e3p_w(double W, double S2, double B2, double SB2, double U, double *out3) {
+  double dx;
+  e3p_w(W, S2, B2, SB2, U, &out3[0], &out3[1], &out3[2], &dx);
+}
+
+/* From network outputs y (normalised; y[1] = 25 log(W/W0)) to primitives.
+ * Returns 0 when the state passes the gate (after `iters` Newton iterations
+ * the Newton correction dW at the final W is below tol * W, the change it
+ * implies in 1 - v^2 is below tol * (1 - v^2), and 0 <= v^2 < 1),
  * 1 when it must fall back to the plain solver.  u: 8 conserved values
  * multiplied by sqrt(-g) (as stored on the grid); gm = 1/sqrt(-g). */
 int orc_mlp_refine(const double *u_in, const float *y, double g33, double gm, int iters,
@@ -272,22 +295,17 @@ int orc_mlp_refine(const double *u_in, const float *y, double g33, double gm, in
   double S2, B2, SB;
   invariants(u, g33, &S2, &B2, &SB);
   const double SB2 = SB * SB, U = u[4];
-  const double z = 3.0 * ((double)y[0] + 1.0);
-  double x = 1.0 - exp(-fmax(z, 0.0));
-  double W = U * exp(0.5 * (double)y[1]);
-  double R1, R2, J[4];
+  double W = orc_mlp_w0(u, g33) * exp((double)y[1] / 25.0);
+  double f, df, x, dxdw;
   for (int it = 0; it < iters; it++) {
-    mhd_e3p_residual(x, W, S2, B2, SB2, U, &R1, &R2, J);
-    const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
-    x -= (R1 * J[3] - R2 * J[1]) * idet;
-    W -= (J[0] * R2 - J[2] * R1) * idet;
+    e3p_w(W, S2, B2, SB2, U, &f, &df, &x, &dxdw);
+    W -= f / df;
   }
-  mhd_e3p_residual(x, W, S2, B2, SB2, U, &R1, &R2, J);
-  const double idet = 1.0 / (J[0] * J[3] - J[1] * J[2]);
-  const double dx = (R1 * J[3] - R2 * J[1]) * idet;
-  const double dW = (J[0] * R2 - J[2] * R1) * idet;
-  const int ok = (x >= 0.0) && (x < 1.0) && (W > 0.0) && (fabs(dx) <= tol * (1.0 - x)) &&
-                 (fabs(dW) <= tol * W);
+  e3p_w(W, S2, B2, SB2, U, &f, &df, &x, &dxdw);
+  const double dW = f / df;
+  /* the correction's effect on W and on 1 - v^2 (p and rho scale with it) */
+  const int ok = (W > 0.0) && (x >= 0.0) && (x < 1.0) && (fabs(dW) <= tol * W) &&
+                 (fabs(dxdw * dW) <= tol * (1.0 - x));
   if (!ok) return 1;
   /* primitives as at the end of algorithm 5 (system.f90:990-998) */
   const double vb = SB / W;

@@ -14,10 +14,12 @@ float orc_bf16(float x);
 float orc_gelu(float x);
 float orc_gelu_d(float x);
 void orc_mlp_features(const double *u, double g33, float *x);
+double orc_mlp_w0(const double *u, double g33);
 void orc_mlp_targets(const double *u, const double *v, double g33, double *t);
 void orc_mlp_forward(const float *P, const float *x, int round, float *y);
 double orc_mlp_loss(const float *P, const double *u, const double *v, long n, double g33,
                     int round, double *grad);
+void orc_e3p_w(double W, double S2, double B2, double SB2, double U, double *out3);
 int orc_mlp_refine(const double *u_in, const float *y, double g33, double gm, int iters,
                    double tol, double *v);
 #ifdef __cplusplus

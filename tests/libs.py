@@ -224,6 +224,10 @@ class OracleMLP:
         lib.orc_mlp_forward.argtypes = [fp, fp, C.c_int, fp]
         lib.orc_mlp_loss.restype = C.c_double
         lib.orc_mlp_loss.argtypes = [fp, dp, dp, C.c_long, C.c_double, C.c_int, C.c_void_p]
+        lib.orc_mlp_w0.restype = C.c_double
+        lib.orc_mlp_w0.argtypes = [dp, C.c_double]
+        lib.orc_e3p_w.restype = None
+        lib.orc_e3p_w.argtypes = [C.c_double] * 5 + [dp]
         lib.orc_mlp_refine.restype = C.c_int
         lib.orc_mlp_refine.argtypes = [dp, fp, C.c_double, C.c_double, C.c_int, C.c_double, dp]
         self.lib = lib
@@ -263,6 +267,15 @@ class OracleMLP:
         L = self.lib.orc_mlp_loss(P, u, v, u.shape[1], g33, rnd,
                                   g.ctypes.data if grad else None)
         return (L, g) if grad else L
+
+    def w0(self, u, g33=1.0):
+        return self.lib.orc_mlp_w0(np.ascontiguousarray(u, dtype=np.float64), g33)
+
+    def e3p_w(self, W, S2, B2, SB2, U):
+        """(f, df/dW, x) of the W-refinement residual."""
+        o = np.zeros(3)
+        self.lib.orc_e3p_w(W, S2, B2, SB2, U, o)
+        return o
 
     def refine(self, u, y, g33=1.0, gm=1.0, iters=2, tol=1e-11):
         v = np.zeros(8)

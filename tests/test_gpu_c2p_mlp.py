@@ -200,9 +200,18 @@ def test_inversion(om, oracle, coords, t, divclean):
     g.close()
 
 
+def _cold_y1(om, u):
+    """y1 of the cold guess: the state at rest, W = 4 p with
+    p = (U - B^2/2)/3 (floored at U/10)."""
+    B2 = (u[5:8] ** 2).sum(0)
+    W = 4.0 * np.maximum(u[4] - 0.5 * B2, 0.1 * u[4]) / 3.0
+    return np.array([25.0 * np.log(W[s] / om.w0(u[:, s])) for s in range(u.shape[1])])
+
+
 def test_network_gain_over_cold_start(om):
     """With the trained network two Newton iterations satisfy the gate for
-    (almost) every state; from the cold guess (v = 0, W = U) they do not."""
+    (almost) every state; from the cold guess (the state at rest) they do
+    not."""
     from echoamd import C2pNet
     from echoamd.surrogate import e3p_grid, mhd_e3p_states
     P = _weights_or_skip()
@@ -214,14 +223,15 @@ def test_network_gain_over_cold_start(om):
     net = C2pNet(0, P)
     net.forward(g, du.ptr, n, dy.ptr)
     y = dy.numpy()
-    cold = np.array([-1.0, 0.0, 0, 0, 0], dtype=np.float32)   # z = 0, W = U
+    y1c = _cold_y1(om, u)
     acc_net = acc_cold = 0
     for s in range(n):
         acc_net += om.refine(u[:, s], y[:, s], iters=2, tol=1e-11)[0] == 0
-        acc_cold += om.refine(u[:, s], cold, iters=2, tol=1e-11)[0] == 0
+        yc = np.array([0.0, y1c[s], 0, 0, 0], dtype=np.float32)
+        acc_cold += om.refine(u[:, s], yc, iters=2, tol=1e-11)[0] == 0
     print(f"\naccepted after 2 Newton iterations: network guess {acc_net}/{n}, "
           f"cold guess {acc_cold}/{n}")
     assert acc_net >= 0.95 * n
-    assert acc_net - acc_cold >= 0.5 * n
+    assert acc_net - acc_cold >= 0.3 * n
     net.close()
     g.close()

@@ -6,7 +6,8 @@ restatement independently of it:
 - Newton refinement + primitive formulas: convergence to the root that the
   reference's own algorithm 5 (restated in oracle_sys.c, bit-exact against
   the reference Fortran) reaches, and to the primitives the state was built
-  from;
+  from; the refinement's residual vanishes at the exact solution and its
+  derivative matches finite differences;
 - residual gate: rejects an unconverged guess, accepts a converged one;
 - features: invariance under rotations and under the scaling
   (D, S, U, B) -> (l D, l S, l U, sqrt(l) B) that maps solutions of the
@@ -109,13 +110,13 @@ def test_targets_encode_the_solution(om):
         v2 = (vt[1:4, s] ** 2).sum()
         assert abs(1 - np.exp(-3 * (t[0] + 1)) - v2) < 1e-13
         W = 4 * vt[4, s] / (1 - v2)
-        assert abs(u[4, s] * np.exp(0.5 * t[1]) / W - 1) < 1e-13
+        assert abs(om.w0(u[:, s]) * np.exp(t[1] / 25.0) / W - 1) < 1e-13
         assert abs(t[4] - (2 * v2 - 1)) < 1e-15
 
 
 @pytest.mark.parametrize("coords,t", [(1, 1.0), (2, 1.7)])
 def test_refine_reaches_the_reference_root(oracle, om, coords, t):
-    """From a guess 1e-4 off in (v^2, W), two Newton iterations reach the
+    """From a W guess 1e-4 off, two Newton iterations reach the
     primitives the state was built from to 1e-11 relative, on a Minkowski and
     a Bjorken metric; the reference's algorithm 5 reaches the same root only
     to its own tolerance (v^2 to about 1e-12 absolute, so p to ~1e-12 gamma^2
@@ -134,7 +135,7 @@ def test_refine_reaches_the_reference_root(oracle, om, coords, t):
         e, us, _ = oracle.prim2cons(vp)
         assert e == 0
         tg = om.targets(us / gp, vp, g33)
-        y = (tg[:5] * (1 + 1e-4 * rng.standard_normal(5))).astype(np.float32)
+        y = (tg + 25 * 1e-4 * rng.standard_normal(5)).astype(np.float32)   # W off by ~1e-4
         st, vr = om.refine(us, y, g33=g33, gm=1.0 / gp, iters=2, tol=1e-11)
         guess = vw[:, s].copy()
         if coords == 2:
@@ -149,12 +150,33 @@ def test_refine_reaches_the_reference_root(oracle, om, coords, t):
     assert nacc >= 190
 
 
+def test_refinement_residual(om):
+    """At the exact solution the residual vanishes and x(W) = v^2; its
+    derivative matches central differences."""
+    u, vt, _ = _states(60, seed=15)
+    for s in range(60):
+        us = u[:, s]
+        S2 = (us[1:4] ** 2).sum()
+        B2 = (us[5:8] ** 2).sum()
+        SB2 = (us[1:4] @ us[5:8]) ** 2
+        v2 = (vt[1:4, s] ** 2).sum()
+        W = 4 * vt[4, s] / (1 - v2)
+        f, df, x = om.e3p_w(W, S2, B2, SB2, us[4])
+        assert abs(f) <= 1e-12 * W ** 3 and abs(x - v2) <= 1e-13
+        for Wt in (0.9 * W, 1.3 * W):
+            h = 1e-6 * Wt
+            fd = (om.e3p_w(Wt + h, S2, B2, SB2, us[4])[0] -
+                  om.e3p_w(Wt - h, S2, B2, SB2, us[4])[0]) / (2 * h)
+            d = om.e3p_w(Wt, S2, B2, SB2, us[4])[1]
+            assert abs(fd - d) <= 1e-6 * abs(d), (s, fd, d)
+
+
 def test_gate_rejects_unconverged_and_accepts_converged(om):
     u, vt, _ = _states(100, seed=12)
     rej = acc = 0
     for s in range(100):
         tg = om.targets(u[:, s], vt[:, s])
-        y = (tg + np.array([3e-3, 3e-3, 0, 0, 0])).astype(np.float32)
+        y = (tg + np.array([0, 25 * 3e-3, 0, 0, 0])).astype(np.float32)   # W off by 3e-3
         st0, _ = om.refine(u[:, s], y, iters=0, tol=1e-11)
         st3, _ = om.refine(u[:, s], y, iters=4, tol=1e-11)
         rej += st0 == 1
@@ -196,3 +218,34 @@ def test_generator_ranges():
     assert gam.min() >= 1.0 and 18 < gam.max() < 22.5
     assert 0.7e-4 < sig.min() < 2e-4 and 0.7 < sig.max() < 1.5
     assert np.all(np.isfinite(u))
+
+
+def test_trained_network_gain_cpu(om):
+    """The committed network (run through the oracle's bfloat16 forward pass)
+    against the method without it: the share of states whose two Newton
+    iterations pass the 1e-11 gate, from the network's W versus from the
+    state at rest (W = 4p, p = (U - B^2/2)/3)."""
+    import os
+    from echoamd.surrogate import WEIGHTS, load_weights
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained weights")
+    P = load_weights()
+    u, vt, _ = _states(2000, seed=16)
+    B2 = (u[5:8] ** 2).sum(0)
+    Wc = 4.0 * np.maximum(u[4] - 0.5 * B2, 0.1 * u[4]) / 3.0
+    y1c = [25.0 * np.log(Wc[s] / om.w0(u[:, s])) for s in range(u.shape[1])]
+    acc_net = acc_cold = 0
+    worst = 0.0
+    for s in range(u.shape[1]):
+        y = om.forward(P, om.features(u[:, s]), 1)
+        st, v = om.refine(u[:, s], y, iters=2, tol=1e-11)
+        if st == 0:
+            acc_net += 1
+            worst = max(worst, float(np.max(np.abs(v - vt[:, s]) / _scales(vt[:, s]))))
+        yc = np.array([0.0, y1c[s], 0, 0, 0], dtype=np.float32)
+        acc_cold += om.refine(u[:, s], yc, iters=2, tol=1e-11)[0] == 0
+    print(f"\naccepted after 2 iterations: network {acc_net}/2000, at-rest guess {acc_cold}/2000; "
+          f"max error of accepted states {worst:.1e}")
+    assert worst <= 1e-10
+    assert acc_net >= 0.95 * 2000
+    assert acc_net - acc_cold >= 0.3 * 2000
