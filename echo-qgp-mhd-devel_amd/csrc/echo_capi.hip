@@ -234,9 +234,10 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   if (c.mhd != 0 && c.mhd != 1) return fail(ECHO_E_ARG, "MHD must be 0 or 1");
   if (c.mhd && c.viscosity) return fail(ECHO_E_ARG, "viscosity together with MHD is not supported");
   if (c.mhd && !((c.algo_solver == 4 && c.eqstate == 2) || (c.algo_solver == 5 && c.eqstate == 1) ||
-                 (c.algo_solver == 6 && c.eqstate == 1 && c.coordinates == 1)))
-    return fail(ECHO_E_ARG, "MHD needs ALG_SOLV 4 with EQ_STATE 2, or 5 (or 6 in Minkowski) with "
-                            "EQ_STATE 1 (got %d, %d)", c.algo_solver, c.eqstate);
+                 (c.algo_solver == 6 && c.eqstate == 1 && c.coordinates == 1) ||
+                 ((c.algo_solver == 1 || c.algo_solver == 2) && (c.eqstate == 1 || c.eqstate == 3))))
+    return fail(ECHO_E_ARG, "MHD needs ALG_SOLV 1 or 2 with EQ_STATE 1 or 3, 4 with EQ_STATE 2, "
+                            "or 5 (6 in Minkowski) with EQ_STATE 1 (got %d, %d)", c.algo_solver, c.eqstate);
   if (c.coordinates != 1 && c.coordinates != 2) return fail(ECHO_E_ARG, "COORD must be 1 or 2");
   if (c.nrk != 2 && c.nrk != 3) return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2) or RK3 (3)");
   if (c.boundary_cond != 0 && c.boundary_cond != 2)

@@ -644,8 +644,153 @@ __device__ __forceinline__ double get_x9(double y, double s2, double b2, double 
   return (s2 * (y * y) + b2 * sb2 + 2 * sb2 * y) / ((y * y) * ((b2 * b2) + 2 * b2 * y + (y * y)));
 }
 
+// Brent's method for N nonlinear equations (ACM algorithm 554, the
+// reference's vendored BRENTM, libacm554.f90:8-359), with the Jacobian
+// estimated row by row from divided differences and reduced by Householder
+// steps.  fcn(x, f, k) evaluates component k.  N <= 3, so every array lives
+// in registers (the loops over components are unrolled).  Returns INFO.
+template <int N, typename Fcn>
+__device__ __forceinline__ int brentm(Fcn&& fcn, double* x, double ftol, double xtol, int maxfev, int mopt) {
+  const double p05 = 5.e-2, scale = 1.e1;
+  const double eps = 1.4901161193847656e-08;   // sqrt(EPSILON(1.d0))
+  double q[N][N], sigma[N], wa1[N], wa2[N], fvec[N];
+  int info = 0, nfcall = 0, nfev = 0;
+  int nier6 = -1, nier7 = -1, nier8 = 0;
+  double fnorm = 0., difit = 0., xnorm = 0.;
+#pragma unroll
+  for (int i = 0; i < N; i++) xnorm = dmax(xnorm, fabs(x[i]));
+  double delta = scale * xnorm;
+  if (xnorm == 0.) delta = scale;
+  for (;;) {
+    int nsing = N;
+    double fnorm1 = fnorm, difit1 = difit;
+    fnorm = 0.;
+    double h = eps * xnorm;
+    if (h == 0.) h = eps;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+#pragma unroll
+      for (int i = 0; i < N; i++) q[j][i] = 0.;
+      q[j][j] = h;
+      wa1[j] = x[j];
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      fcn(wa1, fvec, k);
+      const double fky = fvec[k];
+      nfcall++;
+#pragma unroll
+      for (int j = k; j < N; j++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) wa2[i] = wa1[i] + q[j][i];
+        fcn(wa2, fvec, k);
+        nfcall++;
+        sigma[j] = fvec[k] - fky;
+      }
+      nfev = nfcall / N;
+      fnorm = dmax(fnorm, fabs(fky));
+      fvec[k] = fky;
+      double eta = 0.;
+#pragma unroll
+      for (int i = k; i < N; i++) eta = dmax(eta, fabs(sigma[i]));
+      if (eta == 0.) continue;
+      nsing = nsing - 1;
+      double sknorm = 0.;
+#pragma unroll
+      for (int i = k; i < N; i++) {
+        sigma[i] = sigma[i] / eta;
+        sknorm = sknorm + sigma[i] * sigma[i];
+      }
+      sknorm = sqrt(sknorm);
+      if (sigma[k] < 0.) sknorm = -sknorm;
+      sigma[k] = sigma[k] + sknorm;
+#pragma unroll
+      for (int i = 0; i < N; i++) wa2[i] = 0.;
+#pragma unroll
+      for (int j = k; j < N; j++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) wa2[i] = wa2[i] + sigma[j] * q[j][i];
+      }
+#pragma unroll
+      for (int j = k; j < N; j++) {
+        const double temp = sigma[j] / (sknorm * sigma[k]);
+#pragma unroll
+        for (int i = 0; i < N; i++) q[j][i] = q[j][i] - temp * wa2[i];
+      }
+      sigma[k] = sknorm * eta;
+      double temp = fky / sigma[k];
+      if (h * fabs(temp) > delta) temp = copysign(delta / h, temp);
+#pragma unroll
+      for (int i = 0; i < N; i++) wa1[i] = wa1[i] + temp * q[k][i];
+    }
+    xnorm = 0.;
+    difit = 0.;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      xnorm = dmax(xnorm, fabs(wa1[i]));
+      difit = dmax(difit, fabs(x[i] - wa1[i]));
+      x[i] = wa1[i];
+    }
+    delta = dmax(delta, scale * xnorm);
+    const bool conv = (fnorm < fnorm1 && difit < difit1 && nsing == 0);
+    nier6++;
+    nier7++;
+    nier8++;
+    if (conv) nier6 = 0;
+    if (fnorm < fnorm1 || difit < difit1) nier7 = 0;
+    if (difit > eps * xnorm) nier8 = 0;
+    if (fnorm <= ftol) info = 1;
+    if (difit <= xtol * xnorm && conv) info = 2;
+    if (fnorm <= ftol && info == 2) info = 3;
+    if (info != 0) return info;
+    if (nfev >= maxfev) info = 4;
+    if (nsing == N) info = 5;
+    if (nier6 == 5) info = 6;
+    if (nier7 == 3) info = 7;
+    if (nier8 == 4) info = 8;
+    if (info != 0) return info;
+    if (!conv || difit > p05 * xnorm || mopt == 1) continue;
+    for (int m = 2; m <= mopt; m++) {       // iterative refinement
+      fnorm1 = fnorm;
+      fnorm = 0.;
+      bool stop = false;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        if (stop) break;
+        fcn(wa1, fvec, k);
+        const double fky = fvec[k];
+        nfcall++;
+        nfev = nfcall / N;
+        fnorm = dmax(fnorm, fabs(fky));
+        if (!(fnorm < fnorm1)) {
+          fnorm = fnorm1;
+          stop = true;
+          break;
+        }
+        const double temp = fky / sigma[k];
+#pragma unroll
+        for (int i = 0; i < N; i++) wa1[i] = wa1[i] + temp * q[k][i];
+      }
+      if (stop) break;
+      xnorm = 0.;
+      difit = 0.;
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        xnorm = dmax(xnorm, fabs(wa1[i]));
+        difit = dmax(difit, fabs(x[i] - wa1[i]));
+        x[i] = wa1[i];
+      }
+      if (fnorm <= ftol) info = 1;
+      if (difit <= xtol * xnorm) info = 2;
+      if (fnorm <= ftol && info == 2) info = 3;
+      if (nfev >= maxfev && info == 0) info = 4;
+      if (info != 0) return info;
+    }
+  }
+}
+
 template <int SYS>
-__device__ __forceinline__ int mhd_cons2prim(int algo, const Metric& M, const double* u, double* v) {
+__device__ __forceinline__ int mhd_cons2prim(const Eos& E, int algo, const Metric& M, const double* u, double* v) {
   constexpr int NV = SysT<SYS>::NV;
   const double gm = M.gm, g3 = M.g3;
   double uin[NV];
@@ -655,6 +800,55 @@ __device__ __forceinline__ int mhd_cons2prim(int algo, const Metric& M, const do
   const double bx = uin[KBX], by = uin[KBY], bz = uin[KBZ];
   const double SB = (sx * bx + sy * by + sz * bz);
   const double SB2 = SB * SB;
+  if (algo == 1 || algo == 2) {
+    // system.f90:668-793: BRENTM on the 3x3 system sist3 (v^2, W, p) or the
+    // 2x2 system sist4 (W, p; v^2 explicit), system.f90:1199-1266, warm
+    // started from the cell's previous primitives.  The error code of the
+    // lookup of the previous energy density is the routine's error code.
+    const double B2 = bx * bx + by * by + bz * bz * g3;
+    const double SS2 = sx * sx + sy * sy + sz * sz / g3;
+    double pr = v[KPR], en = 0.;
+    const int err0 = eos_energy(E, v[KRH], pr, en);
+    const double v2 = M.g1 * (v[KVX] * v[KVX]) + M.g2 * (v[KVY] * v[KVY]) + g3 * (v[KVZ] * v[KVZ]);
+    double xs, ys;
+    if (algo == 1) {
+      double X[3] = {v2, (en + pr) / (1 - v2), pr};
+      brentm<3>([&](const double* Xk, double* F, int k) {
+        const double xx = Xk[0], yy = Xk[1], pp = Xk[2];
+        if (k == 0) F[0] = (yy + B2) * (yy + B2) * xx - SB2 / (yy * yy) * (2 * yy + B2) - SS2;
+        if (k == 1) F[1] = yy - pp + 0.5 * (1. + xx) * B2 - 0.5 * SB2 / (yy * yy) - UU;
+        if (k == 2) {
+          double e3 = 0.;
+          eos_energy(E, D * sqrt(1 - xx), pp, e3);
+          F[2] = yy - (pp + e3) / (1 - xx);
+        }
+      }, X, 1.e-14, 1.e-14, 300, 3);
+      xs = X[0]; ys = X[1]; pr = X[2];
+    } else {
+      double X[2] = {(en + pr) / (1 - v2), pr};
+      double xin = 0.;
+      brentm<2>([&](const double* Xk, double* F, int k) {
+        const double yy = Xk[0], pp = Xk[1];
+        const double xx = (SS2 + SB2 * (2. * yy + B2) / (yy * yy)) / ((yy + B2) * (yy + B2));
+        xin = xx;
+        if (k == 1) {
+          double e3 = 0.;
+          eos_energy(E, D * sqrt(1 - xx), pp, e3);
+          F[1] = yy - (pp + e3) / (1 - xx);
+        }
+        if (k == 0) F[0] = yy - pp + 0.5 * (1. + xx) * B2 - 0.5 * SB2 / (yy * yy) - UU;
+      }, X, 1.e-14, 1.e-14, 300, 3);
+      ys = X[0]; pr = X[1]; xs = xin;   // v^2 of the last sist4 call
+    }
+    v[KBX] = bx; v[KBY] = by; v[KBZ] = bz;
+    v[KRH] = D * sqrt(1 - xs);
+    v[KVX] = (sx + SB * bx / ys) / (ys + B2);
+    v[KVY] = (sy + SB * by / ys) / (ys + B2);
+    v[KVZ] = (sz / g3 + SB * bz / ys) / (ys + B2);
+    v[KPR] = pr;
+    if constexpr (SysT<SYS>::GLM) v[KGLM] = u[KGLM];
+    return err0;
+  }
   if (algo == 4 || algo == 5) {
     MhdPar P;
     P.S2 = sx * sx + sy * sy + sz * sz / g3;
@@ -786,7 +980,7 @@ __device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const do
                                          int algo = 5) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
-  if constexpr (SysT<SYS>::MHD) return mhd_cons2prim<SYS>(algo, M, u, v);
+  if constexpr (SysT<SYS>::MHD) return mhd_cons2prim<SYS>(E, algo, M, u, v);
   const double tol = 1.e-12, eps = 1.e-12;
   const double gm = M.gm;
   int errcode = 0;

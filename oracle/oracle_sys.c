@@ -268,6 +268,168 @@ static int solver10_approximate(double b, double c, double d, double oldsol, dou
   return iter == 100 ? 12 : 0;
 }
 
+/* ---------------------------------------------------------------------- */
+/* BRENTM: Brent's method for n nonlinear equations (ACM algorithm 554,
+ * More and Cosnard), restated from libacm554.f90:8-359 as the reference
+ * vendors it.  fcn(x, f, k) evaluates component k (0-based) only.  q is
+ * column-major with leading dimension n.  Returns INFO. */
+typedef void (*brent_fcn)(const double *x, double *f, int k);
+
+static int brentm(brent_fcn fcn, int n, double *x, double *fvec, double ftol, double xtol, int maxfev,
+                  int mopt, double *q, double *sigma, double *wa1, double *wa2) {
+  const double p05 = 5.e-2, scale = 1.e1;
+  const double epsmch = 2.220446049250313e-16;   /* EPSILON(1.d0) */
+  int info = 0, nfcall = 0, nfev = 0;
+  int nier6 = -1, nier7 = -1, nier8 = 0;
+  double fnorm = 0., difit = 0., xnorm = 0.;
+  for (int i = 0; i < n; i++) xnorm = fmax2(xnorm, fabs(x[i]));
+  const double eps = sqrt(epsmch);
+  double delta = scale * xnorm;
+  if (xnorm == 0.) delta = scale;
+  for (;;) {                                   /* label 20 */
+    int nsing = n;
+    double fnorm1 = fnorm, difit1 = difit;
+    fnorm = 0.;
+    double h = eps * xnorm;
+    if (h == 0.) h = eps;
+    for (int j = 0; j < n; j++) {
+      for (int i = 0; i < n; i++) q[i + j * n] = 0.;
+      q[j + j * n] = h;
+      wa1[j] = x[j];
+    }
+    for (int k = 0; k < n; k++) {              /* subiteration, label 150 */
+      fcn(wa1, fvec, k);
+      const double fky = fvec[k];
+      nfcall++;
+      nfev = nfcall / n;
+      fnorm = fmax2(fnorm, fabs(fky));
+      for (int j = k; j < n; j++) {
+        for (int i = 0; i < n; i++) wa2[i] = wa1[i] + q[i + j * n];
+        fcn(wa2, fvec, k);
+        const double fkz = fvec[k];
+        nfcall++;
+        nfev = nfcall / n;
+        sigma[j] = fkz - fky;
+      }
+      fvec[k] = fky;
+      double eta = 0.;
+      for (int i = k; i < n; i++) eta = fmax2(eta, fabs(sigma[i]));
+      if (eta == 0.) continue;
+      nsing = nsing - 1;
+      double sknorm = 0.;
+      for (int i = k; i < n; i++) {
+        sigma[i] = sigma[i] / eta;
+        sknorm = sknorm + sigma[i] * sigma[i];
+      }
+      sknorm = sqrt(sknorm);
+      if (sigma[k] < 0.) sknorm = -sknorm;
+      sigma[k] = sigma[k] + sknorm;
+      for (int i = 0; i < n; i++) wa2[i] = 0.;
+      for (int j = k; j < n; j++) {
+        const double temp = sigma[j];
+        for (int i = 0; i < n; i++) wa2[i] = wa2[i] + temp * q[i + j * n];
+      }
+      for (int j = k; j < n; j++) {
+        const double temp = sigma[j] / (sknorm * sigma[k]);
+        for (int i = 0; i < n; i++) q[i + j * n] = q[i + j * n] - temp * wa2[i];
+      }
+      sigma[k] = sknorm * eta;
+      double temp = fky / sigma[k];
+      if (h * fabs(temp) > delta) temp = copysign(delta / h, temp);
+      for (int i = 0; i < n; i++) wa1[i] = wa1[i] + temp * q[i + k * n];
+    }
+    xnorm = 0.;
+    difit = 0.;
+    for (int i = 0; i < n; i++) {
+      xnorm = fmax2(xnorm, fabs(wa1[i]));
+      difit = fmax2(difit, fabs(x[i] - wa1[i]));
+      x[i] = wa1[i];
+    }
+    delta = fmax2(delta, scale * xnorm);
+    const int conv = (fnorm < fnorm1 && difit < difit1 && nsing == 0);
+    nier6++;
+    nier7++;
+    nier8++;
+    if (conv) nier6 = 0;
+    if (fnorm < fnorm1 || difit < difit1) nier7 = 0;
+    if (difit > eps * xnorm) nier8 = 0;
+    if (fnorm <= ftol) info = 1;
+    if (difit <= xtol * xnorm && conv) info = 2;
+    if (fnorm <= ftol && info == 2) info = 3;
+    if (info != 0) return info;
+    if (nfev >= maxfev) info = 4;
+    if (nsing == n) info = 5;
+    if (nier6 == 5) info = 6;
+    if (nier7 == 3) info = 7;
+    if (nier8 == 4) info = 8;
+    if (info != 0) return info;
+    if (!conv || difit > p05 * xnorm || mopt == 1) continue;
+    for (int m = 2; m <= mopt; m++) {          /* iterative refinement */
+      fnorm1 = fnorm;
+      fnorm = 0.;
+      int stop = 0;
+      for (int k = 0; k < n; k++) {
+        fcn(wa1, fvec, k);
+        const double fky = fvec[k];
+        nfcall++;
+        nfev = nfcall / n;
+        fnorm = fmax2(fnorm, fabs(fky));
+        if (!(fnorm < fnorm1)) {
+          fnorm = fnorm1;
+          stop = 1;
+          break;
+        }
+        const double temp = fky / sigma[k];
+        for (int i = 0; i < n; i++) wa1[i] = wa1[i] + temp * q[i + k * n];
+      }
+      if (stop) break;
+      xnorm = 0.;
+      difit = 0.;
+      for (int i = 0; i < n; i++) {
+        xnorm = fmax2(xnorm, fabs(wa1[i]));
+        difit = fmax2(difit, fabs(x[i] - wa1[i]));
+        x[i] = wa1[i];
+      }
+      if (fnorm <= ftol) info = 1;
+      if (difit <= xtol * xnorm) info = 2;
+      if (fnorm <= ftol && info == 2) info = 3;
+      if (nfev >= maxfev && info == 0) info = 4;
+      if (info != 0) return info;
+    }
+  }
+}
+
+/* sist3 / sist4, system.f90:1199-1266: the 3x3 (v^2, W, p) and 2x2 (W, p)
+ * systems of algorithms 1 and 2.  xinner is the v^2 of the last sist4 call
+ * (a module variable in the reference, read after BRENTM returns).  An EoS
+ * error inside leaves en as it was (the reference leaves it undefined). */
+static double xinner;
+
+static void sist3(const double *X, double *F, int k) {
+  const double xx = X[0], yy = X[1], pp = X[2];
+  const double B2 = solver_par[0], SB2 = solver_par[1], S2 = solver_par[2], UU = solver_par[3], D = solver_par[4];
+  if (k == 0) F[0] = (yy + B2) * (yy + B2) * xx - SB2 / (yy * yy) * (2 * yy + B2) - S2;
+  if (k == 1) F[1] = yy - pp + 0.5 * (1. + xx) * B2 - 0.5 * SB2 / (yy * yy) - UU;
+  if (k == 2) {
+    double rh = D * sqrt(1 - xx), en = 0.;
+    eos_energy(rh, pp, &en);
+    F[2] = yy - (pp + en) / (1 - xx);
+  }
+}
+
+static void sist4(const double *X, double *F, int k) {
+  const double yy = X[0], pp = X[1];
+  const double B2 = solver_par[0], SB2 = solver_par[1], S2 = solver_par[2], UU = solver_par[3], D = solver_par[4];
+  const double xx = (S2 + SB2 * (2. * yy + B2) / (yy * yy)) / ((yy + B2) * (yy + B2));
+  xinner = xx;
+  if (k == 1) {
+    double rh = D * sqrt(1 - xx), en = 0.;
+    eos_energy(rh, pp, &en);
+    F[1] = yy - (pp + en) / (1 - xx);
+  }
+  if (k == 0) F[0] = yy - pp + 0.5 * (1. + xx) * B2 - 0.5 * SB2 / (yy * yy) - UU;
+}
+
 static int mhd_cons2prim(double *u, double *v) {
   const double eps = 1.e-12;
   const double *g = G.g_cov;
@@ -275,6 +437,45 @@ static int mhd_cons2prim(double *u, double *v) {
   const int kbx = G.kbx, kby = G.kby, kbz = G.kbz;
   double uin[MAXNV];
   int ierr = 0;
+  if (G.algo_solver == 1 || G.algo_solver == 2) {
+    /* system.f90:668-793: BRENTM on sist3 (3x3) or sist4 (2x2), warm start
+     * from the cell's previous primitives */
+    for (int k = 0; k < G.nv; k++) uin[k] = gm * u[k];
+    double DD = uin[KRH], sx = uin[KVX], sy = uin[KVY], sz = uin[KVZ], UU = uin[KPR];
+    double bx = uin[kbx], by = uin[kby], bz = uin[kbz];
+    v[kbx] = uin[kbx]; v[kby] = uin[kby]; v[kbz] = uin[kbz];
+    double B2 = bx * bx + by * by + bz * bz * g[2];
+    double SB = sx * bx + sy * by + sz * bz;
+    double SB2 = SB * SB;
+    double SS2 = sx * sx + sy * sy + sz * sz / g[2];
+    double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en = 0.;
+    /* the error code of this lookup is the routine's error code (the
+     * reference passes its errcode argument to eos_energy here) */
+    const int err0 = eos_energy(rh, pr, &en);
+    double v2 = g[0] * (vx * vx) + g[1] * (vy * vy) + g[2] * (vz * vz);
+    solver_par[0] = B2; solver_par[1] = SB2; solver_par[2] = SS2; solver_par[3] = UU; solver_par[4] = DD;
+    double q[9], sigma[3], wa1[3], wa2[3], fv[3], X[3];
+    double xs, ys;
+    if (G.algo_solver == 1) {
+      X[0] = v2;
+      X[1] = (en + pr) / (1 - v2);
+      X[2] = pr;
+      brentm(sist3, 3, X, fv, 1.e-14, 1.e-14, 300, 3, q, sigma, wa1, wa2);
+      xs = X[0]; ys = X[1]; pr = X[2];
+    } else {
+      X[0] = (en + pr) / (1 - v2);
+      X[1] = pr;
+      brentm(sist4, 2, X, fv, 1.e-14, 1.e-14, 300, 3, q, sigma, wa1, wa2);
+      ys = X[0]; pr = X[1]; xs = xinner;
+    }
+    rh = DD * sqrt(1 - xs);
+    vx = (sx + SB * bx / ys) / (ys + B2);
+    vy = (sy + SB * by / ys) / (ys + B2);
+    vz = (sz / g[2] + SB * bz / ys) / (ys + B2);
+    v[KRH] = rh; v[KVX] = vx; v[KVY] = vy; v[KVZ] = vz; v[KPR] = pr;
+    if (G.divclean) v[G.kglm] = u[G.kglm];
+    return err0;
+  }
   if (G.algo_solver == 4 || G.algo_solver == 5) {
     /* system.f90:880-998 */
     for (int k = 0; k < G.nv; k++) uin[k] = gm * u[k];
@@ -407,7 +608,7 @@ static int mhd_cons2prim(double *u, double *v) {
     if (G.divclean) v[G.kglm] = u[G.kglm];
     return 0;
   }
-  return 11;   /* algorithms 1-3 (BRENTM / LU-Newton) are not restated */
+  return 11;   /* algorithm 3 (LU-Newton) is not restated: getfun never sets Fnorm */
 }
 
 /* system_cons2prim, system.f90:423-665: ideal HD Newton on p (511-562) and

@@ -120,9 +120,10 @@ def mhd2d(n=32, eqstate=1, algo=5, divclean=1, coordinates=1, rec="WENO5", nrk=3
     return ic, rc, v
 
 
-def mhd_fireball3d(nx=16, ny=14, nz=10, algo=5, divclean=1, B0=0.3):
-    """3-D Bjorken fireball with a transverse magnetic field, p = e/3."""
-    ic, rc, v5 = fireball3d(nx, ny, nz)
+def mhd_fireball3d(nx=16, ny=14, nz=10, algo=5, divclean=1, B0=0.3, eqstate=1):
+    """3-D Bjorken fireball with a transverse magnetic field (p = e/3, or the
+    tabulated EoS with algorithms 1/2)."""
+    ic, rc, v5 = fireball3d(nx, ny, nz, eqstate=eqstate)
     ic[6] = 1
     ic[7] = divclean
     ic[12] = algo
@@ -136,4 +137,30 @@ def mhd_fireball3d(nx=16, ny=14, nz=10, algo=5, divclean=1, B0=0.3):
     v[6] = B0 * np.exp(-(X ** 2 + Y ** 2) / 8)
     v[5] = 0.2 * B0 * np.sin(Y)
     v[7] = 0.05 * B0
+    return ic, rc, v
+
+
+def mhd3d(nx=12, ny=10, nz=8, eqstate=3, algo=1, divclean=1, coordinates=2, B0=0.3):
+    """A smooth 3-D MHD state (pressure bump, rotating flow, oblique field
+    with a longitudinal modulation) on a Bjorken grid with outflow, for the
+    tabulated EoS with algorithms 1/2."""
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=coordinates, mhd=1, divclean=divclean,
+                         eqstate=eqstate, algo_solver=algo, bc=2, init_type=7,
+                         xlim=(-1.0, 1.0), ylim=(-1.0, 1.0), zlim=(-0.5, 0.5), tmin=1.0,
+                         cfl=0.4, maxdt=0.01, glm_alpha=5.0)
+    nv = 8 + (1 if divclean else 0)
+    x = cell_centres(nx, -1, 1)
+    y = cell_centres(ny, -1, 1)
+    z = cell_centres(nz, -0.5, 0.5)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r2 = X ** 2 + Y ** 2
+    v = np.zeros((nv, nz, ny, nx))
+    v[0] = 1.0 + 0.2 * np.exp(-4 * r2)
+    v[1] = -0.3 * Y * np.exp(-2 * r2)
+    v[2] = 0.3 * X * np.exp(-2 * r2)
+    v[3] = 0.05 * np.sin(2 * np.pi * Z)
+    v[4] = 1.0 + 0.5 * np.exp(-8 * r2 - 4 * Z ** 2)
+    v[5] = B0 * (1 + 0.1 * np.sin(np.pi * Y))
+    v[6] = 0.5 * B0 * (1 + 0.1 * np.cos(np.pi * X))
+    v[7] = 0.1 * B0 * (1 + 0.2 * np.cos(2 * np.pi * Z))
     return ic, rc, v

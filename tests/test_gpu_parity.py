@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from libs import Oracle
-from scenarios import fireball3d, gubser2d, mhd2d, mhd_fireball3d, rel_linf, shock2d
+from scenarios import fireball3d, gubser2d, mhd2d, mhd3d, mhd_fireball3d, rel_linf, shock2d
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -172,3 +172,20 @@ def _gpu_scalar_after(ic, rc, v0, n):
     s = g.scalars()["glm_ch_old"]
     g.close()
     return s
+
+
+@pytest.mark.parametrize("make,steps", [
+    (lambda: mhd2d(32, eqstate=3, algo=1), 10),
+    (lambda: mhd2d(32, eqstate=3, algo=2, divclean=0), 10),
+    (lambda: mhd2d(32, algo=1, coordinates=2, bc=2), 10),
+    (lambda: mhd3d(16, 14, 10, algo=1), 6),
+    (lambda: mhd3d(16, 14, 10, algo=2), 6),
+], ids=["2d-algo1-table", "2d-algo2-table-noglm", "2d-algo1-conformal-bjorken",
+        "3d-algo1-table", "3d-algo2-table"])
+def test_mhd_brentm(oracle, make, steps):
+    """MHD con2prim algorithms 1/2 (BRENTM on the 3x3 / 2x2 systems) with the
+    tabulated and conformal EoS."""
+    ic, rc, v0 = make()
+    (vg, ug), (vo, uo) = _run_both(oracle, ic, rc, v0, steps)
+    assert rel_linf(vg, vo) <= TOL
+    assert rel_linf(ug, uo) <= TOL

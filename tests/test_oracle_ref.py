@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from libs import REC_NGC, make_config
-from scenarios import fireball3d, gubser2d, mhd2d, mhd_fireball3d, shock2d
+from scenarios import fireball3d, gubser2d, mhd2d, mhd3d, mhd_fireball3d, shock2d
 
 RNG = np.random.default_rng(20261202)
 
@@ -107,6 +107,11 @@ SCENARIOS = {
     "mhd2d-bjorken-outflow": (lambda: mhd2d(24, coordinates=2, bc=2), 8),
     "mhd-fireball3d-glm": (lambda: mhd_fireball3d(12, 10, 8), 4),
     "mhd-fireball3d-noglm-rk2": (lambda: mhd_fireball3d(12, 10, 8, divclean=0), 4),
+    "mhd2d-algo1-table": (lambda: mhd2d(24, eqstate=3, algo=1), 6),
+    "mhd2d-algo2-table-noglm": (lambda: mhd2d(24, eqstate=3, algo=2, divclean=0), 6),
+    "mhd2d-algo1-conformal-bjorken": (lambda: mhd2d(24, algo=1, coordinates=2, bc=2), 6),
+    "mhd3d-algo1-table-bjorken": (lambda: mhd3d(12, 10, 8, algo=1), 5),
+    "mhd3d-algo2-table-bjorken": (lambda: mhd3d(12, 10, 8, algo=2), 5),
 }
 
 
@@ -137,7 +142,8 @@ def test_trajectory_bitexact(oracle, ref, name):
         assert np.array_equal(oracle.deriv()[:16], ref.deriv()[:16])
 
 
-@pytest.mark.parametrize("algo,eqstate,coords", [(5, 1, 2), (5, 1, 1), (6, 1, 1), (4, 2, 1)])
+@pytest.mark.parametrize("algo,eqstate,coords", [(5, 1, 2), (5, 1, 1), (6, 1, 1), (4, 2, 1),
+                                                 (1, 3, 1), (1, 3, 2), (2, 3, 1), (1, 1, 2), (2, 1, 1)])
 @pytest.mark.parametrize("divclean", [0, 1])
 def test_pointwise_mhd_bitexact(oracle, ref, algo, eqstate, coords, divclean):
     ic, rc = make_config(mhd=1, divclean=divclean, algo_solver=algo, eqstate=eqstate,
