@@ -138,30 +138,17 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
 
     const DevScalars* S = a.S;
     const Metric M = make_metric(a.coordinates, S->t);
-    double wl[NV], wr[NV], c0[NV];
-    double vel[3][6];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s[6];
-      const double* Vk = V + (long long)k * a.N;
-#pragma unroll
-      for (int j = 0; j < 6; j++) s[j] = Vk[off[j]];
-      rec_face<REC>(s, wl[k], wr[k]);
-      c0[k] = s[2];
-      if constexpr (VISC) {
-        if (k >= KVX && k <= KVZ) {
-#pragma unroll
-          for (int j = 0; j < 6; j++) vel[k - KVX][j] = s[j];
-        }
-      }
-    }
     if constexpr (VISC) {
-      // Lorentz-velocity reconstruction feeding deriv, evolve.f90:324-331,371-374
+      // Lorentz-velocity reconstruction feeding deriv, evolve.f90:324-331,
+      // 371-374; done first so that its 42 temporaries are dead before the
+      // nv reconstructions below
       double wd[4][6];
 #pragma unroll
       for (int j = 0; j < 6; j++) {
-        double vx = vel[0][j], vy = vel[1][j], vz = vel[2][j];
-        double gl = 1. / sqrt(1. - vx * vx - vy * vy - M.g3 * (vz * vz));
+        const double vx = V[(long long)KVX * a.N + off[j]];
+        const double vy = V[(long long)KVY * a.N + off[j]];
+        const double vz = V[(long long)KVZ * a.N + off[j]];
+        const double gl = 1. / sqrt(1. - vx * vx - vy * vy - M.g3 * (vz * vz));
         wd[0][j] = gl; wd[1][j] = gl * vx; wd[2][j] = gl * vy; wd[3][j] = gl * vz;
       }
       const long long nfl = nf;
@@ -171,6 +158,17 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
         rec_face<REC>(wd[q], l, r);
         L[q * nfl + f] = l + r;
       }
+    }
+    // c0: the cell left of the interface, used by the repair (KVX..KPR)
+    double wl[NV], wr[NV], c0[5];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s[6];
+      const double* Vk = V + (long long)k * a.N;
+#pragma unroll
+      for (int j = 0; j < 6; j++) s[j] = Vk[off[j]];
+      rec_face<REC>(s, wl[k], wr[k]);
+      if (k <= KPR) c0[k] = s[2];
     }
     // repair of reconstructed states with the cell left of the interface
     // (evolve.f90:334-369; the reference uses w0(ix) for both sides)

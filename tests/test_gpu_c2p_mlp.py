@@ -176,17 +176,19 @@ def test_inversion(om, oracle, coords, t, divclean):
     g.cons2prim_batch_dev(du.ptr, dvp.ptr, dsp.ptr, n)
     vp = dvp.numpy()
     sc = _scales(vtc)
-    err = np.abs(v[:8] - vtc) / sc
-    err_plain = np.abs(vp[:8] - vtc) / sc
+    err = (np.abs(v[:8] - vtc) / sc).max(0)
+    err_plain = (np.abs(vp[:8] - vtc) / sc).max(0)
+    # fallback states are the plain solver's output, bit for bit; the rest
+    # are the learned inversion's
+    same = np.all(v == vp, axis=0)
+    acc = ~same
     print(f"\ninversion coords={coords} t={t} divclean={divclean}: fallback {nfb}/{n}, "
-          f"max err vs exact {err.max():.2e} (plain solver {err_plain.max():.2e})")
-    assert nfb < 0.05 * n
-    assert err.max() <= 1e-10
+          f"max err vs exact: learned {err[acc].max():.2e}, plain solver {err_plain.max():.2e}")
+    assert same.sum() >= nfb
+    assert nfb < 0.01 * n
+    assert err[acc].max() <= 1e-10
     if divclean:
         assert np.array_equal(v[8], u[8])
-    # identify fallback states: equal to the plain solver bit for bit
-    same = np.all(v == vp, axis=0)
-    assert same.sum() >= nfb
     # oracle refinement of the GPU network output on a sample of the states
     if coords == 1 and not divclean:
         dy = _zeros((5, n), np.float32)
