@@ -476,11 +476,78 @@ int echo_download_state(echo_grid* g, double* v, double* u) {
   return 0;
 }
 
+static int run_init(echo_grid* g);
+
 int echo_init_from_prim(echo_grid* g, const double* v) {
   if (!g || !v) return fail(ECHO_E_ARG, "null argument");
   HIPCHECK(hipSetDevice(g->dev));
   const size_t sz = sizeof(double) * (size_t)g->nv * g->N;
   HIPCHECK(hipMemcpyAsync(g->V, v, sz, hipMemcpyHostToDevice, g->stream));
+  return run_init(g);
+}
+
+int echo_init_glauber(echo_grid* g, const echo_glauber_params* p, const double* part_a, int32_t n_a,
+                      const double* part_b, int32_t n_b, const double* coll, int32_t n_coll) {
+  if (!g || !p) return fail(ECHO_E_ARG, "null argument");
+  if (n_a < 0 || n_b < 0 || n_coll < 0 || (n_a && !part_a) || (n_b && !part_b) || (n_coll && !coll))
+    return fail(ECHO_E_ARG, "invalid source arrays");
+  const int ns = n_a + n_b + n_coll;
+  if (ns < 1 || ns > GLAUBER_MAX_SRC) return fail(ECHO_E_ARG, "1..%d sources supported, got %d", GLAUBER_MAX_SRC, ns);
+  if (!(p->sig_mc > 0.) || !(p->ybeam > 0.)) return fail(ECHO_E_ARG, "need sig_mc > 0 and ybeam > 0");
+  if (g->cfg.mhd && g->cfg.viscosity) return fail(ECHO_E_ARG, "unsupported system");
+  HIPCHECK(hipSetDevice(g->dev));
+  // centre of mass of the sources, in the reference's summation order
+  // (glaubermc.f90:806-821)
+  double cmx_p = 0., cmy_p = 0., cmx_c = 0., cmy_c = 0.;
+  for (int i = 0; i < n_a; i++) { cmx_p = cmx_p + part_a[2 * i]; cmy_p = cmy_p + part_a[2 * i + 1]; }
+  for (int i = 0; i < n_b; i++) { cmx_p = cmx_p + part_b[2 * i]; cmy_p = cmy_p + part_b[2 * i + 1]; }
+  for (int i = 0; i < n_coll; i++) { cmx_c = cmx_c + coll[2 * i]; cmy_c = cmy_c + coll[2 * i + 1]; }
+  const double ah = p->ah;
+  const double den = (1 - ah) * (n_a + n_b) + ah * n_coll;
+  GlauberArgs q;
+  q.kappa = p->kappa; q.ah = ah; q.s2 = p->sig_mc * p->sig_mc; q.ybeam = p->ybeam; q.deta = p->deta;
+  q.sigeta = p->sigeta; q.rhocenter = p->rhocenter;
+  q.cmx = ((1 - ah) * cmx_p + ah * cmx_c) / den;
+  q.cmy = ((1 - ah) * cmy_p + ah * cmy_c) / den;
+  q.na = n_a; q.nb = n_b; q.nc = n_coll;
+  std::vector<double> src(2 * (size_t)ns);
+  if (n_a) memcpy(src.data(), part_a, sizeof(double) * 2 * n_a);
+  if (n_b) memcpy(src.data() + 2 * n_a, part_b, sizeof(double) * 2 * n_b);
+  if (n_coll) memcpy(src.data() + 2 * (n_a + n_b), coll, sizeof(double) * 2 * n_coll);
+  double *dsrc = nullptr, *dsum = nullptr;
+  const long long n2 = (long long)g->cfg.nx * g->cfg.ny;
+  if (hipMalloc(&dsrc, sizeof(double) * src.size()) != hipSuccess ||
+      hipMalloc(&dsum, sizeof(double) * 3 * n2) != hipSuccess) {
+    hipFree(dsrc);
+    return fail(ECHO_E_NOMEM, "source buffers");
+  }
+  const echo_config& c = g->cfg;
+  const double dx = (c.x2 - c.x1) / c.nx, dy = (c.y2 - c.y1) / c.ny, dz = (c.z2 - c.z1) / c.nz;
+  int zero = 0;
+  hipError_t e = hipMemcpyAsync(dsrc, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->status, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) {
+    k_glauber_sums<<<nblocks(n2, GLAUBER_BLOCK), GLAUBER_BLOCK, 0, g->stream>>>(g->ga, q, c.x1, dx, c.y1, dy,
+                                                                                dsrc, dsum);
+    dispatch_sys(g->cfg, [&](auto sys, auto) {
+      k_glauber_prim<decltype(sys)::value><<<nblocks(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+          g->ga, q, c.z1, dz, c.tmin, c.enezero, acos(-1.), dsum, g->V, g->status);
+    });
+    e = hipGetLastError();
+  }
+  int st = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&st, g->status, sizeof(int), hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  hipFree(dsrc);
+  hipFree(dsum);
+  if (e != hipSuccess) return fail(ECHO_E_HIP, "glauber initial state: %s", hipGetErrorString(e));
+  if (st) return st;
+  return run_init(g);
+}
+
+// prim2cons (+ con2prim for non-viscous systems) of the primitives in V,
+// as the reference initialisers do (init.f90:2937-2958)
+static int run_init(echo_grid* g) {
   int zero = 0;
   HIPCHECK(hipMemcpyAsync(g->status, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHECK(hipMemcpyAsync(&g->S->vs_parity, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));

@@ -578,6 +578,99 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_init(GridArgs a, double* __res
   if (VISC) { VS0[c] = v[KVX]; VS0[N + c] = v[KVY]; VS0[2 * N + c] = v[KVZ]; }
 }
 
+// ---------------------------------------------------------------------------
+// Glauber-MC initial state.  k_glauber_sums: per transverse cell, the sums
+// of the Gaussian sources of the participants of nuclei A and B and of the
+// binary collisions around the centre of mass (glaubermc.f90:822-842); the
+// sources are staged in LDS.  k_glauber_prim: per cell, the energy density
+// (those sums with the (ybeam +- eta)/ybeam weights, kappa, ah and the
+// longitudinal profile feta, glaubermc.f90:661-682,838), the primitives of
+// the Glauber-MC initialisation (init.f90:2662-2730) and, for viscous runs,
+// the Navier-Stokes stresses of viscous_initio (viscous.f90:101-167).
+// ---------------------------------------------------------------------------
+struct GlauberArgs {
+  double kappa, ah, s2, ybeam, deta, sigeta, rhocenter, cmx, cmy;
+  int na, nb, nc;
+};
+
+constexpr int GLAUBER_BLOCK = 256;
+constexpr int GLAUBER_MAX_SRC = 4096;   // LDS: 64 KiB of (x, y) pairs
+
+static __global__ __launch_bounds__(GLAUBER_BLOCK) void k_glauber_sums(GridArgs a, GlauberArgs q, double x1,
+                                                               double dx, double y1, double dy,
+                                                               const double* __restrict__ src,
+                                                               double* __restrict__ sums) {
+  __shared__ double2 s[GLAUBER_MAX_SRC];
+  const int ns = q.na + q.nb + q.nc;
+  for (int i = threadIdx.x; i < ns; i += GLAUBER_BLOCK) s[i] = make_double2(src[2 * i], src[2 * i + 1]);
+  __syncthreads();
+  const int c = blockIdx.x * GLAUBER_BLOCK + threadIdx.x;
+  if (c >= a.nx * a.ny) return;
+  const double x = x1 + ((double)(c % a.nx + 1) - 0.5) * dx;
+  const double y = y1 + ((double)(c / a.nx + 1) - 0.5) * dy;
+  double e[3] = {0., 0., 0.};
+  int i = 0;
+#pragma unroll
+  for (int part = 0; part < 3; part++) {
+    const int end = i + (part == 0 ? q.na : part == 1 ? q.nb : q.nc);
+    double acc = 0.;
+    for (; i < end; i++) {
+      const double u = x - s[i].x + q.cmx, w = y - s[i].y + q.cmy;
+      acc = acc + exp(-(u * u + w * w) / 2. / q.s2);
+    }
+    e[part] = acc;
+  }
+  const long long n2 = (long long)a.nx * a.ny;
+  sums[c] = e[0];
+  sums[n2 + c] = e[1];
+  sums[2 * n2 + c] = e[2];
+}
+
+template <int SYS>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_glauber_prim(GridArgs a, GlauberArgs q, double z1, double dz,
+                                                               double tau0, double enezero, double pi,
+                                                               const double* __restrict__ sums,
+                                                               double* __restrict__ V, int* __restrict__ status) {
+  constexpr int NV = SysT<SYS>::NV;
+  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N) return;
+  const long long n2 = (long long)a.nx * a.ny;
+  const long long t = c % n2;
+  const int iz = (int)(c / n2);
+  const double z = z1 + ((double)(iz + 1) - 0.5) * dz;
+  const double ea = sums[t], eb = sums[n2 + t], ec = sums[2 * n2 + t];
+  const double ep = ea * (q.ybeam + z) / q.ybeam + eb * (q.ybeam - z) / q.ybeam;
+  double fe;
+  const double half = q.deta / 2.;
+  if (fabs(z) >= q.ybeam) fe = 0.;
+  else if (fabs(z) <= half) fe = 1.;
+  else {
+    const double d = fabs(z) - half;
+    fe = exp(-(d * d) / (2. * (q.sigeta * q.sigeta)));
+  }
+  const double e = q.kappa / (2. * pi * q.s2) * (q.ah * ec + (1 - q.ah) * ep) * fe;
+  double pr, d1, d2;
+  int err = eos_pressure(a.eos, q.rhocenter, e + enezero, pr, d1, d2);
+#pragma unroll
+  for (int k = 0; k < NV; k++) V[k * a.N + c] = 0.;
+  V[KRH * a.N + c] = 1.;
+  V[KPR * a.N + c] = pr;
+  if constexpr (SysT<SYS>::VISC) {
+    double en, temp, s, cs2;
+    if (err == 0) err = get_derived_data(a.eos, 1., pr, en, temp, s);
+    if (err == 0) err = eos_sound(a.eos, 1., en, pr, cs2);
+    const double hbar = 0.197326;   // common.f90:51
+    const double eta = hbar * a.vp.eta_over_s * s;
+    const double zeta = a.vp.bulkvis ? 2. * eta * (1. / 3. - cs2) : 0.;
+    const double g3 = a.coordinates == BJORKEN ? tau0 * tau0 : 1.;
+    V[KPIZZ * a.N + c] = -g3 * 4. * eta / (3. * tau0);
+    V[KPIXX * a.N + c] = 2. * eta / (3. * tau0);
+    V[KPIYY * a.N + c] = 2. * eta / (3. * tau0);
+    V[KPIBU * a.N + c] = -zeta / tau0;
+  }
+  if (err) atomicMax(status, err);
+}
+
 // batched con2prim over n independent states (system_cons2prim)
 template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_c2p_batch(GridArgs a, const double* __restrict__ Ub,

@@ -33,6 +33,7 @@ EXPORTS = (
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
+    "echo_init_glauber",
 )
 
 MLP_NPARAM = 9221
@@ -63,6 +64,12 @@ class EchoConfig(C.Structure):
         rc = np.ascontiguousarray(rc, dtype=np.float64)
         assert ic.size == 16 and rc.size == 19
         return cls.from_buffer_copy(ic.tobytes() + rc.tobytes())
+
+
+class GlauberParams(C.Structure):
+    """Mirror of struct echo_glauber_params."""
+    _fields_ = [(n, C.c_double) for n in ("kappa", "ah", "sig_mc", "ybeam", "deta", "sigeta",
+                                          "rhocenter")]
 
 
 class TrainConfig(C.Structure):
@@ -123,6 +130,8 @@ def load_library(path=LIB_PATH):
         "echo_ensemble_tune": (C.c_int, [P, C.c_int32]),
         "echo_ensemble_steps": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32)]),
         "echo_ensemble_last_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "echo_init_glauber": (C.c_int, [P, C.POINTER(GlauberParams), _dp, C.c_int32, _dp,
+                                        C.c_int32, _dp, C.c_int32]),
         "echo_dev_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
         "echo_dev_free": (C.c_int, [C.c_int32, P]),
         "echo_dev_upload": (C.c_int, [C.c_int32, P, P, C.c_int64]),
@@ -262,6 +271,17 @@ class EchoGrid:
         v = np.ascontiguousarray(v, dtype=np.float64)
         assert v.shape == self.shape, (v.shape, self.shape)
         return self._check(self.lib.echo_init_from_prim(self.h, v.ravel()))
+
+    def init_glauber(self, params, part_a, part_b, coll):
+        """Glauber-MC initial state on the device.  params: dict of
+        GlauberParams fields; part_a, part_b, coll: (n, 2) arrays of (x, y)."""
+        gp = GlauberParams(**params)
+        arrs = [np.ascontiguousarray(np.reshape(a, (-1, 2)), dtype=np.float64) for a in
+                (part_a, part_b, coll)]
+        flat = [a.ravel() if a.size else np.zeros(2) for a in arrs]
+        return self._check(self.lib.echo_init_glauber(self.h, C.byref(gp), flat[0], arrs[0].shape[0],
+                                                      flat[1], arrs[1].shape[0], flat[2],
+                                                      arrs[2].shape[0]))
 
     def upload(self, v, u):
         v = np.ascontiguousarray(v, dtype=np.float64)

@@ -91,6 +91,27 @@ int echo_nv(const echo_grid *g);
  * v: nv*nx*ny*nz doubles (Fortran layout), host memory. */
 int echo_init_from_prim(echo_grid *g, const double *v);
 
+/* Glauber Monte-Carlo initial state computed on the device: the energy
+ * deposition of generate_energy_density_profile (glaubermc.f90:686-842,
+ * nucleus-nucleus branch) from given participant and collision positions,
+ * the primitives of the Glauber-MC initialisation (init.f90:2662-2730:
+ * rho = 1, v = 0, p = eos_pressure(rhocenter, e + enezero)), for viscous
+ * runs the Navier-Stokes stresses of viscous_initio (viscous.f90:101-136),
+ * then prim2cons (+ con2prim) as echo_init_from_prim.  part_a, part_b, coll:
+ * (x, y) pairs in fm (host memory), at most 4096 sources in total; the
+ * random sampling of generate_events stays with the caller. */
+typedef struct echo_glauber_params {
+  double kappa;       /* [KAPPA] */
+  double ah;          /* [AH] weight of the binary collisions */
+  double sig_mc;      /* [SIG] Gaussian width of a source (fm) */
+  double ybeam;       /* beam rapidity log(sqrt(s_NN)/m_N), glaubermc.f90:741 */
+  double deta;        /* [DETA] width of the eta plateau */
+  double sigeta;      /* [SIGETA] Gaussian width of the eta tails */
+  double rhocenter;   /* [RHOCENTER] */
+} echo_glauber_params;
+int echo_init_glauber(echo_grid *g, const echo_glauber_params *p, const double *part_a, int32_t n_a,
+                      const double *part_b, int32_t n_b, const double *coll, int32_t n_coll);
+
 /* Upload / download primitive and conserved state (host memory, Fortran
  * layout).  Replaces the reference's direct access to module arrays v, u
  * (common.f90:203-214). */

@@ -57,6 +57,10 @@ int  orc_eos_derived(double rh, double pr, double *out3);
 int  orc_eos_table(int n, double *t, double *e, double *p, double *c);
 double orc_przero(void);
 
+/* Glauber-MC initial primitives (oracle_init.c) */
+int orc_glauber_prim(const double *par, const double *pa, int na, const double *pb, int nb,
+                     const double *pc, int nc, double *vout);
+
 #ifdef __cplusplus
 }
 #endif

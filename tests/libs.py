@@ -179,6 +179,21 @@ class Oracle(_Lib):
         return self._csetup(ic, rc, EOS_TABLE.encode())
 
 
+def oracle_glauber_prim(oracle, params, part_a, part_b, coll):
+    """Primitive Glauber-MC state of the oracle's current grid (oracle_init.c)."""
+    f = oracle.lib.orc_glauber_prim
+    f.restype = C.c_int
+    f.argtypes = [_dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
+    par = np.array([params[k] for k in ("kappa", "ah", "sig_mc", "ybeam", "deta", "sigeta",
+                                        "rhocenter")], dtype=np.float64)
+    arrs = [np.ascontiguousarray(np.reshape(a, (-1, 2)), dtype=np.float64) for a in (part_a, part_b, coll)]
+    flat = [a.ravel() if a.size else np.zeros(2) for a in arrs]
+    n = oracle.nx * oracle.ny * oracle.nz
+    v = np.zeros(oracle.nv * n)
+    e = f(par, flat[0], arrs[0].shape[0], flat[1], arrs[1].shape[0], flat[2], arrs[2].shape[0], v)
+    return e, v.reshape(oracle.nv, oracle.nz, oracle.ny, oracle.nx)
+
+
 class Ref(_Lib):
     prefix = "ref_"
 
