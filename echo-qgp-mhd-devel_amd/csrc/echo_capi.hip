@@ -169,7 +169,15 @@ static void enqueue_end_step(echo_grid* g) {
   k_end_step<<<1, 64, 0, g->stream>>>(g->S, g->cfg.coordinates, g->cfg.viscosity ? 1 : 0);
 }
 
+static void enqueue_check(echo_grid* g) {
+  if (!g->stop_check) return;
+  const unsigned nb = nblocks(g->N, CHECK_BLOCK);
+  k_check_temp<<<nb, CHECK_BLOCK, 0, g->stream>>>(g->ga, g->V, g->partial);
+  k_check_decide<<<1, 1024, 0, g->stream>>>(g->S, g->partial, (int)nb);
+}
+
 static void enqueue_step(echo_grid* g) {
+  enqueue_check(g);
   for (int irk = 1; irk <= g->cfg.nrk; irk++) enqueue_stage(g, irk);
   enqueue_end_step(g);
 }
@@ -260,7 +268,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
   g->nf[2] = (long long)c.nx * c.ny * (c.nz + 1);
   g->active[0] = c.nx > 1; g->active[1] = c.ny > 1; g->active[2] = c.nz > 1;
-  long long nb_cap = 1;
+  long long nb_cap = 1 + (long long)nblocks(g->N, CHECK_BLOCK);
   for (int d = 0; d < 3; d++) {
     g->nb[d] = (int)flux_blocks(c, d, 1);
     unsigned mx = 0;
@@ -576,6 +584,7 @@ int echo_rk_stage(echo_grid* g, int32_t irk) {
   if (irk != g->next_stage || irk < 1 || irk > g->cfg.nrk)
     return fail(ECHO_E_ARG, "stages must be called in order 1..nrk (expected %d)", g->next_stage);
   HIPCHECK(hipSetDevice(g->dev));
+  if (irk == 1) enqueue_check(g);
   enqueue_stage(g, irk);
   HIPCHECK(hipGetLastError());
   g->next_stage = irk + 1;
@@ -601,6 +610,7 @@ int echo_steps(echo_grid* g, int32_t nsteps) {
     double acc[4] = {0, 0, 0, 0};
     int nst = 0;
     for (int s = 0; s < nsteps; s++) {
+      enqueue_check(g);
       for (int irk = 1; irk <= g->cfg.nrk; irk++) {
         enqueue_stage(g, irk);
         HIPCHECK(hipEventSynchronize(g->ev[5]));
@@ -673,6 +683,33 @@ int echo_cons2prim_batch(echo_grid* g, const double* u, double* v, int32_t* stat
   HIPCHECK(hipMemcpyAsync(status, ds, sizeof(int) * n, hipMemcpyDeviceToHost, g->stream));
   HIPCHECK(hipStreamSynchronize(g->stream));
   hipFree(du); hipFree(dv); hipFree(ds);
+  return 0;
+}
+
+int echo_set_stop(echo_grid* g, double temp_end, int32_t freeze_type) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (freeze_type != 0 && freeze_type != 1) return fail(ECHO_E_ARG, "freeze_type must be 0 or 1");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_stop inside a step");
+  HIPCHECK(hipSetDevice(g->dev));
+  int zero = 0;
+  HIPCHECK(hipMemcpyAsync(&g->S->temp_end, &temp_end, sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(&g->S->freeze_type, &freeze_type, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(&g->S->temp_check, &zero, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  g->stop_check = temp_end > 0.;
+  drop_graph(g);
+  if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
+int echo_get_stop(echo_grid* g, int32_t* stopped, double* max_value) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  if (stopped) *stopped = S.temp_check;
+  if (max_value) *max_value = S.check_max;
   return 0;
 }
 

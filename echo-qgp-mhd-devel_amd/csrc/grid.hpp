@@ -68,6 +68,7 @@ struct echo_grid {
   bool owns_stream = true;
   long long partial_cap = 0;  // doubles in partial
   bool* graph_dirty = nullptr; // set when the launch shape changes (ensemble graph)
+  bool stop_check = false;      // check_temp at the start of each step
 };
 
 

@@ -19,10 +19,15 @@ namespace echo {
 
 struct DevScalars {
   double t, dt, timeold, timeinterval, g_cov_3_old, tout, glm_ch, glm_ch_old;
-  double smooth_pressure, dtout, tmax, pad;
-  double aflux[3], pad1;   // max(ap, am) at the last interface of each sweep
+  double smooth_pressure, dtout, tmax;
+  double temp_end;         // check_temp threshold (<= 0: no check)
+  double aflux[3];         // max(ap, am) at the last interface of each sweep
+  double check_max;        // max temperature (or energy density) of the last check
   int iter, run_crashed, errcode, nfail;
-  int vs_parity, pad2, pad3, pad4;
+  int vs_parity;
+  int freeze_type;         // 0: temperature, 1: energy density
+  int temp_check;          // 1 once the check has stopped the run
+  int pad;
 };
 
 struct GridArgs {
@@ -94,6 +99,7 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   static_assert(TY == 1 || DIR != 1, "x sweeps use TY = 1");
+  if (a.S->temp_check) return;   // the run has stopped (check_temp)
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int m = DIR == 1 ? nx : (DIR == 2 ? ny : nz);
   const long long nf = DIR == 1 ? (long long)(nx + 1) * ny * nz
@@ -260,6 +266,7 @@ __global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* _
 // ---------------------------------------------------------------------------
 static __global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* __restrict__ partial,
                                               int nb0, int nb1, int nb2) {
+  if (a.S->temp_check) return;
   __shared__ double red[3][16];
   double m[3] = {0., 0., 0.};
   const int nb[3] = {nb0, nb1, nb2};
@@ -370,7 +377,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
-  if (c >= a.N) return;
+  if (c >= a.N || a.S->temp_check) return;
   DevScalars* S = a.S;
   if (S->run_crashed) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -476,6 +483,7 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
   constexpr int NV = SysT<SYS>::NV;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevScalars* S = a.S;
+  if (S->temp_check) return;
   // glm_ch_old = maxval(aflux) of this stage; the next stage starts with
   // glm_ch = glm_ch_old (evolve.f90:157-163)
   S->glm_ch_old = dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
@@ -542,7 +550,7 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
 // vnewstep -> vold, tout cadence)
 static __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (S->run_crashed) return;
+  if (S->run_crashed || S->temp_check) return;
   S->timeold = S->t;
   S->t = S->t + S->dt;
   // g_cov_3_old = g_cov(3) of the step just taken (echo.f90:213)
@@ -553,6 +561,54 @@ static __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
   if ((S->t > S->tout) || ((S->tout - S->t) < 1.e-10)) {
     S->tout = S->tout + S->dtout;
     if ((fabs(S->tout - S->tmax) < 1.e-10) || (S->tout > S->tmax)) S->tout = S->tmax;
+  }
+}
+
+// check_temp (out.f90:1487-1557), run at the start of each step
+// (echo.f90:189-190): the maximum over the cells of the temperature
+// (freeze_type 0) or energy density (1) from get_derived_temp_for_check
+// (eos.f90:706-759; the same interpolation as get_derived_data).
+// k_check_temp writes one maximum per block, k_check_decide reduces them and
+// sets temp_check when the maximum is below temp_end; every later kernel of
+// the run then returns at once, as the reference leaves its time loop.
+constexpr int CHECK_BLOCK = 256;
+
+static __global__ __launch_bounds__(CHECK_BLOCK) void k_check_temp(GridArgs a, const double* __restrict__ V,
+                                                                   double* __restrict__ partial) {
+  double m = 0.;
+  const long long c = (long long)blockIdx.x * CHECK_BLOCK + threadIdx.x;
+  if (c < a.N && !a.S->temp_check) {
+    double en, temp, s;
+    get_derived_data(a.eos, V[c], V[KPR * a.N + c], en, temp, s);
+    m = a.S->freeze_type == 0 ? temp : en;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = dmax(m, __shfl_xor(m, o, 64));
+  __shared__ double red[CHECK_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < CHECK_BLOCK / 64; w++) r = dmax(r, red[w]);
+    partial[blockIdx.x] = r;
+  }
+}
+
+static __global__ __launch_bounds__(1024) void k_check_decide(DevScalars* S, const double* __restrict__ partial,
+                                                              int nb) {
+  if (S->temp_check) return;
+  __shared__ double red[16];
+  double m = 0.;
+  for (int i = threadIdx.x; i < nb; i += 1024) m = dmax(m, partial[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = dmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 16; w++) r = dmax(r, red[w]);
+    S->check_max = r;
+    if (r < S->temp_end) S->temp_check = 1;
   }
 }
 

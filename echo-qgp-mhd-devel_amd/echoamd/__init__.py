@@ -33,7 +33,7 @@ EXPORTS = (
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
-    "echo_init_glauber",
+    "echo_init_glauber", "echo_set_stop", "echo_get_stop",
 )
 
 MLP_NPARAM = 9221
@@ -132,6 +132,8 @@ def load_library(path=LIB_PATH):
         "echo_ensemble_last_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
         "echo_init_glauber": (C.c_int, [P, C.POINTER(GlauberParams), _dp, C.c_int32, _dp,
                                         C.c_int32, _dp, C.c_int32]),
+        "echo_set_stop": (C.c_int, [P, C.c_double, C.c_int32]),
+        "echo_get_stop": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "echo_dev_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
         "echo_dev_free": (C.c_int, [C.c_int32, P]),
         "echo_dev_upload": (C.c_int, [C.c_int32, P, P, C.c_int64]),
@@ -271,6 +273,16 @@ class EchoGrid:
         v = np.ascontiguousarray(v, dtype=np.float64)
         assert v.shape == self.shape, (v.shape, self.shape)
         return self._check(self.lib.echo_init_from_prim(self.h, v.ravel()))
+
+    def set_stop(self, temp_end, freeze_type=0):
+        """check_temp stop criterion (temp_end <= 0 disables it)."""
+        self._check(self.lib.echo_set_stop(self.h, float(temp_end), int(freeze_type)))
+
+    def stop_state(self):
+        """(stopped, maximum of the last check)."""
+        st, mx = C.c_int32(), C.c_double()
+        self._check(self.lib.echo_get_stop(self.h, C.byref(st), C.byref(mx)))
+        return st.value, mx.value
 
     def init_glauber(self, params, part_a, part_b, coll):
         """Glauber-MC initial state on the device.  params: dict of

@@ -139,6 +139,16 @@ int echo_get_scalars(echo_grid *g, double *s);
  * evolve.f90:179-188 use it. */
 int echo_set_tout(echo_grid *g, double tout);
 
+/* Stop criterion check_temp (out.f90:1487-1557; echo.f90:189-190): with
+ * temp_end > 0, every step starts by taking the maximum over the cells of
+ * the temperature (freeze_type 0) or the energy density (freeze_type 1, where
+ * temp_end is an energy density); below temp_end the run stops: that step and
+ * all later ones do nothing (t, iter and the state stay), as the reference
+ * leaves its time loop.  temp_end <= 0 disables the check (the default).
+ * echo_get_stop: stopped flag and the maximum of the last check. */
+int echo_set_stop(echo_grid *g, double temp_end, int32_t freeze_type);
+int echo_get_stop(echo_grid *g, int32_t *stopped, double *max_value);
+
 /* Batched con2prim: system_cons2prim (system.f90:423) on n independent
  * states at the grid's current metric.  u: nv*n conserved (variable-major,
  * u[k*n + i]); v: nv*n primitives in/out (in: guess, out: solution);

@@ -56,6 +56,8 @@ int  orc_eos_sound(double rh, double en, double pr, double *cs2);
 int  orc_eos_derived(double rh, double pr, double *out3);
 int  orc_eos_table(int n, double *t, double *e, double *p, double *c);
 double orc_przero(void);
+void orc_set_stop(double temp_end, int freeze_type);
+void orc_get_stop(int *stopped, double *max_value);
 
 /* Glauber-MC initial primitives (oracle_init.c) */
 int orc_glauber_prim(const double *par, const double *pa, int na, const double *pb, int nb,

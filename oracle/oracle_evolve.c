@@ -105,6 +105,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.tout = G.t + G.dtout;
   G.iter = 0; G.run_crashed = 0; G.errcode = 0;
   G.smooth_first = 1;
+  G.temp_end = 0.; G.check_max = 0.; G.freeze_type = 0; G.temp_check = 0;
   return G.nv;
 }
 
@@ -556,9 +557,44 @@ static int evolve_main(int irk) {
 }
 
 /* main-loop body of echo.f90:185-262 without output */
+/* check_temp, out.f90:1487-1557: the maximum over the cells of the
+ * temperature (freeze_type 0) or energy density (1) from
+ * get_derived_temp_for_check (eos.f90:706-759, the interpolation of
+ * get_derived_data); temp_check = 1 below temp_end. */
+static void check_temp(void) {
+  double mx = 0.;
+  for (int iz = 0; iz < G.nz; iz++)
+    for (int iy = 0; iy < G.ny; iy++)
+      for (int ix = 0; ix < G.nx; ix++) {
+        double en = 0., temp = 0., s;
+        get_derived_data(V(ix, iy, iz, KRH), V(ix, iy, iz, KPR), &en, &temp, &s);
+        mx = fmax2(mx, G.freeze_type == 0 ? temp : en);
+      }
+  G.check_max = mx;
+  G.temp_check = mx < G.temp_end ? 1 : 0;
+}
+
+void orc_set_stop(double temp_end, int freeze_type) {
+  G.temp_end = temp_end;
+  G.freeze_type = freeze_type;
+  G.temp_check = 0;
+  G.check_max = 0.;
+}
+
+void orc_get_stop(int *stopped, double *max_value) {
+  *stopped = G.temp_check;
+  *max_value = G.check_max;
+}
+
 int orc_steps(int nsteps) {
   const size_t nc = ncell();
   for (int s = 0; s < nsteps; s++) {
+    /* echo.f90:189-190: the check opens each pass of the time loop */
+    if (G.temp_check) break;
+    if (G.temp_end > 0) {
+      check_temp();
+      if (G.temp_check) break;
+    }
     memcpy(G.vnewstep, G.v, 5 * nc * sizeof(double));
     for (int irk = 1; irk <= G.nrk; irk++) {
       int e = evolve_main(irk);

@@ -45,6 +45,8 @@ typedef struct {
   int entries;
   double *tab_t, *tab_e, *tab_p, *tab_c;
   double smooth_pressure; int smooth_first;
+  /* check_temp stop criterion */
+  double temp_end, check_max; int freeze_type, temp_check;
   /* fields */
   double *v, *u, *u0, *du, *du_stiff, *deriv, *vnewstep, *vold;
   double *ax, *ay, *az;

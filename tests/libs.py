@@ -179,6 +179,22 @@ class Oracle(_Lib):
         return self._csetup(ic, rc, EOS_TABLE.encode())
 
 
+def oracle_set_stop(oracle, temp_end, freeze_type=0):
+    f = oracle.lib.orc_set_stop
+    f.restype = None
+    f.argtypes = [C.c_double, C.c_int]
+    f(temp_end, freeze_type)
+
+
+def oracle_get_stop(oracle):
+    f = oracle.lib.orc_get_stop
+    f.restype = None
+    f.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    st, mx = C.c_int(), C.c_double()
+    f(C.byref(st), C.byref(mx))
+    return st.value, mx.value
+
+
 def oracle_glauber_prim(oracle, params, part_a, part_b, coll):
     """Primitive Glauber-MC state of the oracle's current grid (oracle_init.c)."""
     f = oracle.lib.orc_glauber_prim
