@@ -27,9 +27,10 @@ fi
 FFLAGS="-fdefault-real-8 -fdefault-double-8 -O2 -fPIC -ffp-contract=off"
 mkdir -p "$OUT"
 cd "$OUT"
-for m in parallel_nompi holib eos common viscous system work evolve; do
+for m in parallel_nompi holib eos common viscous system work evolve hypersurface out; do
   "$FC" $FFLAGS -I"$REF" -c "$REF/$m.f90" -o "$m.o" 2> "$m.log" || { cat "$m.log"; exit 1; }
 done
 "$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o ref_harness.o 2> ref_harness.log || { cat ref_harness.log; exit 1; }
-"$FC" -shared -o libechoref.so parallel_nompi.o holib.o eos.o common.o viscous.o system.o work.o evolve.o ref_harness.o
+"$FC" -shared -o libechoref.so parallel_nompi.o holib.o eos.o common.o viscous.o system.o work.o evolve.o \
+  hypersurface.o out.o ref_harness.o
 echo "build_ref: built $OUT/libechoref.so"

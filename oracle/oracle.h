@@ -57,6 +57,7 @@ int  orc_eos_derived(double rh, double pr, double *out3);
 int  orc_eos_table(int n, double *t, double *e, double *p, double *c);
 double orc_przero(void);
 void orc_set_stop(double temp_end, int freeze_type);
+int orc_out_frame(const int *sel, int prec, int iout, const char *path, const char *tpath);
 void orc_get_stop(int *stopped, double *max_value);
 
 /* Glauber-MC initial primitives (oracle_init.c) */

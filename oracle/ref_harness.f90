@@ -143,6 +143,13 @@ contains
   ! Frees what common_alloc, evolve_alloc, work_alloc and the table reader
   ! allocated, so that ref_setup can be called again in one process.
   subroutine release_all()
+    use out, only: arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter, arr_to_print_8_quater, &
+                   arr_to_print_8_quinquies
+    if (allocated(arr_to_print_8)) deallocate(arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter)
+    if (allocated(arr_to_print_8_quater)) deallocate(arr_to_print_8_quater, arr_to_print_8_quinquies)
+    if (allocated(prefix_dir)) deallocate(prefix_dir)
+    if (allocated(recvcounts)) deallocate(recvcounts)
+    if (allocated(displs)) deallocate(displs)
     if (allocated(kv)) deallocate(kv)
     if (allocated(ku)) deallocate(ku)
     if (allocated(v)) deallocate(v)
@@ -394,5 +401,36 @@ contains
   real(c_double) function ref_przero() bind(C, name="ref_przero")
     ref_przero = przero
   end function ref_przero
+
+  ! out_vars (out.f90:232-591) on the current state: sel(1:28) are the
+  ! out_sel flags in the order of type out_type (common.f90:122-151);
+  ! writes <dir>/outr0001/out<iout>.dat and appends <dir>/outr0001/time.dat
+  subroutine ref_out_vars(sel, precision, iout, dirlen, dir) bind(C, name="ref_out_vars")
+    use out
+    integer(c_int), intent(in) :: sel(28)
+    integer(c_int), value :: precision, iout, dirlen
+    character(kind=c_char), intent(in) :: dir(*)
+    integer :: i, io
+    custom_output_directory = ' '
+    do i = 1, dirlen
+      custom_output_directory(i:i) = dir(i)
+    end do
+    out_sel%density = sel(1); out_sel%vx = sel(2); out_sel%vy = sel(3); out_sel%vz = sel(4)
+    out_sel%pressure = sel(5); out_sel%energy_density = sel(6); out_sel%temperature = sel(7)
+    out_sel%entropy_density = sel(8); out_sel%bulk = sel(9); out_sel%pitt = sel(10)
+    out_sel%pitx = sel(11); out_sel%pity = sel(12); out_sel%pitz = sel(13); out_sel%pixy = sel(14)
+    out_sel%pixz = sel(15); out_sel%piyz = sel(16); out_sel%pixx = sel(17); out_sel%piyy = sel(18)
+    out_sel%pizz = sel(19); out_sel%v0 = sel(20); out_sel%bx = sel(21); out_sel%by = sel(22)
+    out_sel%bz = sel(23); out_sel%ex = sel(24); out_sel%ey = sel(25); out_sel%ez = sel(26)
+    out_sel%glm = sel(27); out_sel%rc = sel(28)
+    output_precision = precision
+    out_freeze = .false.
+    derivatives_out = .false.
+    flows_out = .false.
+    id_of_the_run = 1
+    if (.not. allocated(arr_to_print_8)) call out_alloc
+    io = iout
+    call out_vars(io, 0)
+  end subroutine ref_out_vars
 
 end module ref_harness
