@@ -209,6 +209,16 @@ static double smooth_threshold(const Eos& E, double temp_input) {
 static void release(echo_grid* g) {
   if (!g) return;
   hipSetDevice(g->dev);
+  if (g->writer) {
+    g->writer->join();
+    delete g->writer;
+    g->writer = nullptr;
+  }
+  if (g->frame_dev) hipFree(g->frame_dev);
+  if (g->frame_host) hipHostFree(g->frame_host);
+  if (g->frame_ready) hipEventDestroy(g->frame_ready);
+  if (g->frame_copied) hipEventDestroy(g->frame_copied);
+  if (g->copy_stream) hipStreamDestroy(g->copy_stream);
   if (g->gexec) hipGraphExecDestroy(g->gexec);
   if (g->graph) hipGraphDestroy(g->graph);
   hipFree(g->V); hipFree(g->U[0]); hipFree(g->U[1]);
@@ -857,6 +867,143 @@ int echo_ensemble_steps(echo_ensemble* e, int32_t nsteps, int32_t* n_crashed) {
 int echo_ensemble_last_ms(const echo_ensemble* e, double* ms) {
   if (!e || !ms) return fail(ECHO_E_ARG, "null argument");
   *ms = e->last_ms;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// output frames (out_vars, out.f90:232-591), written asynchronously
+// ---------------------------------------------------------------------------
+static int join_writer(echo_grid* g) {
+  if (g->writer) {
+    g->writer->join();
+    delete g->writer;
+    g->writer = nullptr;
+  }
+  const int st = g->writer_status;
+  g->writer_status = 0;
+  return st;
+}
+
+int echo_write_frame(echo_grid* g, const int32_t* sel, int32_t precision, int32_t iout, const char* dir) {
+  if (!g || !sel || !dir) return fail(ECHO_E_ARG, "null argument");
+  if (precision != 4 && precision != 8) return fail(ECHO_E_ARG, "precision must be 4 or 8");
+  if (iout < 0 || iout > 9999) return fail(ECHO_E_ARG, "iout must be 0..9999");
+  if (sel[27]) return fail(ECHO_E_ARG, "the comoving density (rc) is not written");
+  const echo_config& c = g->cfg;
+  const bool visc = c.viscosity != 0, mhd = c.mhd != 0;
+  const bool anypit = sel[9] || sel[10] || sel[11] || sel[12];
+  if (visc && sel[18] && (!c.obtained_zz || !anypit))
+    return fail(ECHO_E_ARG, "pi^{eta eta} is written with TRACE_IMP 'zz' together with a pi^{t mu} field");
+  if (int e = join_writer(g)) return e;
+  HIPCHECK(hipSetDevice(g->dev));
+  // file order of out_vars: (kind, index): kind 0 = primitive component,
+  // kind 1 = derived field
+  std::vector<std::pair<int, int>> order;
+  const int prim5[5] = {KRH, KVX, KVY, KVZ, KPR};
+  for (int q = 0; q < 5; q++) if (sel[q]) order.push_back({0, prim5[q]});
+  for (int q = 0; q < 3; q++) if (sel[5 + q]) order.push_back({1, q});
+  if (visc) {
+    if (sel[8]) order.push_back({0, KPIBU});
+    for (int q = 0; q < 4; q++) if (sel[9 + q]) order.push_back({1, 3 + q});
+    const int pis[5] = {KPIXY, KPIXZ, KPIYZ, KPIXX, KPIYY};
+    for (int q = 0; q < 5; q++) if (sel[13 + q]) order.push_back({0, pis[q]});
+    if (sel[18]) order.push_back({1, 7});
+  }
+  if (sel[19]) order.push_back({1, 8});
+  if (mhd) {
+    if (sel[20]) order.push_back({0, KBX});
+    if (sel[21]) order.push_back({0, KBY});
+    if (sel[22]) order.push_back({0, KBZ});
+    for (int q = 0; q < 3; q++) if (sel[23 + q]) order.push_back({1, 9 + q});
+    if (c.divclean && sel[26]) order.push_back({0, KGLM});
+  }
+  const long long N = g->N;
+  const size_t need = 1 + order.size() * (size_t)N;
+  if (need > g->frame_cap) {
+    if (g->frame_dev) hipFree(g->frame_dev);
+    if (g->frame_host) hipHostFree(g->frame_host);
+    g->frame_dev = nullptr;
+    g->frame_host = nullptr;
+    g->frame_cap = 0;
+    if (hipMalloc(&g->frame_dev, sizeof(double) * (need + 1)) != hipSuccess ||
+        hipHostMalloc(&g->frame_host, sizeof(double) * (need + 1)) != hipSuccess)
+      return fail(ECHO_E_NOMEM, "frame buffers");
+    g->frame_cap = need;
+  }
+  if (!g->copy_stream) {
+    HIPCHECK(hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&g->frame_ready, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&g->frame_copied, hipEventDisableTiming));
+  }
+  // snapshot on the grid stream: t, the primitive arrays, the derived fields;
+  // the extra slot after the arrays holds the EoS status of the derived pass
+  double* fd = g->frame_dev;
+  HIPCHECK(hipMemcpyAsync(fd, &g->S->t, sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+  OutSlots o;
+  bool any_derived = false;
+  for (int j = 0; j < 12; j++) o.dst[j] = -1;
+  for (size_t s = 0; s < order.size(); s++) {
+    if (order[s].first == 0)
+      HIPCHECK(hipMemcpyAsync(fd + 1 + s * N, g->V + (size_t)order[s].second * N, sizeof(double) * N,
+                              hipMemcpyDeviceToDevice, g->stream));
+    else {
+      o.dst[order[s].second] = (int)s;
+      any_derived = true;
+    }
+  }
+  int* fstat = reinterpret_cast<int*>(fd + need);
+  HIPCHECK(hipMemsetAsync(fstat, 0, sizeof(int), g->stream));
+  if (any_derived) {
+    dispatch_sys(g->cfg, [&](auto sys, auto zz) {
+      k_out_derived<decltype(sys)::value, decltype(zz)::value>
+          <<<nblocks(N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, o, fd, fstat);
+    });
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(g->frame_ready, g->stream));
+  HIPCHECK(hipStreamWaitEvent(g->copy_stream, g->frame_ready, 0));
+  HIPCHECK(hipMemcpyAsync(g->frame_host, fd, sizeof(double) * (need + 1), hipMemcpyDeviceToHost, g->copy_stream));
+  HIPCHECK(hipEventRecord(g->frame_copied, g->copy_stream));
+  char name[32];
+  snprintf(name, sizeof name, "/out%04d.dat", iout);
+  const std::string path = std::string(dir) + name;
+  const std::string tpath = std::string(dir) + "/time.dat";
+  const size_t nslot = order.size();
+  g->writer = new std::thread([g, path, tpath, precision, iout, nslot, N, need]() {
+    hipSetDevice(g->dev);
+    if (hipEventSynchronize(g->frame_copied) != hipSuccess) { g->writer_status = ECHO_E_HIP; return; }
+    const double* h = g->frame_host;
+    const int st = *reinterpret_cast<const int*>(h + need);
+    if (st) { g->writer_status = st; return; }
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) { g->writer_status = ECHO_E_ARG; return; }
+    bool ok = true;
+    if (precision == 8) {
+      ok = fwrite(h, sizeof(double), 1 + nslot * (size_t)N, f) == 1 + nslot * (size_t)N;
+    } else {
+      std::vector<float> buf(1 << 16);
+      const size_t tot = 1 + nslot * (size_t)N;
+      for (size_t i0 = 0; i0 < tot && ok; i0 += buf.size()) {
+        const size_t m = std::min(buf.size(), tot - i0);
+        for (size_t i = 0; i < m; i++) buf[i] = (float)h[i0 + i];
+        ok = fwrite(buf.data(), sizeof(float), m, f) == m;
+      }
+    }
+    ok = (fclose(f) == 0) && ok;
+    FILE* tf = fopen(tpath.c_str(), "a");
+    if (tf) {
+      fprintf(tf, "%5d%16.9f\n", iout, h[0]);
+      fclose(tf);
+    }
+    g->writer_status = ok ? 0 : ECHO_E_ARG;
+  });
+  return 0;
+}
+
+int echo_wait_frames(echo_grid* g) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  const int st = join_writer(g);
+  if (st) return fail(st, "frame writer failed with code %d", st);
   return 0;
 }
 

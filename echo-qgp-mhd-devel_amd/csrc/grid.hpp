@@ -5,6 +5,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string>
+#include <thread>
 #include <type_traits>
 
 #include "../../include/echo_amd.h"
@@ -69,6 +70,14 @@ struct echo_grid {
   long long partial_cap = 0;  // doubles in partial
   bool* graph_dirty = nullptr; // set when the launch shape changes (ensemble graph)
   bool stop_check = false;      // check_temp at the start of each step
+  // asynchronous output frames
+  double* frame_dev = nullptr;
+  size_t frame_cap = 0;         // doubles
+  double* frame_host = nullptr; // pinned
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t frame_ready = nullptr, frame_copied = nullptr;
+  std::thread* writer = nullptr;
+  int writer_status = 0;
 };
 
 

@@ -564,6 +564,58 @@ static __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
   }
 }
 
+// Output frames (out_vars, out.f90:232-591): the derived fields of a
+// frame, per cell.  dst[j] is the frame slot of derived field j or -1:
+// 0 energy density, 1 temperature, 2 entropy density (get_derived_data),
+// 3..6 pi^{tt}, pi^{tx}, pi^{ty}, pi^{t eta} and 7 pi^{eta eta} (TRACE_IMP
+// 'zz') from get_derived_pi(_zz), 8 gamma, 9..11 the ideal-MHD electric
+// field of system_compute_E_in_ideal_MHD (system.f90:1885-1897) with E_eta
+// made contravariant.  The frame is [t][arrays], slot s at 1 + s*N.
+struct OutSlots { int dst[12]; };
+
+template <int SYS, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_out_derived(GridArgs a, const double* __restrict__ V,
+                                                             OutSlots o, double* __restrict__ frame,
+                                                             int* __restrict__ status) {
+  constexpr int NV = SysT<SYS>::NV;
+  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N) return;
+  const long long N = a.N;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = V[k * N + c];
+  double* F = frame + 1;
+  if (o.dst[0] >= 0 || o.dst[1] >= 0 || o.dst[2] >= 0) {
+    double en = 0., temp = 0., s = 0.;
+    const int err = get_derived_data(a.eos, v[KRH], v[KPR], en, temp, s);
+    if (err) atomicMax(status, err);
+    if (o.dst[0] >= 0) F[o.dst[0] * N + c] = en;
+    if (o.dst[1] >= 0) F[o.dst[1] * N + c] = temp;
+    if (o.dst[2] >= 0) F[o.dst[2] * N + c] = s;
+  }
+  if constexpr (SysT<SYS>::VISC) {
+    if (o.dst[3] >= 0 || o.dst[4] >= 0 || o.dst[5] >= 0 || o.dst[6] >= 0 || o.dst[7] >= 0) {
+      const PiDer d = derived_pi<ZZ>(v, M);
+      if (o.dst[3] >= 0) F[o.dst[3] * N + c] = d.pitt;
+      if (o.dst[4] >= 0) F[o.dst[4] * N + c] = d.pitx;
+      if (o.dst[5] >= 0) F[o.dst[5] * N + c] = d.pity;
+      if (o.dst[6] >= 0) F[o.dst[6] * N + c] = d.pitz;
+      if (o.dst[7] >= 0) F[o.dst[7] * N + c] = v[KPIZZ];
+    }
+  }
+  if (o.dst[8] >= 0) {
+    const double vx = v[KVX], vy = v[KVY], vz = v[KVZ];
+    F[o.dst[8] * N + c] = 1. / sqrt(1. - (vx * vx) * M.g1 - (vy * vy) * M.g2 - (vz * vz) * M.g3);
+  }
+  if constexpr (SysT<SYS>::MHD) {
+    const double vx = v[KVX], vy = v[KVY], vz = v[KVZ], bx = v[KBX], by = v[KBY], bz = v[KBZ];
+    if (o.dst[9] >= 0) F[o.dst[9] * N + c] = -M.gp * (vy * bz - vz * by);
+    if (o.dst[10] >= 0) F[o.dst[10] * N + c] = -M.gp * (vz * bx - vx * bz);
+    if (o.dst[11] >= 0) F[o.dst[11] * N + c] = -M.gp * (vx * by - vy * bx) / M.g3;
+  }
+}
+
 // check_temp (out.f90:1487-1557), run at the start of each step
 // (echo.f90:189-190): the maximum over the cells of the temperature
 // (freeze_type 0) or energy density (1) from get_derived_temp_for_check

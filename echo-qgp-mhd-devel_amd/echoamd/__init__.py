@@ -33,8 +33,13 @@ EXPORTS = (
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
-    "echo_init_glauber", "echo_set_stop", "echo_get_stop",
+    "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
 )
+
+# out_vars field flags (type out_type of the reference), in file order
+OUT_FIELDS = ("density", "vx", "vy", "vz", "pressure", "energy_density", "temperature",
+              "entropy_density", "bulk", "pitt", "pitx", "pity", "pitz", "pixy", "pixz", "piyz",
+              "pixx", "piyy", "pizz", "v0", "bx", "by", "bz", "ex", "ey", "ez", "glm", "rc")
 
 MLP_NPARAM = 9221
 
@@ -133,6 +138,8 @@ def load_library(path=LIB_PATH):
         "echo_init_glauber": (C.c_int, [P, C.POINTER(GlauberParams), _dp, C.c_int32, _dp,
                                         C.c_int32, _dp, C.c_int32]),
         "echo_set_stop": (C.c_int, [P, C.c_double, C.c_int32]),
+        "echo_write_frame": (C.c_int, [P, _ip, C.c_int32, C.c_int32, C.c_char_p]),
+        "echo_wait_frames": (C.c_int, [P]),
         "echo_get_stop": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "echo_dev_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
         "echo_dev_free": (C.c_int, [C.c_int32, P]),
@@ -273,6 +280,18 @@ class EchoGrid:
         v = np.ascontiguousarray(v, dtype=np.float64)
         assert v.shape == self.shape, (v.shape, self.shape)
         return self._check(self.lib.echo_init_from_prim(self.h, v.ravel()))
+
+    def write_frame(self, fields, iout, outdir, precision=8):
+        """Queue an output frame of the named fields (OUT_FIELDS) as
+        <outdir>/out<iout>.dat; returns at once (see wait_frames)."""
+        sel = np.zeros(len(OUT_FIELDS), dtype=np.int32)
+        for f in fields:
+            sel[OUT_FIELDS.index(f)] = 1
+        self._check(self.lib.echo_write_frame(self.h, sel, int(precision), int(iout),
+                                              os.fsencode(outdir)))
+
+    def wait_frames(self):
+        self._check(self.lib.echo_wait_frames(self.h))
 
     def set_stop(self, temp_end, freeze_type=0):
         """check_temp stop criterion (temp_end <= 0 disables it)."""

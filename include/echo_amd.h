@@ -139,6 +139,22 @@ int echo_get_scalars(echo_grid *g, double *s);
  * evolve.f90:179-188 use it. */
 int echo_set_tout(echo_grid *g, double tout);
 
+/* Output frame of out_vars (out.f90:232-591): the reference's stream file
+ * format -- t, then the fields selected in sel[0..27] (the flags of
+ * type out_type, common.f90:122-151: density, vx, vy, vz, pressure,
+ * energy_density, temperature, entropy_density, bulk, pitt, pitx, pity,
+ * pitz, pixy, pixz, piyz, pixx, piyy, pizz, v0, bx, by, bz, ex, ey, ez, glm,
+ * rc) as nx*ny*nz arrays, FP64 (precision 8) or FP32 (4) -- written to
+ * <dir>/out<iout, 4 digits>.dat, with "iout t" appended to <dir>/time.dat.
+ * The state and the derived fields are copied on the device when the call
+ * is made; the device-to-host copy and the file write then run in the
+ * background while the grid keeps stepping.  echo_wait_frames waits for the
+ * frame in flight (a new echo_write_frame waits for the previous one).
+ * rc is not supported; pizz needs TRACE_IMP 'zz' and a pi^{t mu} field. */
+int echo_write_frame(echo_grid *g, const int32_t *sel, int32_t precision, int32_t iout,
+                     const char *dir);
+int echo_wait_frames(echo_grid *g);
+
 /* Stop criterion check_temp (out.f90:1487-1557; echo.f90:189-190): with
  * temp_end > 0, every step starts by taking the maximum over the cells of
  * the temperature (freeze_type 0) or the energy density (freeze_type 1, where
