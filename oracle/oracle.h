@@ -64,6 +64,13 @@ void orc_get_stop(int *stopped, double *max_value);
 int orc_glauber_prim(const double *par, const double *pa, int na, const double *pb, int nb,
                      const double *pc, int nc, double *vout);
 
+/* freeze-out hypersurface (oracle_fo.c) */
+int  orc_fo_setup(int freeze_type, double freeze_value, double *hdr);
+int  orc_fo_slice(long *n_elements);
+int  orc_fo_width(void);
+void orc_fo_get(double *out);
+int  orc_fo_write(const char *path, int header);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ static void free_fields(void) {
   free(G.xc); free(G.yc); free(G.zc);
   G.v = G.u = G.u0 = G.du = G.du_stiff = G.deriv = G.vnewstep = G.vold = NULL;
   G.ax = G.ay = G.az = G.xc = G.yc = G.zc = NULL;
+  fo_release();
 }
 
 void orc_free(void) {
@@ -468,7 +469,6 @@ static int evolve_u2v(void) {
 /* evolve_smooth_viscosity, evolve.f90:1159-1207.  The threshold pressure
  * comes from find_pressure_from_temperature (eos.f90:1221-1260), restated
  * here as a bisection on T(p) over the EoS range. */
-static int find_pressure_from_temperature(double temp, double *pres);
 static int evolve_smooth_viscosity(void) {
   const size_t nc = ncell();
   if (G.smooth_first) {
@@ -499,7 +499,7 @@ static int evolve_smooth_viscosity(void) {
  * the lower bisection bound pr_low uninitialised; it starts at 0 here.  The
  * bisection stops at |T(p) - T_cut| <= 1e-10 T_cut, so the threshold agrees
  * with the reference to that tolerance whatever the starting bound. */
-static int find_pressure_from_temperature(double temp_input, double *pres) {
+int find_pressure_from_temperature(double temp_input, double *pres) {
   const double tolerance = 1.e-10;
   double en_try = G.eqstate == 3 ? G.tab_e[0] : 0.;
   if (en_try == 0.) en_try = tolerance / 2.;

@@ -60,6 +60,8 @@ int eos_energy(double rh, double pr, double *en);
 int eos_sound(double rh, double en, double pr, double *cs2);
 int get_derived_data(double rh, double pr, double *en, double *temp, double *s);
 int eos_load_table(const char *path);
+int find_pressure_from_temperature(double temp_input, double *pres);
+void fo_release(void);   /* oracle_fo.c: drops the freeze-out slices */
 
 /* reconstruction (holib.f90) - f points to element 0 of f(-ngc..n+ngc) */
 void rec_pencil(int code, const double *f, double *gl, double *gr, int n);

@@ -145,6 +145,7 @@ contains
   subroutine release_all()
     use out, only: arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter, arr_to_print_8_quater, &
                    arr_to_print_8_quinquies
+    call release_fo()
     if (allocated(arr_to_print_8)) deallocate(arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter)
     if (allocated(arr_to_print_8_quater)) deallocate(arr_to_print_8_quater, arr_to_print_8_quinquies)
     if (allocated(prefix_dir)) deallocate(prefix_dir)
@@ -432,5 +433,85 @@ contains
     io = iout
     call out_vars(io, 0)
   end subroutine ref_out_vars
+
+  ! Frees what allocate_hypersurface_arrays (hypersurface.f90:93-168) and
+  ! out_alloc (out.f90:62-120) allocated, and clears the slice times, so that
+  ! ref_fo_setup can be called again in one process.
+  subroutine release_fo()
+    use out, only: arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter, arr_to_print_8_quater, &
+                   arr_to_print_8_quinquies
+    use hypersurface
+    if (allocated(field)) deallocate(field)
+    if (allocated(v_backup)) deallocate(v_backup)
+    if (allocated(primitives)) deallocate(primitives)
+    if (allocated(der_backup)) deallocate(der_backup)
+    if (allocated(hy_ide)) deallocate(hy_ide)
+    if (allocated(hy_vis)) deallocate(hy_vis)
+    if (allocated(hy_deriv)) deallocate(hy_deriv)
+    if (allocated(hy_mhd)) deallocate(hy_mhd)
+    if (allocated(arr_to_print_8)) deallocate(arr_to_print_8, arr_to_print_8_bis, arr_to_print_8_ter)
+    if (allocated(arr_to_print_8_quater)) deallocate(arr_to_print_8_quater, arr_to_print_8_quinquies)
+    if (allocated(prefix_dir)) deallocate(prefix_dir)
+    if (allocated(recvcounts)) deallocate(recvcounts)
+    if (allocated(displs)) deallocate(displs)
+    time_arr = 0.d0
+  end subroutine release_fo
+
+  ! Freeze-out set-up as echo.f90:80-148 does it with OUT_FREEZE on:
+  ! out_alloc (which calls allocate_hypersurface_arrays, out.f90:81-84) and
+  ! get_hysufile_ready (hypersurface.f90:1137-1200), which writes the header
+  ! of <dir>/outr0001/hypersurface.txt.
+  subroutine ref_fo_setup(ftype, fvalue, dirlen, dir) bind(C, name="ref_fo_setup")
+    use out
+    use hypersurface
+    integer(c_int), value :: ftype, dirlen
+    real(c_double), value :: fvalue
+    character(kind=c_char), intent(in) :: dir(*)
+    integer :: i
+    call release_fo()
+    custom_output_directory = ' '
+    do i = 1, dirlen
+      custom_output_directory(i:i) = dir(i)
+    end do
+    id_of_the_run = 1
+    out_freeze = .true.
+    derivatives_out = .false.
+    print_rho_comov = .false.
+    restart_type = 0
+    restarted_run = .false.
+    freeze_type = ftype
+    freeze_value = fvalue
+    call out_alloc
+    call get_hysufile_ready()
+  end subroutine ref_fo_setup
+
+  ! out_hyp (out.f90:1568-1645) on the current state at the current time:
+  ! the first call stores the slice, later calls find the crossings between
+  ! the stored slice and this one and append them to hypersurface.txt.
+  ! Returns 1 when out_hyp flagged run_crashed.
+  integer(c_int) function ref_fo_slice() bind(C, name="ref_fo_slice")
+    use out
+    use hypersurface
+    integer :: io
+    t_collect_d_a = -1.d30
+    run_crashed = .false.
+    io = 1
+    call out_hyp(io)
+    ref_fo_slice = merge(1, 0, run_crashed)
+  end function ref_fo_slice
+
+  ! the stored elements: hy_ide(1:n, 1:13), then hy_vis(1:n, 1:7) (viscous)
+  ! or hy_mhd(1:n, 1:3) (MHD), as rows of out(1:nw, 1:n)
+  subroutine ref_fo_get(n, nw, out) bind(C, name="ref_fo_get")
+    use hypersurface
+    integer(c_int), value :: n, nw
+    real(c_double), intent(out) :: out(nw, n)
+    integer :: i
+    do i = 1, n
+      out(1:13, i) = hy_ide(i, 1:13)
+      if (nw >= 20) out(14:20, i) = hy_vis(i, 1:7)
+      if (nw == 16) out(14:16, i) = hy_mhd(i, 1:3)
+    end do
+  end subroutine ref_fo_get
 
 end module ref_harness

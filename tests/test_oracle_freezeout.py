@@ -1,0 +1,176 @@
+"""Freeze-out hypersurface: the oracle's restatement (oracle_fo.c) against
+the reference's own out_hyp / find_hypersurface_* (oracle/_ref) on the same
+evolved states -- the elements bit for bit, hypersurface.txt byte for byte
+-- and against what the geometry fixes (closed forms on a planar front,
+the normal of a front crossing one cell face, no element for a field that
+does not cross the freeze-out value)."""
+import numpy as np
+import pytest
+
+from libs import OracleFO, RefFO
+from scenarios import fireball3d, mhd3d
+
+SCENARIOS = [
+    ("visc-tab-T", lambda: fireball3d(12, 10, 8, viscosity=1, eqstate=3), 0, 0.2),
+    ("visc-tab-e", lambda: fireball3d(12, 10, 8, viscosity=1, eqstate=3), 1, 1.5),
+    ("visc-tab-T-no-zz", lambda: fireball3d(12, 10, 8, viscosity=1, eqstate=3, obtained_zz=0), 0, 0.2),
+    ("ideal-tab-T", lambda: fireball3d(12, 10, 8, eqstate=3), 0, 0.2),
+    ("ideal-e3p-e", lambda: fireball3d(12, 10, 8, eqstate=1), 1, 2.0),
+    ("mhd-tab-T", lambda: mhd3d(12, 10, 8, algo=1), 0, 0.24),
+    ("mhd-tab-e", lambda: mhd3d(12, 10, 8, algo=1), 1, 5.3),
+]
+
+
+def _sync(oracle, ref):
+    """the oracle takes the reference's state and time, so that the two
+    finders see the same slices bit for bit"""
+    v, u = ref.state()
+    oracle.set_state(v, u)
+    oracle.set_time(ref.scalars()["t"])
+
+
+def _run_pair(oracle, ref, tmp_path, make, ftype, fval, slices=3, steps=2):
+    ic, rc, v0 = make()
+    for lib in (oracle, ref):
+        lib.setup(ic, rc)
+    assert ref.init_from_prim(v0) == 0
+    _sync(oracle, ref)
+    ofo = OracleFO(oracle)
+    rfo = RefFO(ref, str(tmp_path / "ref"))
+    e, hdr = ofo.setup(ftype, fval)
+    assert e == 0
+    rfo.setup(ftype, fval)
+    opath = str(tmp_path / "orc.txt")
+    assert ofo.write(opath, 1) == 0
+    counts = []
+    for k in range(slices):
+        if k:
+            assert ref.steps(steps) == 0
+            _sync(oracle, ref)
+        e, n = ofo.slice()
+        assert e == 0 and rfo.slice() == 0
+        if k == 0:
+            assert n == 0
+            continue
+        assert ofo.write(opath, 0) == 0
+        assert rfo.last_count() == n
+        w = ofo.width
+        a = ofo.elements()
+        b = rfo.elements(n, min(w, 20))
+        assert np.array_equal(a[:, :b.shape[1]], b)
+        counts.append(n)
+    return opath, rfo.path, counts
+
+
+@pytest.mark.parametrize("name,make,ftype,fval", SCENARIOS, ids=[s[0] for s in SCENARIOS])
+def test_elements_and_file_match_reference(oracle, ref, tmp_path, name, make, ftype, fval):
+    opath, rpath, counts = _run_pair(oracle, ref, tmp_path, make, ftype, fval)
+    assert min(counts) > 20, counts
+    a = open(rpath, "rb").read().split(b"\n", 1)
+    b = open(opath, "rb").read().split(b"\n", 1)
+    assert a[1] == b[1]
+    _same_header(a[0], b[0], ftype)
+
+
+def _same_header(ha, hb, ftype):
+    """get_hysufile_ready's header: freeze_type, T_f, e_f.  For freeze_type 0
+    e_f comes from find_pressure_from_temperature, a bisection to 1e-10 in
+    T whose lower bound the reference leaves uninitialised, so e_f agrees to
+    that tolerance; everything else is the same text."""
+    fa, fb = ha.split(), hb.split()
+    assert fa[0] == fb[0] == str(ftype).encode()
+    assert fa[1] == fb[1]
+    if ftype == 1:
+        assert fa[2] == fb[2]
+    else:
+        ea, eb = float(fa[2]), float(fb[2])
+        assert abs(ea - eb) <= 1e-10 * abs(ea)
+
+
+def _planar(oracle, g=0.25, p0=0.6, fv=1.5, s=0.35, t1=1.0, t2=1.2, nx=16, ny=4, nz=3):
+    """p = e/3 with p linear in x (front at x* where 3 p = fv) and v^x = 0.1 x;
+    the second slice has the front moved by s."""
+    from libs import make_config, cell_centres
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, eqstate=1, bc=2,
+                         xlim=(-2.0, 2.0), ylim=(-1.0, 1.0), zlim=(-0.5, 0.5), tmin=t1)
+    oracle.setup(ic, rc)
+    x = cell_centres(nx, -2.0, 2.0)
+    fo = OracleFO(oracle)
+    assert fo.setup(1, fv)[0] == 0
+
+    def put(shift, t):
+        v = np.zeros((5, nz, ny, nx))
+        v[0] = 1.0
+        v[1] = 0.1 * x
+        v[4] = p0 + g * (x - shift)
+        oracle.set_state(v, np.zeros_like(v))
+        oracle.set_time(t)
+        return fo.slice()
+
+    assert put(0.0, t1) == (0, 0)
+    e, n = put(s, t2)
+    assert e == 0
+    xs = (fv / 3 - p0) / g
+    return fo, fo.elements(), x, xs, dict(g=g, p0=p0, fv=fv, s=s, t1=t1, t2=t2, nx=nx, ny=ny, nz=nz)
+
+
+def test_planar_front_closed_form(oracle):
+    fo, a, x, xs, P = _planar(oracle)
+    dx, dy, dz = 4.0 / P["nx"], 2.0 / P["ny"], 1.0 / P["nz"]
+    dt = P["t2"] - P["t1"]
+    moved = (x > xs) & (x < xs + P["s"])        # cell centres the front passed
+    assert len(a) == P["ny"] * P["nz"] * (1 + moved.sum())
+    sp = a[a[:, 10] != 0]                       # elements normal to x
+    tm = a[a[:, 9] != 0]                        # elements normal to tau
+    assert len(sp) == P["ny"] * P["nz"] and len(tm) == P["ny"] * P["nz"] * moved.sum()
+    # spatial elements: on the older slice, at the front, v^x interpolated
+    # linearly, p = fv/3, normal -dt dy dz tau (e grows with x)
+    assert np.all(sp[:, 0] == P["t1"])
+    assert np.allclose(sp[:, 1], xs, rtol=0, atol=1e-14)
+    assert np.allclose(sp[:, 5], 0.1 * xs, rtol=0, atol=1e-15)
+    assert np.allclose(sp[:, 8], P["fv"] / 3, rtol=1e-15, atol=0)
+    assert np.allclose(sp[:, 10], -dt * dy * dz * P["t1"], rtol=1e-14, atol=0)
+    assert np.all(sp[:, [9, 11, 12]] == 0)
+    # tau elements: the time at which the front, moving by s over dt,
+    # passes the cell centre; normal +dx dy dz tau (the cell cools)
+    xc = tm[:, 1]
+    tau = P["t1"] + (xc - xs) / P["s"] * dt
+    assert np.allclose(tm[:, 0], tau, rtol=1e-14, atol=0)
+    assert np.allclose(tm[:, 9], dx * dy * dz * tau, rtol=1e-14, atol=0)
+    assert np.allclose(tm[:, 8], P["fv"] / 3, rtol=1e-14, atol=0)
+    assert np.all(tm[:, 10:13] == 0)
+    # reference loop order: ix outermost, then iy, then iz, tau before x
+    u = (a[:, 1] + 2.0) / dx - 0.5            # fractional cell index along x
+    spatial = a[:, 10] != 0
+    ix = np.where(spatial, np.floor(u), np.round(u)).astype(int)
+    iy = np.round((a[:, 2] + 1.0) / dy - 0.5).astype(int)
+    iz = np.round((a[:, 3] + 0.5) / dz - 0.5).astype(int)
+    key = list(zip(ix, iy, iz, spatial.astype(int)))
+    assert all(key[i] < key[i + 1] for i in range(len(key) - 1))
+
+
+def test_no_crossing_no_element(oracle):
+    fo, a, *_ = _planar(oracle, fv=10.0)    # 3 p < 3.3 everywhere
+    assert len(a) == 0
+
+
+def test_closed_surface_normals_cancel(oracle):
+    """A hot blob inside the grid: every line along x, y or eta enters and
+    leaves it, so the spatial normals of the older slice sum to zero; with
+    p = e/3 and freeze-out on e every element has p = fv/3."""
+    ic, rc, v0 = fireball3d(24, 20, 16, eqstate=1)
+    oracle.setup(ic, rc)
+    assert oracle.init_from_prim(v0) == 0
+    fo = OracleFO(oracle)
+    fv = 3.0
+    assert fo.setup(1, fv)[0] == 0
+    assert fo.slice() == (0, 0)
+    assert oracle.steps(3) == 0
+    e, n = fo.slice()
+    assert e == 0 and n > 100
+    a = fo.elements()
+    for k in (10, 11, 12):
+        col = a[:, k]
+        assert np.any(col != 0)
+        assert abs(col.sum()) <= 1e-13 * np.abs(col).sum()
+    assert np.allclose(a[:, 8], fv / 3, rtol=1e-14, atol=0)
