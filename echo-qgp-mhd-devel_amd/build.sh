@@ -1,7 +1,8 @@
 #!/bin/bash
 # Builds libechoamd.so (HIP kernels for gfx950 + the C-ABI of include/echo_amd.h)
-# from two translation units compiled in parallel: the grid update
-# (echo_capi.hip) and the batched / learned con2prim (c2p_mlp.hip).
+# from three translation units compiled in parallel: the grid update
+# (echo_capi.hip), the batched / learned con2prim (c2p_mlp.hip) and the
+# freeze-out hypersurface finder (freezeout.hip).
 # FP64 multiply-adds are contracted to FMA (-ffp-contract=fast); results then
 # agree with the reference to rounding rather than bit for bit.  Packed-FP32
 # instructions are disabled.
@@ -18,8 +19,12 @@ FLAGS=(--offload-arch="$ARCH" -O3 -std=c++17 -fPIC
 P1=$!
 "$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/c2p_mlp.o" "$HERE/csrc/c2p_mlp.hip" &
 P2=$!
+"$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/freezeout.o" "$HERE/csrc/freezeout.hip" &
+P3=$!
 wait $P1
 wait $P2
-"$HIPCC" --offload-arch="$ARCH" -shared -fPIC -o "$HERE/libechoamd.so" "$OBJ/echo_capi.o" "$OBJ/c2p_mlp.o"
+wait $P3
+"$HIPCC" --offload-arch="$ARCH" -shared -fPIC -o "$HERE/libechoamd.so" "$OBJ/echo_capi.o" "$OBJ/c2p_mlp.o" \
+  "$OBJ/freezeout.o"
 echo "build: $HERE/libechoamd.so"
 "$HIPCC" --offload-arch="$ARCH" -O3 -o "$HERE/copy_calib" "$HERE/csrc/copy_calib.hip"

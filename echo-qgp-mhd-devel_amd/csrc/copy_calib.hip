@@ -25,7 +25,7 @@ int main(int argc, char** argv) {
   for (int r = 0; r < 3; r++) k_copy<<<4096, 256>>>(a, b, n);
   if (hipDeviceSynchronize() != hipSuccess) return 1;
   printf("copy_calib: %lld bytes read and %lld bytes written per launch, 3 launches\n", bytes, bytes);
-  hipFree(a);
-  hipFree(b);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }

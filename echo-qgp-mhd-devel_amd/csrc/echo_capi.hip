@@ -182,30 +182,6 @@ static void enqueue_step(echo_grid* g) {
   enqueue_end_step(g);
 }
 
-// ---------------------------------------------------------------------------
-// host restatement of find_pressure_from_temperature (eos.f90:1221-1252) for
-// the viscous smoothing threshold, with the lower bound started at 0 (the
-// reference leaves it uninitialised)
-static double smooth_threshold(const Eos& E, double temp_input) {
-  const double tolerance = 1.e-10;
-  double en_try = (E.eqstate == 3) ? E.te[0] : 0.;
-  if (en_try == 0.) en_try = tolerance / 2.;
-  double pr_try, d1, d2, en, temp_try, s;
-  eos_pressure(E, 1., en_try, pr_try, d1, d2);
-  get_derived_data(E, 1., pr_try, en, temp_try, s);
-  while (temp_try <= temp_input) {
-    get_derived_data(E, 1., pr_try, en, temp_try, s);
-    pr_try = pr_try * 1.1;
-  }
-  double pr_low = 0., pr_high = pr_try;
-  while (fabs(temp_try - temp_input) > temp_input * tolerance) {
-    pr_try = (pr_low + pr_high) / 2.;
-    get_derived_data(E, 1., pr_try, en, temp_try, s);
-    if (temp_try >= temp_input) pr_high = pr_try; else pr_low = pr_try;
-  }
-  return pr_try;
-}
-
 static void release(echo_grid* g) {
   if (!g) return;
   hipSetDevice(g->dev);
@@ -228,6 +204,8 @@ static void release(echo_grid* g) {
   }
   hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->tab); hipFree(g->tabix);
   hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
+  hipFree(g->fo_f[0]); hipFree(g->fo_f[1]); hipFree(g->fo_f[2]); hipFree(g->fo_vb); hipFree(g->fo_mask);
+  hipFree(g->fo_bsum); hipFree(g->fo_status); hipFree(g->fo_rec);
   for (auto& e : g->ev) if (e) hipEventDestroy(e);
   if (g->stream && g->owns_stream) hipStreamDestroy(g->stream);
   delete g;
@@ -388,12 +366,13 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
     S0.dt = c.maxdt; S0.tout = c.tmin + c.dtout; S0.dtout = c.dtout; S0.tmax = c.tmax;
     S0.glm_ch_old = 0.5;
     S0.glm_ch = 0.5;
-    S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? smooth_threshold(Eh, c.smooth_temp) : 0.;
+    S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? pressure_from_temperature(Eh, c.smooth_temp) : 0.;
     if (hipMemcpy(g->S, &S0, sizeof S0, hipMemcpyHostToDevice) != hipSuccess) {
       release(g);
       return fail(ECHO_E_HIP, "scalar upload failed");
     }
   }
+  g->tab_host = std::move(tab);
   if (c.eqstate == 3) {
     E.tt = g->tab; E.te = g->tab + n_rows; E.tp = g->tab + 2 * n_rows; E.tc = g->tab + 3 * n_rows;
     E.ixe = g->tabix;

@@ -7,6 +7,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/echo_amd.h"
 #include "kernels.hpp"
@@ -78,6 +79,23 @@ struct echo_grid {
   hipEvent_t frame_ready = nullptr, frame_copied = nullptr;
   std::thread* writer = nullptr;
   int writer_status = 0;
+  // EoS table as converted at set-up (T, e, p, c_s^2 columns), host copy
+  std::vector<double> tab_host;
+  // freeze-out hypersurface: the freeze-out field of the older slice (fo_f[0]),
+  // of the newer one (fo_f[1]) and a scratch field (fo_f[2]), the primitives
+  // of the older slice, the per-cell crossing masks (reference loop order),
+  // per-block counts, the elements of the last slice
+  bool fo_on = false;
+  bool fo_searched = false;     // the last slice was compared (not only stored)
+  int fo_type = 0;
+  double fo_value = 0., fo_t1 = 0., fo_t2 = 0., fo_hdr[2] = {0., 0.};
+  double* fo_f[3] = {nullptr, nullptr, nullptr};
+  double* fo_vb = nullptr;
+  unsigned char* fo_mask = nullptr;
+  long long* fo_bsum = nullptr;
+  int* fo_status = nullptr;
+  double* fo_rec = nullptr;
+  long long fo_cap = 0, fo_n = 0;
 };
 
 
@@ -103,6 +121,42 @@ inline void dispatch_sys(const echo_config& c, Fn&& fn) {
   } else {
     fn(std::integral_constant<int, SYS_HD>(), std::false_type());
   }
+}
+
+// find_pressure_from_temperature (eos.f90:1221-1252) on a host copy of the
+// EoS, with the lower bisection bound started at 0 (the reference leaves it
+// uninitialised); the bisection stops at |T(p) - T| <= 1e-10 T
+inline double pressure_from_temperature(const Eos& E, double temp_input) {
+  const double tolerance = 1.e-10;
+  double en_try = (E.eqstate == 3) ? E.te[0] : 0.;
+  if (en_try == 0.) en_try = tolerance / 2.;
+  double pr_try, d1, d2, en, temp_try, s;
+  eos_pressure(E, 1., en_try, pr_try, d1, d2);
+  get_derived_data(E, 1., pr_try, en, temp_try, s);
+  while (temp_try <= temp_input) {
+    get_derived_data(E, 1., pr_try, en, temp_try, s);
+    pr_try = pr_try * 1.1;
+  }
+  double pr_low = 0., pr_high = pr_try;
+  while (fabs(temp_try - temp_input) > temp_input * tolerance) {
+    pr_try = (pr_low + pr_high) / 2.;
+    get_derived_data(E, 1., pr_try, en, temp_try, s);
+    if (temp_try >= temp_input) pr_high = pr_try; else pr_low = pr_try;
+  }
+  return pr_try;
+}
+
+// the grid's EoS with host pointers into tab_host (plain bisection brackets)
+inline Eos host_eos(const echo_grid* g) {
+  Eos E = g->ga.eos;
+  E.ixe = nullptr;
+  E.ixp = nullptr;
+  if (E.eqstate == 3) {
+    const size_t n = (size_t)g->n_tab;
+    const double* t = g->tab_host.data();
+    E.tt = t; E.te = t + n; E.tp = t + 2 * n; E.tc = t + 3 * n;
+  }
+  return E;
 }
 
 }  // namespace echo

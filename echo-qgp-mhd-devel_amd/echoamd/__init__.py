@@ -34,6 +34,7 @@ EXPORTS = (
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
     "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
+    "echo_fo_setup", "echo_fo_slice", "echo_fo_width", "echo_fo_elements", "echo_fo_write",
 )
 
 # out_vars field flags (type out_type of the reference), in file order
@@ -141,6 +142,11 @@ def load_library(path=LIB_PATH):
         "echo_write_frame": (C.c_int, [P, _ip, C.c_int32, C.c_int32, C.c_char_p]),
         "echo_wait_frames": (C.c_int, [P]),
         "echo_get_stop": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+        "echo_fo_setup": (C.c_int, [P, C.c_int32, C.c_double, _dp]),
+        "echo_fo_slice": (C.c_int, [P, C.POINTER(C.c_int64)]),
+        "echo_fo_width": (C.c_int, [P]),
+        "echo_fo_elements": (C.c_int, [P, _dp, C.c_int64]),
+        "echo_fo_write": (C.c_int, [P, C.c_char_p, C.c_int32]),
         "echo_dev_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
         "echo_dev_free": (C.c_int, [C.c_int32, P]),
         "echo_dev_upload": (C.c_int, [C.c_int32, P, P, C.c_int64]),
@@ -302,6 +308,38 @@ class EchoGrid:
         st, mx = C.c_int32(), C.c_double()
         self._check(self.lib.echo_get_stop(self.h, C.byref(st), C.byref(mx)))
         return st.value, mx.value
+
+    # -- freeze-out hypersurface (out_hyp / find_hypersurface_*) ----------
+    def fo_setup(self, freeze_type, freeze_value):
+        """Freeze-out on the temperature (0) or energy density (1) at
+        freeze_value; returns the header's (T_f, e_f)."""
+        hdr = np.zeros(2)
+        self._check(self.lib.echo_fo_setup(self.h, int(freeze_type), float(freeze_value), hdr))
+        return hdr
+
+    def fo_slice(self):
+        """One out_hyp: the number of elements found against the stored
+        slice (0 on the first call)."""
+        n = C.c_int64(0)
+        self._check(self.lib.echo_fo_slice(self.h, C.byref(n)))
+        self._fo_n = n.value
+        return n.value
+
+    @property
+    def fo_width(self):
+        return self.lib.echo_fo_width(self.h)
+
+    def fo_elements(self):
+        """The elements of the last slice, shape (n, fo_width)."""
+        n = getattr(self, "_fo_n", 0)
+        a = np.zeros((max(n, 1), self.fo_width))
+        self._check(self.lib.echo_fo_elements(self.h, a.ravel(), n))
+        return a[:n]
+
+    def fo_write(self, path, header=False):
+        """hypersurface.txt: header=True creates it with the header line,
+        otherwise the last slice is appended."""
+        self._check(self.lib.echo_fo_write(self.h, os.fsencode(path), int(bool(header))))
 
     def init_glauber(self, params, part_a, part_b, coll):
         """Glauber-MC initial state on the device.  params: dict of

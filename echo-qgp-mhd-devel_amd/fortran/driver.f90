@@ -11,18 +11,25 @@
 ! (T0 of the Gubser initialiser in fm^-1; the reference fixes 2.0 at
 ! common.f90:1468), NSTEPS...= (step limit), CHUNK....= (steps per device
 ! call), DUMP_FILE= (final v then u, FP64 stream), DEVICE...=.
+! Freeze-out hypersurface: HYP_COMPU, FREEZKIND, FREEZEVAL, HYPSURFTI as in
+! the reference (common.f90:1062-1076), written to HYP_FILE.= (default
+! hypersurface.txt) at the cadence of echo.f90:176-177,252-258; with
+! HYP_COMPU=1 the time loop advances one step per device call, as the
+! reference checks the cadence after every step.
 program echo_amd_driver
   use iso_c_binding
   use echo_amd_f
   implicit none
   type(echo_config) :: cfg
   type(c_ptr) :: grid
-  character(len=256) :: pfile, eos_file, dump_file
+  character(len=256) :: pfile, eos_file, dump_file, hyp_file
   character(len=10) :: flag
   character(len=120) :: val
   integer :: ios, nsteps, chunk, dev, ierr, nv, iter, n, k
   integer(c_int32_t) :: nrows
-  real(8) :: t0gub, s(10), ttot
+  real(8) :: t0gub, s(10), ttot, freeze_value, freeze_dt, thyp, hdr(2)
+  integer :: hyp_compu, freeze_kind
+  integer(c_int64_t) :: nfo
   real(8), allocatable :: v(:,:,:,:), u(:,:,:,:), rows(:,:)
   integer(8) :: c0, c1, crate
   real(8), parameter :: hbar = 0.197326d0
@@ -39,6 +46,11 @@ program echo_amd_driver
   dev = 0
   eos_file = 'qcdIEOS0.dat'
   dump_file = ' '
+  hyp_file = 'hypersurface.txt'
+  hyp_compu = 0
+  freeze_kind = 1
+  freeze_value = 0.5d0
+  freeze_dt = 0.1d0
   open(29, file=trim(pfile), status='old', iostat=ios)
   if (ios /= 0) then
     write(*,*) 'cannot open ', trim(pfile)
@@ -109,6 +121,11 @@ program echo_amd_driver
     case ('CHUNK....='); read(val, *) chunk
     case ('DEVICE...='); read(val, *) dev
     case ('DUMP_FILE='); dump_file = adjustl(val)
+    case ('HYP_COMPU='); read(val, *) hyp_compu
+    case ('FREEZKIND='); read(val, *) freeze_kind
+    case ('FREEZEVAL='); read(val, *) freeze_value
+    case ('HYPSURFTI='); read(val, *) freeze_dt
+    case ('HYP_FILE.='); hyp_file = adjustl(val)
     end select
   end do
   close(29)
@@ -130,6 +147,15 @@ program echo_amd_driver
   ierr = echo_init_from_prim(grid, v)
   if (ierr /= 0) call fail('echo_init_from_prim', ierr)
   ierr = echo_set_tout(grid, cfg%tmin + cfg%dtout)
+  if (hyp_compu == 1) then
+    ! get_hysufile_ready (hypersurface.f90:1137-1200)
+    ierr = echo_fo_setup(grid, int(freeze_kind, c_int32_t), freeze_value, hdr)
+    if (ierr /= 0) call fail('echo_fo_setup', ierr)
+    ierr = echo_fo_write(grid, trim(hyp_file)//c_null_char, 1_c_int32_t)
+    if (ierr /= 0) call fail('echo_fo_write', ierr)
+    chunk = 1
+  end if
+  thyp = cfg%tmin
 
   write(*,'(a,3i6,a,i3,a,i2)') ' grid', cfg%nx, cfg%ny, cfg%nz, '  nv', nv, '  RK', cfg%nrk
   iter = 0
@@ -148,6 +174,17 @@ program echo_amd_driver
     if (s(1) >= s(7)) then
       ! output time reached (echo.f90:243-247): next tout
       ierr = echo_set_tout(grid, s(7) + cfg%dtout)
+    end if
+    if (hyp_compu == 1) then
+      ! echo.f90:252-258
+      if ((s(1) > thyp) .or. ((thyp - s(1)) < 1.d-10)) then
+        thyp = thyp + freeze_dt
+        if ((abs(thyp - cfg%tmax) < 1.d-10) .or. (thyp > cfg%tmax)) thyp = cfg%tmax
+        ierr = echo_fo_slice(grid, nfo)
+        if (ierr /= 0) call fail('echo_fo_slice', ierr)
+        ierr = echo_fo_write(grid, trim(hyp_file)//c_null_char, 0_c_int32_t)
+        if (ierr /= 0) call fail('echo_fo_write', ierr)
+      end if
     end if
   end do
   write(*,'(a,es12.4,a,i8,a)') ' zone-updates/s', dble(cfg%nx) * cfg%ny * cfg%nz * iter / ttot, &

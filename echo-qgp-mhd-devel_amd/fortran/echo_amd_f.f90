@@ -13,7 +13,9 @@ module echo_amd_f
   public :: echo_config, echo_create, echo_destroy, echo_nv, echo_init_from_prim, &
             echo_upload_state, echo_download_state, echo_steps, echo_rk_stage, echo_end_step, &
             echo_get_scalars, echo_set_tout, echo_cons2prim_batch, echo_set_timing, &
-            echo_get_timing, echo_last_error, echo_error_message
+            echo_get_timing, echo_last_error, echo_error_message, echo_set_stop, echo_get_stop, &
+            echo_write_frame, echo_wait_frames, echo_fo_setup, echo_fo_slice, echo_fo_width, &
+            echo_fo_elements, echo_fo_write
 
   ! struct echo_config: 16 int32 then 19 double (param.dat keys in brackets)
   type, bind(C) :: echo_config
@@ -122,6 +124,60 @@ module echo_amd_f
     end function
     type(c_ptr) function echo_last_error() bind(C, name="echo_last_error")
       import :: c_ptr
+    end function
+    ! check_temp stop criterion (out.f90:1487-1557)
+    integer(c_int) function echo_set_stop(grid, temp_end, freeze_type) bind(C, name="echo_set_stop")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), value :: temp_end
+      integer(c_int32_t), value :: freeze_type
+    end function
+    integer(c_int) function echo_get_stop(grid, stopped, max_value) bind(C, name="echo_get_stop")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: grid
+      integer(c_int32_t), intent(out) :: stopped
+      real(c_double), intent(out) :: max_value
+    end function
+    ! out_vars frames (out.f90:232-591); dir is a C string (trim(d)//c_null_char)
+    integer(c_int) function echo_write_frame(grid, sel, precision, iout, dir) bind(C, name="echo_write_frame")
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: grid
+      integer(c_int32_t), intent(in) :: sel(28)
+      integer(c_int32_t), value :: precision, iout
+      character(kind=c_char), intent(in) :: dir(*)
+    end function
+    integer(c_int) function echo_wait_frames(grid) bind(C, name="echo_wait_frames")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: grid
+    end function
+    ! freeze-out hypersurface: get_hysufile_ready + out_hyp (out.f90:1568-1645)
+    integer(c_int) function echo_fo_setup(grid, freeze_type, freeze_value, hdr) bind(C, name="echo_fo_setup")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: grid
+      integer(c_int32_t), value :: freeze_type
+      real(c_double), value :: freeze_value
+      real(c_double), intent(out) :: hdr(2)
+    end function
+    integer(c_int) function echo_fo_slice(grid, n_elements) bind(C, name="echo_fo_slice")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int64_t), intent(out) :: n_elements
+    end function
+    integer(c_int) function echo_fo_width(grid) bind(C, name="echo_fo_width")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: grid
+    end function
+    integer(c_int) function echo_fo_elements(grid, out, n) bind(C, name="echo_fo_elements")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function echo_fo_write(grid, path, header) bind(C, name="echo_fo_write")
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: grid
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: header
     end function
   end interface
 

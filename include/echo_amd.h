@@ -32,6 +32,8 @@ extern "C" {
 #define ECHO_E_HIP     1002  /* HIP runtime error */
 #define ECHO_E_NOMEM   1003  /* device allocation failed */
 #define ECHO_E_CRASHED 1004  /* the run crashed earlier (run_crashed) */
+#define ECHO_E_FREEZEOUT 1005 /* superluminal velocity on a freeze-out element
+                                 (set_fo_*: "you have to fix this", exit(1)) */
 
 /* Run configuration.  Field meaning and units follow the reference
  * param.dat keys parsed in common.f90:874-1215 (key in brackets).  The
@@ -164,6 +166,35 @@ int echo_wait_frames(echo_grid *g);
  * echo_get_stop: stopped flag and the maximum of the last check. */
 int echo_set_stop(echo_grid *g, double temp_end, int32_t freeze_type);
 int echo_get_stop(echo_grid *g, int32_t *stopped, double *max_value);
+
+/* Freeze-out hypersurface: the out_freeze path of the reference.
+ * echo_fo_setup replaces allocate_hypersurface_arrays
+ * (hypersurface.f90:93-168) and get_hysufile_ready (1137-1200): freeze-out
+ * on the temperature (freeze_type 0) or the energy density (1) at
+ * freeze_value; hdr (may be null) receives the header's temperature and
+ * energy density.  echo_fo_slice replaces out_hyp (out.f90:1568-1645) with
+ * find_hypersurface_{ideal,visco,mhd} (hypersurface.f90:172-229, 466-521,
+ * 775-830): the first call stores the current state as a slice; every
+ * later call finds the crossings of freeze_value between the stored slice
+ * and the current state (along tau in each cell, along x, y, eta between
+ * neighbours of the older slice), in the reference's order, and stores the
+ * current state in turn.  Each element is one row of hypersurface.txt:
+ * echo_fo_width values -- 13 for ideal hydro (tau, x, y, eta, rho, v^x,
+ * v^y, v^eta, p, dSigma_tau, dSigma_x, dSigma_y, dSigma_eta), 25 for
+ * viscous hydro (+ Pi, pi^xy, pi^xeta, pi^yeta, pi^xx, pi^yy, pi^etaeta,
+ * pi^tautau, pi^taux, pi^tauy, pi^taueta, 1/(T^2 (e+p))), 16 for MHD
+ * (+ B^x, B^y, B^eta).  echo_fo_elements copies the n elements of the last
+ * slice (row-major, n * width doubles).  echo_fo_write writes
+ * hypersurface.txt as print_hypersurface_* does: header = 1 creates the
+ * file with its header line, header = 0 appends the last slice (nothing
+ * after the first, storing, slice).  Errors:
+ * an EoS code (nothing changes), ECHO_E_FREEZEOUT (v^2 > 1 on an element,
+ * where the reference exits). */
+int echo_fo_setup(echo_grid *g, int32_t freeze_type, double freeze_value, double *hdr);
+int echo_fo_slice(echo_grid *g, int64_t *n_elements);
+int echo_fo_width(const echo_grid *g);
+int echo_fo_elements(echo_grid *g, double *out, int64_t n);
+int echo_fo_write(echo_grid *g, const char *path, int32_t header);
 
 /* Batched con2prim: system_cons2prim (system.f90:423) on n independent
  * states at the grid's current metric.  u: nv*n conserved (variable-major,

@@ -42,3 +42,50 @@ def test_fortran_driver_gubser(oracle, tmp_path):
     vo, uo = oracle.state()
     assert rel_linf(v, vo) <= 1e-10
     assert rel_linf(u, uo) <= 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
+def test_fortran_driver_freezeout(oracle, tmp_path):
+    """HYP_COMPU=1: the driver writes hypersurface.txt at the reference's
+    cadence (echo.f90:252-258); the oracle, run through the same cadence,
+    writes the same file to the 8 digits of its format."""
+    from libs import OracleFO
+    n, steps, fv, fdt = 32, 12, 0.7, 0.02
+    ic, rc, v0 = gubser2d(n=n)
+    half = n * 0.1 / 2
+    hyp = tmp_path / "hyp_gpu.txt"
+    lines = ["INIT_TYPE=4", "COORD....=2", f"NX.......={n}", f"NY.......={n}", "NZ.......=1",
+             f"XMIN.....={-half!r}", f"XMAX.....={half!r}", f"YMIN.....={-half!r}",
+             f"YMAX.....={half!r}", "ZMIN.....=-0.5", "ZMAX.....=0.5", "TSTART...=1.0",
+             "CFL......=0.4", "MAXDT....=1.0", "REC_ALGO.=WENO5", "INT_ALGO.=RK3",
+             "EOS......=1", "bound_con=2", f"GUB_T0...={GUBSER_THAT0!r}",
+             f"NSTEPS...={steps}", "HYP_COMPU=1", "FREEZKIND=0", f"FREEZEVAL={fv!r}",
+             f"HYPSURFTI={fdt!r}", f"HYP_FILE.={hyp}"]
+    pf = tmp_path / "param.dat"
+    pf.write_text("\n".join(lines) + "\n")
+    r = subprocess.run([DRIVER, str(pf)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    oracle.setup(ic, rc)
+    assert oracle.init_from_prim(v0) == 0
+    fo = OracleFO(oracle)
+    assert fo.setup(0, fv)[0] == 0
+    opath = str(tmp_path / "hyp_orc.txt")
+    assert fo.write(opath, 1) == 0
+    thyp, tmax, first = rc[6], 1e6, True
+    for _ in range(steps):
+        assert oracle.steps(1) == 0
+        t = oracle.scalars()["t"]
+        if t > thyp or thyp - t < 1e-10:
+            thyp = min(thyp + fdt, tmax)
+            assert fo.slice()[0] == 0
+            if not first:
+                assert fo.write(opath, 0) == 0
+            first = False
+    a = open(hyp).read().splitlines()
+    b = open(opath).read().splitlines()
+    assert len(a) == len(b) > 100
+    assert a[0].split()[:2] == b[0].split()[:2]
+    for la, lb in zip(a[1:], b[1:]):
+        va, vb = np.array(la.split(), dtype=float), np.array(lb.split(), dtype=float)
+        assert np.all(np.abs(va - vb) <= 1.5e-7 * np.abs(vb) + 1e-300), (la, lb)
