@@ -200,7 +200,7 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train the learned con2prim network on GPU 0")
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--n-train", type=int, default=1 << 20)
     ap.add_argument("--lr", type=float, default=3e-3)

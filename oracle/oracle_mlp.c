@@ -13,9 +13,10 @@
  *     by prim2cons round trips (a state built from primitives is recovered);
  *   - the derivative of the refinement's residual by finite differences.
  *
- * Network: 8 -> 64 -> 64 -> 64 -> 5, activation gelu(x) = x * phi(clamp(x,
- * -3, 3)) with phi an odd degree-9 polynomial fit of the normal CDF (max
- * error 2.6e-4 on [-3, 3], monotone).  With round = 1 the forward pass
+ * Network: 8 -> 64 -> 64 -> 64 -> 5, activation hard-swish
+ * h(x) = x * clamp(x * c6 + 1/2, 0, 1) with c6 the float nearest 1/6 (the
+ * FMA is single-rounded in float, as the kernel's FMA with clamp).  With
+ * round = 1 the forward pass
  * rounds the network input and every hidden activation to bfloat16 (round
  * to nearest even) and uses bfloat16-rounded weights, as the MFMA kernel
  * does; sums are accumulated in FP64 here and in FP32 on the GPU.
@@ -41,12 +42,7 @@ static const int DOUT[4] = {NH, NH, NH, NO};
 /* loss weights of the five heads */
 static const double LAMBDA[NO] = {1.0, 1.0, 0.25, 0.25, 0.25};
 
-/* phi(t) = 0.5 + t*(C1 + t^2*(C3 + t^2*(C5 + t^2*(C7 + t^2*C9)))) */
-#define GC1 0.39794397f
-#define GC3 -0.063981285f
-#define GC5 0.0081565485f
-#define GC7 -0.00060929223f
-#define GC9 1.9460935e-05f
+#define ACT_C6 0.16666667f
 
 float orc_bf16(float x) {
   uint32_t u;
@@ -58,44 +54,27 @@ float orc_bf16(float x) {
 }
 
 /* float evaluation, explicit fma, the same sequence as the HIP kernel */
-float orc_gelu(float x) {
-  float t = fminf(fmaxf(x, -3.0f), 3.0f);
-  float t2 = t * t;
-  float p = fmaf(t2, GC9, GC7);
-  p = fmaf(t2, p, GC5);
-  p = fmaf(t2, p, GC3);
-  p = fmaf(t2, p, GC1);
-  float phi = fmaf(t, p, 0.5f);
-  return x * phi;
+float orc_act(float x) {
+  const float a = fmaf(x, ACT_C6, 0.5f);
+  return x * fminf(fmaxf(a, 0.0f), 1.0f);
 }
 
-/* d gelu / dx (the clamp makes phi constant outside [-3, 3]) */
-float orc_gelu_d(float x) {
-  float t = fminf(fmaxf(x, -3.0f), 3.0f);
-  float t2 = t * t;
-  float p = fmaf(t2, GC9, GC7);
-  p = fmaf(t2, p, GC5);
-  p = fmaf(t2, p, GC3);
-  p = fmaf(t2, p, GC1);
-  float phi = fmaf(t, p, 0.5f);
-  float dp = fmaf(t2, 9.0f * GC9, 7.0f * GC7);
-  dp = fmaf(t2, dp, 5.0f * GC5);
-  dp = fmaf(t2, dp, 3.0f * GC3);
-  dp = fmaf(t2, dp, GC1);
-  return (x > -3.0f && x < 3.0f) ? phi + x * dp : phi;
+/* dh/dx: s + x c6 where the clamp is inactive, else s (0 or 1) */
+float orc_act_d(float x) {
+  const float a = fmaf(x, ACT_C6, 0.5f);
+  const float sv = fminf(fmaxf(a, 0.0f), 1.0f);
+  return (a > 0.0f && a < 1.0f) ? fmaf(x, ACT_C6, sv) : sv;
 }
 
-static double gelu64(double x) {
-  double t = fmin(fmax(x, -3.0), 3.0), t2 = t * t;
-  double p = (((GC9 * t2 + GC7) * t2 + GC5) * t2 + GC3) * t2 + GC1;
-  return x * (0.5 + t * p);
+static double act64(double x) {
+  const double a = x * (double)ACT_C6 + 0.5;
+  return x * fmin(fmax(a, 0.0), 1.0);
 }
 
-static double gelu64_d(double x) {
-  double t = fmin(fmax(x, -3.0), 3.0), t2 = t * t;
-  double p = (((GC9 * t2 + GC7) * t2 + GC5) * t2 + GC3) * t2 + GC1;
-  double dp = (((9.0 * GC9 * t2 + 7.0 * GC7) * t2 + 5.0 * GC5) * t2 + 3.0 * GC3) * t2 + GC1;
-  return (x > -3.0 && x < 3.0) ? 0.5 + t * p + x * dp : 0.5 + t * p;
+static double act64_d(double x) {
+  const double a = x * (double)ACT_C6 + 0.5;
+  const double sv = fmin(fmax(a, 0.0), 1.0);
+  return (a > 0.0 && a < 1.0) ? sv + x * (double)ACT_C6 : sv;
 }
 
 /* Invariants of a de-densitised MHD conserved state u = (D, S_i, U, B^i)
@@ -183,7 +162,7 @@ static void forward(const float *P, const float *x, int round, double *y, double
     }
     for (int o = 0; o < NH; o++) {
       if (zs) zs[l * NH + o] = z[o];
-      h[o] = round ? orc_bf16(orc_gelu((float)z[o])) : gelu64(z[o]);
+      h[o] = round ? orc_bf16(orc_act((float)z[o])) : act64(z[o]);
       if (hs) hs[NI + l * NH + o] = h[o];
     }
     n_in = NH;
@@ -221,7 +200,7 @@ static void backward(const float *P, int round, const double *zs, const double *
     }
     for (int i = 0; i < ni; i++) {
       const double zz = zs[(l - 1) * NH + i];
-      dz[i] = dh[i] * (round ? (double)orc_gelu_d((float)zz) : gelu64_d(zz));
+      dz[i] = dh[i] * (round ? (double)orc_act_d((float)zz) : act64_d(zz));
     }
   }
 }

@@ -11,8 +11,8 @@ extern "C" {
 #define MLP_NPARAM 9221
 
 float orc_bf16(float x);
-float orc_gelu(float x);
-float orc_gelu_d(float x);
+float orc_act(float x);
+float orc_act_d(float x);
 void orc_mlp_features(const double *u, double g33, float *x);
 double orc_mlp_w0(const double *u, double g33);
 void orc_mlp_targets(const double *u, const double *v, double g33, double *t);

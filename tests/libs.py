@@ -335,10 +335,10 @@ class OracleMLP:
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
         lib.orc_bf16.restype = C.c_float
         lib.orc_bf16.argtypes = [C.c_float]
-        lib.orc_gelu.restype = C.c_float
-        lib.orc_gelu.argtypes = [C.c_float]
-        lib.orc_gelu_d.restype = C.c_float
-        lib.orc_gelu_d.argtypes = [C.c_float]
+        lib.orc_act.restype = C.c_float
+        lib.orc_act.argtypes = [C.c_float]
+        lib.orc_act_d.restype = C.c_float
+        lib.orc_act_d.argtypes = [C.c_float]
         lib.orc_mlp_features.restype = None
         lib.orc_mlp_features.argtypes = [dp, C.c_double, fp]
         lib.orc_mlp_targets.restype = None
@@ -358,11 +358,11 @@ class OracleMLP:
     def bf16(self, x):
         return self.lib.orc_bf16(x)
 
-    def gelu(self, x):
-        return self.lib.orc_gelu(x)
+    def act(self, x):
+        return self.lib.orc_act(x)
 
-    def gelu_d(self, x):
-        return self.lib.orc_gelu_d(x)
+    def act_d(self, x):
+        return self.lib.orc_act_d(x)
 
     def features(self, u, g33=1.0):
         x = np.zeros(8, dtype=np.float32)

@@ -57,18 +57,26 @@ def test_bf16_rounding(om):
     assert om.bf16(-3.0) == -3.0
 
 
-def test_gelu_is_close_to_exact_gelu(om):
-    from scipy.special import ndtr
-    x = np.linspace(-6, 6, 241)
-    g = np.array([om.gelu(float(a)) for a in x])
-    exact = x * ndtr(x)
-    inside = np.abs(x) <= 3
-    assert np.max(np.abs(g - exact)[inside] / np.maximum(np.abs(x[inside]), 1.0)) < 3e-4
-    # derivative against central differences (away from the clamp points)
-    for a in np.linspace(-2.9, 2.9, 37):
-        h = 1e-3
-        fd = (om.gelu(a + h) - om.gelu(a - h)) / (2 * h)
-        assert abs(om.gelu_d(a) - fd) < 2e-3, a
+def test_activation_shape(om):
+    """hard-swish: 0 below -3, the identity above 3, continuous at both
+    joins, x^2/6 + x/2 between (to float rounding), monotone above its
+    minimum -3/8 at x = -3/2; the derivative against central differences
+    away from the joins"""
+    for x in (-50.0, -7.5, -3.0001, -3.0):
+        assert om.act(x) == 0.0 or abs(om.act(x)) < 1e-6
+    for x in (3.0001, 4.0, 11.0, 1e4):
+        assert abs(om.act(x) - x) <= 1e-6 * x
+    x = np.linspace(-2.99, 2.99, 599)
+    h = np.array([om.act(float(a)) for a in x])
+    assert np.max(np.abs(h - (x * x / 6 + x / 2))) < 2e-6
+    assert abs(min(h) + 3 / 8) < 1e-3
+    xr = x[x > -1.5]
+    hr = np.array([om.act(float(a)) for a in xr])
+    assert np.all(np.diff(hr) >= 0)
+    for a in np.concatenate([np.linspace(-2.9, 2.9, 37), [-4.0, 4.0]]):
+        d = 1e-3
+        fd = (om.act(a + d) - om.act(a - d)) / (2 * d)
+        assert abs(om.act_d(a) - fd) < 2e-3, a
 
 
 def test_backward_matches_finite_differences(om):
