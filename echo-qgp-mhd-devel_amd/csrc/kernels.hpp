@@ -92,8 +92,14 @@ __host__ __device__ __forceinline__ long long flux_tiles(int nx, int ny, int nz,
   return ntx * nti * (DIR == 2 ? nz : ny);
 }
 
+// Launch bounds (FLUX_BLOCK, 2): at most 256 VGPRs, so two waves share each
+// SIMD.  Unbounded, the viscous WENO5 kernel takes 320 VGPRs (one wave per
+// SIMD) and its FP64 chains and stencil loads stall with nothing to switch
+// to.  Bounded, that variant spills about 48 VGPRs per lane to scratch
+// (cache-resident) and still runs its sweeps 3-7% faster on MI355X; the
+// other variants fit in 256 VGPRs without spills.
 template <int DIR, int REC, int SYS, bool ZZ, int TY = 1>
-__global__ __launch_bounds__(FLUX_BLOCK) void k_flux(GridArgs a, const double* __restrict__ V,
+__global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
                                                      double* __restrict__ partial) {
   constexpr bool VISC = SysT<SYS>::VISC;
