@@ -17,7 +17,10 @@ FLAGS=(--offload-arch="$ARCH" -O3 -std=c++17 -fPIC
        -Xclang -target-feature -Xclang -packed-fp32-ops -I"$HERE/../include")
 "$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/echo_capi.o" "$HERE/csrc/echo_capi.hip" &
 P1=$!
-"$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/c2p_mlp.o" "$HERE/csrc/c2p_mlp.hip" &
+# the network kernels take their MFMA accumulators in VGPRs rather than AGPRs:
+# the activations read every accumulator, and AGPR results cost one
+# v_accvgpr_read per value (208 per 64-state tile)
+"$HIPCC" "${FLAGS[@]}" -mllvm -amdgpu-mfma-vgpr-form -c -o "$OBJ/c2p_mlp.o" "$HERE/csrc/c2p_mlp.hip" &
 P2=$!
 "$HIPCC" "${FLAGS[@]}" -c -o "$OBJ/freezeout.o" "$HERE/csrc/freezeout.hip" &
 P3=$!
