@@ -47,7 +47,7 @@ __device__ __forceinline__ void get_sigma(int coordinates, const Metric& M, cons
                  dutdt / 3. - uy2 * duxdx / 3. - uy2 * duzdz / 3.;
   } else {
     const double t = M.t;
-    const double t2 = pow(t, 2.), t3 = pow(t, 3.);
+    const double t2 = pow(t, 2.), t3 = t * t * t;   // t**3. as flang lowers it (two multiplications)
     sigma[SZZ] = ut * uz * duzdt + uz * ux * duzdx + uz * uy * duzdy + (2. * duzdz) / (3. * t2) + (2. / 3.) * duzdz * uz2 - dutdt / (3. * t2) - duxdx / (3. *
                  t2) - duydy / (3. * t2) - uz2 * dutdt / 3. - uz2 * duxdx / 3. - uz2 * duydy / 3.
                  + 2. / 3. * ut / t3 + 5. / 3. * (ut / t) * uz2;

@@ -71,6 +71,10 @@ int  orc_fo_width(void);
 void orc_fo_get(double *out);
 int  orc_fo_write(const char *path, int header);
 
+/* restart dump (oracle_out.c) */
+int  orc_dump(const char *path, int iout, double thyp);
+int  orc_restart(const char *path, int *iout, double *thyp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,9 +13,10 @@ orc_state G;
 
 static void free_fields(void) {
   free(G.v); free(G.u); free(G.u0); free(G.du); free(G.du_stiff);
-  free(G.deriv); free(G.vnewstep); free(G.vold); free(G.ax); free(G.ay); free(G.az);
+  free(G.deriv); free(G.vnewstep); free(G.vold); free(G.unewstep); free(G.uold);
+  free(G.ax); free(G.ay); free(G.az);
   free(G.xc); free(G.yc); free(G.zc);
-  G.v = G.u = G.u0 = G.du = G.du_stiff = G.deriv = G.vnewstep = G.vold = NULL;
+  G.v = G.u = G.u0 = G.du = G.du_stiff = G.deriv = G.vnewstep = G.vold = G.unewstep = G.uold = NULL;
   G.ax = G.ay = G.az = G.xc = G.yc = G.zc = NULL;
   fo_release();
 }
@@ -95,8 +96,12 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.du = calloc(n * G.nv, sizeof(double));
   G.du_stiff = calloc(n * G.nv, sizeof(double));
   G.deriv = calloc(n * NDERIV, sizeof(double));
-  G.vnewstep = calloc(n * 5, sizeof(double));
-  G.vold = calloc(n * 5, sizeof(double));
+  /* common.f90:379-388: vold, uold, vnewstep, unewstep hold krh..kvold_end */
+  G.kvold_end = G.mhd ? 8 : 5;
+  G.vnewstep = calloc(n * G.kvold_end, sizeof(double));
+  G.vold = calloc(n * G.kvold_end, sizeof(double));
+  G.unewstep = calloc(n * G.kvold_end, sizeof(double));
+  G.uold = calloc(n * G.kvold_end, sizeof(double));
   G.ax = calloc((size_t)(G.nx + 1) * G.ny * G.nz * 2, sizeof(double));
   G.ay = calloc((size_t)G.nx * (G.ny + 1) * G.nz * 2, sizeof(double));
   G.az = calloc((size_t)G.nx * G.ny * (G.nz + 1) * 2, sizeof(double));
@@ -595,7 +600,8 @@ int orc_steps(int nsteps) {
       check_temp();
       if (G.temp_check) break;
     }
-    memcpy(G.vnewstep, G.v, 5 * nc * sizeof(double));
+    memcpy(G.vnewstep, G.v, G.kvold_end * nc * sizeof(double));
+    memcpy(G.unewstep, G.u, G.kvold_end * nc * sizeof(double));
     for (int irk = 1; irk <= G.nrk; irk++) {
       int e = evolve_main(irk);
       if (e || G.run_crashed) { G.run_crashed = 1; return e ? e : 1; }
@@ -606,7 +612,8 @@ int orc_steps(int nsteps) {
     system_metric();
     G.iter++;
     G.timeinterval = G.dt;
-    memcpy(G.vold, G.vnewstep, 5 * nc * sizeof(double));
+    memcpy(G.vold, G.vnewstep, G.kvold_end * nc * sizeof(double));
+    memcpy(G.uold, G.unewstep, G.kvold_end * nc * sizeof(double));
     if ((G.t > G.tout) || ((G.tout - G.t) < 1.e-10)) {
       G.tout = G.tout + G.dtout;
       if ((fabs(G.tout - G.tmax) < 1.e-10) || (G.tout > G.tmax)) G.tout = G.tmax;

@@ -301,3 +301,30 @@ int orc_fo_write(const char *path, int header) {
   fclose(f);
   return 0;
 }
+
+/* the freeze-out part of out_dump (out.f90:1375-1380): thyp, time_arr,
+ * v_backup (the older slice), field(2,:,:,:) */
+int fo_dump_part(FILE *f, double thyp) {
+  if (!F.on) return 0;
+  const size_t nc = ncell();
+  const double ta[2] = {F.t1, F.t2};
+  int bad = fwrite(&thyp, sizeof(double), 1, f) != 1;
+  bad |= fwrite(ta, sizeof(double), 2, f) != 2;
+  bad |= fwrite(F.vb, sizeof(double), nc * G.nv, f) != nc * G.nv;
+  bad |= fwrite(F.f2, sizeof(double), nc, f) != nc;
+  return bad;
+}
+
+/* the freeze-out part of out_restart (out.f90:1440-1445) */
+int fo_restart_part(FILE *f, double *thyp) {
+  if (!F.on) return 0;
+  const size_t nc = ncell();
+  double ta[2];
+  int bad = fread(thyp, sizeof(double), 1, f) != 1;
+  bad |= fread(ta, sizeof(double), 2, f) != 2;
+  bad |= fread(F.vb, sizeof(double), nc * G.nv, f) != nc * G.nv;
+  bad |= fread(F.f2, sizeof(double), nc, f) != nc;
+  F.t1 = ta[0];
+  F.t2 = ta[1];
+  return bad;
+}

@@ -48,7 +48,8 @@ typedef struct {
   /* check_temp stop criterion */
   double temp_end, check_max; int freeze_type, temp_check;
   /* fields */
-  double *v, *u, *u0, *du, *du_stiff, *deriv, *vnewstep, *vold;
+  double *v, *u, *u0, *du, *du_stiff, *deriv, *vnewstep, *vold, *unewstep, *uold;
+  int kvold_end;   /* components krh..kvold_end of vold/uold: 5 (hydro), 8 (MHD) */
   double *ax, *ay, *az;
 } orc_state;
 
@@ -62,6 +63,9 @@ int get_derived_data(double rh, double pr, double *en, double *temp, double *s);
 int eos_load_table(const char *path);
 int find_pressure_from_temperature(double temp_input, double *pres);
 void fo_release(void);   /* oracle_fo.c: drops the freeze-out slices */
+#include <stdio.h>
+int fo_dump_part(FILE *f, double thyp);
+int fo_restart_part(FILE *f, double *thyp);
 
 /* reconstruction (holib.f90) - f points to element 0 of f(-ngc..n+ngc) */
 void rec_pencil(int code, const double *f, double *gl, double *gr, int n);

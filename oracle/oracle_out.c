@@ -15,6 +15,7 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <stdint.h>
 #include <stdlib.h>
 
 #include "oracle.h"
@@ -112,4 +113,51 @@ int orc_out_frame(const int *sel, int prec, int iout, const char *path, const ch
   free(a);
   fclose(f);
   return err ? -3 : 0;
+}
+
+/* out_dump (out.f90:1313-1390), serial path: dump.dat as a Fortran stream
+ * file -- iout (4-byte integer), t, timeold, timeinterval, v, u (nv
+ * fields each), vold, uold (krh..kvold_end), g_cov_3_old, and with the
+ * freeze-out hypersurface on: thyp, time_arr, v_backup, field(2,:,:,:). */
+int orc_dump(const char *path, int iout, double thyp) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return -2;
+  const size_t n = ncell();
+  const int32_t io = iout;
+  int bad = fwrite(&io, 4, 1, f) != 1;
+  bad |= fwrite(&G.t, sizeof(double), 1, f) != 1;
+  bad |= fwrite(&G.timeold, sizeof(double), 1, f) != 1;
+  bad |= fwrite(&G.timeinterval, sizeof(double), 1, f) != 1;
+  bad |= fwrite(G.v, sizeof(double), n * G.nv, f) != n * G.nv;
+  bad |= fwrite(G.u, sizeof(double), n * G.nv, f) != n * G.nv;
+  bad |= fwrite(G.vold, sizeof(double), n * G.kvold_end, f) != n * G.kvold_end;
+  bad |= fwrite(G.uold, sizeof(double), n * G.kvold_end, f) != n * G.kvold_end;
+  bad |= fwrite(&G.g_cov_3_old, sizeof(double), 1, f) != 1;
+  bad |= fo_dump_part(f, thyp);
+  fclose(f);
+  return bad ? -3 : 0;
+}
+
+/* out_restart (out.f90:1394-1483) followed by system_metric and the tout
+ * update of echo.f90:164-172 */
+int orc_restart(const char *path, int *iout, double *thyp) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -2;
+  const size_t n = ncell();
+  int32_t io = 0;
+  int bad = fread(&io, 4, 1, f) != 1;
+  bad |= fread(&G.t, sizeof(double), 1, f) != 1;
+  bad |= fread(&G.timeold, sizeof(double), 1, f) != 1;
+  bad |= fread(&G.timeinterval, sizeof(double), 1, f) != 1;
+  bad |= fread(G.v, sizeof(double), n * G.nv, f) != n * G.nv;
+  bad |= fread(G.u, sizeof(double), n * G.nv, f) != n * G.nv;
+  bad |= fread(G.vold, sizeof(double), n * G.kvold_end, f) != n * G.kvold_end;
+  bad |= fread(G.uold, sizeof(double), n * G.kvold_end, f) != n * G.kvold_end;
+  bad |= fread(&G.g_cov_3_old, sizeof(double), 1, f) != 1;
+  bad |= fo_restart_part(f, thyp);
+  fclose(f);
+  *iout = io;
+  system_metric();
+  G.tout = G.t + G.dtout;
+  return bad ? -3 : 0;
 }

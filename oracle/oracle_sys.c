@@ -897,7 +897,7 @@ static void get_sigma(int ix, int iy, int iz, double *sigma, double *dd) {
                  dutdt / 3. - uy2 * duxdx / 3. - uy2 * duzdz / 3.;
   } else {
     const double t = G.t;
-    const double t2 = pow(t, 2.), t3 = pow(t, 3.);
+    const double t2 = pow(t, 2.), t3 = t * t * t;   /* flang lowers t**3. to two multiplications */
     sigma[SZZ] = ut * uz * duzdt + uz * ux * duzdx + uz * uy * duzdy + (2. * duzdz) / (3. * t2) + (2. / 3.) * duzdz * uz2 - dutdt / (3. * t2) - duxdx / (3. *
                  t2) - duydy / (3. * t2) - uz2 * dutdt / 3. - uz2 * duxdx / 3. - uz2 * duydy / 3.
                  + 2. / 3. * ut / t3 + 5. / 3. * (ut / t) * uz2;

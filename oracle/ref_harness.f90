@@ -454,6 +454,7 @@ contains
     if (allocated(prefix_dir)) deallocate(prefix_dir)
     if (allocated(recvcounts)) deallocate(recvcounts)
     if (allocated(displs)) deallocate(displs)
+    if (allocated(uall)) deallocate(uall, vallold, uallold)
     time_arr = 0.d0
   end subroutine release_fo
 
@@ -477,7 +478,7 @@ contains
     out_freeze = .true.
     derivatives_out = .false.
     print_rho_comov = .false.
-    restart_type = 0
+    restart_type = 1        ! out_alloc also allocates the dump arrays
     restarted_run = .false.
     freeze_type = ftype
     freeze_value = fvalue
@@ -513,5 +514,40 @@ contains
       if (nw == 16) out(14:16, i) = hy_mhd(i, 1:3)
     end do
   end subroutine ref_fo_get
+
+  ! Output set-up for restart dumps without the freeze-out hypersurface:
+  ! out_alloc with restart_type = 1 (out.f90:108-118), files under
+  ! <dir>/outr0001/
+  subroutine ref_dump_setup(dirlen, dir) bind(C, name="ref_dump_setup")
+    use out
+    use hypersurface
+    integer(c_int), value :: dirlen
+    character(kind=c_char), intent(in) :: dir(*)
+    integer :: i
+    call release_fo()
+    custom_output_directory = ' '
+    do i = 1, dirlen
+      custom_output_directory(i:i) = dir(i)
+    end do
+    id_of_the_run = 1
+    out_freeze = .false.
+    derivatives_out = .false.
+    restart_type = 1
+    restarted_run = .false.
+    call out_alloc
+  end subroutine ref_dump_setup
+
+  ! out_dump (out.f90:1313-1390) of the current state, after the collection
+  ! that out_vars / out_hyp make before it; thyp is the freeze-out cadence
+  ! time it records
+  subroutine ref_out_dump(iout, thyp_in) bind(C, name="ref_out_dump")
+    use out
+    integer(c_int), value :: iout
+    real(c_double), value :: thyp_in
+    t_collect_d_a = -1.d30
+    call collect_distributed_arrays
+    thyp = thyp_in
+    call out_dump(iout)
+  end subroutine ref_out_dump
 
 end module ref_harness

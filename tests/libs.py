@@ -302,6 +302,39 @@ class RefFO:
         return a[:n]
 
 
+def oracle_dump(oracle, path, iout, thyp=0.0):
+    f = oracle.lib.orc_dump
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.c_int, C.c_double]
+    return f(str(path).encode(), int(iout), float(thyp))
+
+
+def oracle_restart(oracle, path):
+    """returns (error, iout, thyp)"""
+    f = oracle.lib.orc_restart
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    io, th = C.c_int(0), C.c_double(0.0)
+    e = f(str(path).encode(), C.byref(io), C.byref(th))
+    return e, io.value, th.value
+
+
+def ref_dump(ref, outdir, iout, thyp=0.0, setup=True):
+    """the reference's out_dump of its current state into <outdir>/outr0001/dump.dat;
+    setup=False when ref_fo_setup has already made the output set-up"""
+    os.makedirs(os.path.join(outdir, "outr0001"), exist_ok=True)
+    if setup:
+        f = ref.lib.ref_dump_setup
+        f.restype = None
+        f.argtypes = [C.c_int, C.c_char_p]
+        f(len(outdir), outdir.encode())
+    g = ref.lib.ref_out_dump
+    g.restype = None
+    g.argtypes = [C.c_int, C.c_double]
+    g(int(iout), float(thyp))
+    return os.path.join(outdir, "outr0001", "dump.dat")
+
+
 class Ref(_Lib):
     prefix = "ref_"
 

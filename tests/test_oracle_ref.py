@@ -100,6 +100,7 @@ SCENARIOS = {
     "fireball-visc-mink-periodic": (lambda: fireball3d(12, 10, 8, viscosity=1, coordinates=1,
                                                        bc=0, nrk=2), 5),
     "fireball-visc-no-zz": (lambda: fireball3d(12, 10, 8, viscosity=1, obtained_zz=0), 5),
+    "fireball-visc-tab": (lambda: fireball3d(12, 10, 8, viscosity=1, eqstate=3), 5),
     "mhd2d-algo5-glm": (lambda: mhd2d(24), 8),
     "mhd2d-algo5-noglm-ppm4": (lambda: mhd2d(24, divclean=0, rec="PPM4"), 8),
     "mhd2d-algo6": (lambda: mhd2d(24, algo=6), 8),
