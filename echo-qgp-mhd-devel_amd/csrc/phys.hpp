@@ -197,6 +197,30 @@ __host__ __device__ __forceinline__ int get_derived_data(const Eos& E, double rh
   return 0;
 }
 
+// 1/x for finite x > 0 in the normal range: the hardware reciprocal and two
+// Newton-Raphson steps, within about an ulp of the correctly rounded value,
+// in five instructions instead of the ten of the IEEE division sequence.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// 1/sqrt(x) for 0 < x <= 1 (Lorentz factors of subluminal states): the
+// hardware reciprocal square root and two Newton-Raphson steps, within about
+// an ulp of 1/sqrt(x); x = 0 gives +inf as the IEEE form does.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = __builtin_fma(-h * r, r, 0.5);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-h * r, r, 0.5);
+  r = __builtin_fma(r, e, r);
+  return x == 0.0 ? __builtin_inf() : r;
+}
+
 // ---------------------------------------------------------------------------
 // Reconstruction (holib.f90).  f[0..5] = cells i-2 .. i+3 around the
 // interface i+1/2; returns the left (gl) and right (gr) states there.
@@ -246,8 +270,10 @@ __device__ __forceinline__ double mp5(double f, double f1, double f2, double f3,
 // One WENO5 side (holib.f90:206-217).  The reference's weights
 // alpha_k = d_k / (beta_k/12 + eps)^2, normalised by their sum, are
 // evaluated as  w_k = d_k * prod_{j != k} Q_j  with Q_j = (beta_j + 12 eps)^2:
-// the same ratios (the factor 144 cancels), with one division per side
-// instead of seven.  Results agree with the reference form to rounding.
+// the same ratios (the factor 144 cancels), with one reciprocal per side
+// instead of seven divisions (the sum of the weights is positive, at least
+// (12 eps)^4 / 16, and finite).  Results agree with the reference form to
+// rounding.
 __device__ __forceinline__ double weno5_side(double f1, double f2, double f3, double f4, double f5) {
   const double d1 = 1 / 16., d2 = 10 / 16., d3 = 5 / 16., eps12 = 12. * 1.e-6;
   double t, s;
@@ -262,7 +288,7 @@ __device__ __forceinline__ double weno5_side(double f1, double f2, double f3, do
   double p1 = 3 / 8. * f1 + -10 / 8. * f2 + 15 / 8. * f3;
   double p2 = -1 / 8. * f2 + 6 / 8. * f3 + 3 / 8. * f4;
   double p3 = 3 / 8. * f3 + 6 / 8. * f4 + -1 / 8. * f5;
-  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+  return (w1 * p1 + w2 * p2 + w3 * p3) * rcp_nr(w1 + w2 + w3);
 }
 
 __device__ __forceinline__ double weno3_side(double f1, double f2, double f3) {
@@ -335,17 +361,19 @@ __device__ __forceinline__ PiDer derived_pi(double* p, const Metric& M) {
     r.pitz = xz * vx * g1 + yz * vy * g2 + p[KPIZZ] * vz * g3;
     r.pitt = r.pitx * vx * g1 + r.pity * vy * g2 + r.pitz * vz * g3;
   } else {
-    r.pitt = 1. / (1. - vz * vz * g3) * (vx * g1 * xy * vy * g2 + vx * vx * g1 * g1 * xx +
+    // 1/(1 - v_eta^2 g33) once, by reciprocal (positive for subluminal v)
+    const double ivz = rcp_nr(1. - vz * vz * g3);
+    r.pitt = ivz * (vx * g1 * xy * vy * g2 + vx * vx * g1 * g1 * xx +
              vy * g1 * yz * vz * g3 + 2. * vx * g1 * xz * vz * g3 + vy * g1 * g1 * xy *
              vx + vy * vy * g1 * yy * g2 + vz * g3 * yz * vy * g2 - xx * g1 * vz * vz *
              g3 - yy * g2 * vz * vz * g3);
     r.pitx = xx * vx * g1 + xy * vy * g2 + xz * vz * g3;
     r.pity = xy * vx * g1 + yy * vy * g2 + yz * vz * g3;
-    r.pitz = 1. / (1. - vz * vz * g3) * (xz * vx * g1 + xz * vx * g1 * vz * vz * g3 + yz * vy *
+    r.pitz = ivz * (xz * vx * g1 + xz * vx * g1 * vz * vz * g3 + yz * vy *
              g2 + vz * vx * g1 * xy * vy * g2 - vz * xx * g1 - vz * yy * g2 + vz * vx *
              vx * g1 * g1 * xx + vy * g1 * yz * vz * vz * g3 + vz * vy * g1 * g1 *
              xy * vx + vz * vy * vy * g1 * yy * g2);
-    p[KPIZZ] = 1. / (g3 * (1. - vz * vz * g3)) * (vx * g1 * xy * vy * g2 - xx * g1 - yy * g2 +
+    p[KPIZZ] = (ivz * rcp_nr(g3)) * (vx * g1 * xy * vy * g2 - xx * g1 - yy * g2 +
              vx * vx * g1 * g1 * xx + vy * g1 * yz * vz * g3 + 2. * vx * g1 * xz * vz *
              g3 + vy * g1 * g1 * xy * vx + vy * vy * g1 * yy * g2 + vz * g3 * yz *
              vy * g2);
@@ -443,7 +471,7 @@ __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double*
   if (err > 0) return err;
   double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
   if (v2 > 1) return 333;
-  double glf = 1. / sqrt(1. - v2);
+  double glf = rsqrt_nr(1. - v2);
   double w = VISC ? en + pr + v[KPIBU] : en + pr;
   s.ww = w * (glf * glf);
   s.rh = rh; s.glf = glf; s.pr = pr; s.v2 = v2;
