@@ -176,8 +176,15 @@ static void enqueue_check(echo_grid* g) {
   k_check_decide<<<1, 1024, 0, g->stream>>>(g->S, g->partial, (int)nb);
 }
 
+static int dump_kend(const echo_grid* g) { return g->cfg.mhd ? 8 : 5; }   // common.f90:379-386
+
 static void enqueue_step(echo_grid* g) {
   enqueue_check(g);
+  if (g->dump_track) {
+    const long long n = (long long)dump_kend(g) * g->N;
+    k_dump_copy<<<nblocks(n, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(g->S, g->V, g->U[0], g->dump_buf[0],
+                                                                          g->dump_buf[1], n);
+  }
   for (int irk = 1; irk <= g->cfg.nrk; irk++) enqueue_stage(g, irk);
   enqueue_end_step(g);
 }
@@ -206,6 +213,7 @@ static void release(echo_grid* g) {
   hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
   hipFree(g->fo_f[0]); hipFree(g->fo_f[1]); hipFree(g->fo_f[2]); hipFree(g->fo_vb); hipFree(g->fo_mask);
   hipFree(g->fo_bsum); hipFree(g->fo_status); hipFree(g->fo_rec);
+  hipFree(g->dump_buf[0]); hipFree(g->dump_buf[1]);
   for (auto& e : g->ev) if (e) hipEventDestroy(e);
   if (g->stream && g->owns_stream) hipStreamDestroy(g->stream);
   delete g;
@@ -688,6 +696,151 @@ int echo_set_stop(echo_grid* g, double temp_end, int32_t freeze_type) {
   g->stop_check = temp_end > 0.;
   drop_graph(g);
   if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
+int echo_set_dump(echo_grid* g, int32_t on) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_dump inside a step");
+  HIPCHECK(hipSetDevice(g->dev));
+  if (on && !g->dump_buf[0]) {
+    const size_t sz = sizeof(double) * 2 * dump_kend(g) * (size_t)g->N;
+    for (int b = 0; b < 2; b++)
+      if (hipMalloc(&g->dump_buf[b], sz) != hipSuccess) {
+        hipFree(g->dump_buf[0]);
+        g->dump_buf[0] = g->dump_buf[1] = nullptr;
+        return fail(ECHO_E_NOMEM, "echo_set_dump: allocation of %zu bytes failed", 2 * sz);
+      }
+  }
+  if (on) {
+    // until a step has run, vold / uold are the current state (init.f90:3271-3272)
+    DevScalars S;
+    HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    const size_t n = (size_t)dump_kend(g) * g->N;
+    double* B = g->dump_buf[(S.iter - 1) & 1];
+    HIPCHECK(hipMemcpyAsync(B, g->V, sizeof(double) * n, hipMemcpyDeviceToDevice, g->stream));
+    HIPCHECK(hipMemcpyAsync(B + n, g->U[0], sizeof(double) * n, hipMemcpyDeviceToDevice, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  }
+  g->dump_track = on != 0;
+  drop_graph(g);
+  if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
+// the freeze-out part of the dump: thyp, time_arr, the slice the next
+// echo_fo_slice compares against, field(2,:,:,:)
+static int fo_dump_io(echo_grid* g, FILE* f, bool write, double* thyp) {
+  const size_t N = (size_t)g->N, nvN = N * g->nv;
+  std::vector<double> a(nvN), fld(N);
+  double ta[2] = {g->fo_t1, g->fo_t2};
+  if (write) {
+    HIPCHECK(hipMemcpyAsync(a.data(), g->fo_vb, sizeof(double) * nvN, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipMemcpyAsync(fld.data(), g->fo_f[1], sizeof(double) * N, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    bool bad = fwrite(thyp, sizeof(double), 1, f) != 1;
+    bad |= fwrite(ta, sizeof(double), 2, f) != 2;
+    bad |= fwrite(a.data(), sizeof(double), nvN, f) != nvN;
+    bad |= fwrite(fld.data(), sizeof(double), N, f) != N;
+    return bad ? fail(ECHO_E_ARG, "dump: write failed") : 0;
+  }
+  bool bad = fread(thyp, sizeof(double), 1, f) != 1;
+  bad |= fread(ta, sizeof(double), 2, f) != 2;
+  bad |= fread(a.data(), sizeof(double), nvN, f) != nvN;
+  bad |= fread(fld.data(), sizeof(double), N, f) != N;
+  if (bad) return fail(ECHO_E_ARG, "restart: the file ends early");
+  HIPCHECK(hipMemcpyAsync(g->fo_vb, a.data(), sizeof(double) * nvN, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(g->fo_f[1], fld.data(), sizeof(double) * N, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  g->fo_t1 = ta[0];
+  g->fo_t2 = ta[1];
+  g->fo_n = 0;
+  g->fo_searched = false;
+  return 0;
+}
+
+int echo_dump(echo_grid* g, const char* path, int32_t iout, double thyp) {
+  if (!g || !path) return fail(ECHO_E_ARG, "null argument");
+  if (!g->dump_track) return fail(ECHO_E_ARG, "echo_dump: call echo_set_dump(g, 1) before the steps to dump");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_dump inside a step");
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t N = (size_t)g->N, nvN = N * g->nv, kN = N * dump_kend(g);
+  DevScalars S;
+  std::vector<double> v(nvN), u(nvN), old(2 * kN);
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  HIPCHECK(hipMemcpyAsync(v.data(), g->V, sizeof(double) * nvN, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipMemcpyAsync(u.data(), g->U[0], sizeof(double) * nvN, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipMemcpyAsync(old.data(), g->dump_buf[(S.iter - 1) & 1], sizeof(double) * 2 * kN,
+                          hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(ECHO_E_ARG, "echo_dump: cannot open %s", path);
+  bool bad = fwrite(&iout, 4, 1, f) != 1;
+  bad |= fwrite(&S.t, sizeof(double), 1, f) != 1;
+  bad |= fwrite(&S.timeold, sizeof(double), 1, f) != 1;
+  bad |= fwrite(&S.timeinterval, sizeof(double), 1, f) != 1;
+  bad |= fwrite(v.data(), sizeof(double), nvN, f) != nvN;
+  bad |= fwrite(u.data(), sizeof(double), nvN, f) != nvN;
+  bad |= fwrite(old.data(), sizeof(double), 2 * kN, f) != 2 * kN;   // vold, then uold
+  bad |= fwrite(&S.g_cov_3_old, sizeof(double), 1, f) != 1;
+  int e = 0;
+  if (!bad && g->fo_on) e = fo_dump_io(g, f, true, &thyp);
+  bad |= fclose(f) != 0;
+  if (e) return e;
+  return bad ? fail(ECHO_E_ARG, "echo_dump: write to %s failed", path) : 0;
+}
+
+int echo_restart(echo_grid* g, const char* path, int32_t* iout, double* thyp) {
+  if (!g || !path || !iout) return fail(ECHO_E_ARG, "null argument");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_restart inside a step");
+  HIPCHECK(hipSetDevice(g->dev));
+  const size_t N = (size_t)g->N, nvN = N * g->nv, kN = N * dump_kend(g);
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(ECHO_E_ARG, "echo_restart: cannot open %s", path);
+  int32_t io = 0;
+  double t, timeold, timeinterval, g3old;
+  std::vector<double> v(nvN), u(nvN), old(2 * kN);
+  bool bad = fread(&io, 4, 1, f) != 1;
+  bad |= fread(&t, sizeof(double), 1, f) != 1;
+  bad |= fread(&timeold, sizeof(double), 1, f) != 1;
+  bad |= fread(&timeinterval, sizeof(double), 1, f) != 1;
+  bad |= fread(v.data(), sizeof(double), nvN, f) != nvN;
+  bad |= fread(u.data(), sizeof(double), nvN, f) != nvN;
+  bad |= fread(old.data(), sizeof(double), 2 * kN, f) != 2 * kN;
+  bad |= fread(&g3old, sizeof(double), 1, f) != 1;
+  if (bad) {
+    fclose(f);
+    return fail(ECHO_E_ARG, "echo_restart: %s ends early for this grid", path);
+  }
+  double th = 0.;
+  int e = g->fo_on ? fo_dump_io(g, f, false, &th) : 0;
+  fclose(f);
+  if (e) return e;
+  DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  // out_restart, then system_metric and tout = t + dtout (echo.f90:164-172)
+  S.t = t; S.timeold = timeold; S.timeinterval = timeinterval; S.g_cov_3_old = g3old;
+  S.tout = t + S.dtout;
+  S.run_crashed = 0; S.errcode = 0; S.temp_check = 0;
+  HIPCHECK(hipMemcpyAsync(g->S, &S, sizeof S, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(g->V, v.data(), sizeof(double) * nvN, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(g->U[0], u.data(), sizeof(double) * nvN, hipMemcpyHostToDevice, g->stream));
+  if (g->cfg.viscosity) {
+    // vnewstep of the next step: the restored velocities; vold: the dump's
+    const int par = S.vs_parity;
+    HIPCHECK(hipMemcpyAsync(g->VS[par], v.data() + N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, g->stream));
+    HIPCHECK(hipMemcpyAsync(g->VS[par ^ 1], old.data() + N, sizeof(double) * 3 * N, hipMemcpyHostToDevice,
+                            g->stream));
+  }
+  if (g->dump_track)
+    HIPCHECK(hipMemcpyAsync(g->dump_buf[(S.iter - 1) & 1], old.data(), sizeof(double) * 2 * kN,
+                            hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  *iout = io;
+  if (thyp) *thyp = th;
   return 0;
 }
 

@@ -79,6 +79,11 @@ struct echo_grid {
   hipEvent_t frame_ready = nullptr, frame_copied = nullptr;
   std::thread* writer = nullptr;
   int writer_status = 0;
+  // restart dump: v and u components krh..kvold_end at the start of the
+  // last two steps (buffer iter & 1 is written when step iter starts), so
+  // that vold / uold of out_dump are at hand between steps
+  double* dump_buf[2] = {nullptr, nullptr};
+  bool dump_track = false;
   // EoS table as converted at set-up (T, e, p, c_s^2 columns), host copy
   std::vector<double> tab_host;
   // freeze-out hypersurface: the freeze-out field of the older slice (fo_f[0]),

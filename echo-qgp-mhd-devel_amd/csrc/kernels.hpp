@@ -160,7 +160,7 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
         const double vx = V[(long long)KVX * a.N + off[j]];
         const double vy = V[(long long)KVY * a.N + off[j]];
         const double vz = V[(long long)KVZ * a.N + off[j]];
-        const double gl = 1. / sqrt(1. - vx * vx - vy * vy - M.g3 * (vz * vz));
+        const double gl = rsqrt_nr(1. - vx * vx - vy * vy - M.g3 * (vz * vz));
         wd[0][j] = gl; wd[1][j] = gl * vx; wd[2][j] = gl * vy; wd[3][j] = gl * vz;
       }
       const long long nfl = nf;
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
       amax = dmax(ap, am);
       // aflux(dir) keeps the last interface of the sweep (work.f90:217,310,401)
       if (f == nf - 1) a.S->aflux[DIR - 1] = amax;
-      double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
+      double a1 = (ap == 0.0 && am == 0.0) ? 1. : rcp_nr(ap + am);
 #pragma unroll
       for (int k = 0; k < NV; k++) {
         double ulk = u_comp<SYS>(k, M, sl, wl) * M.gp;
@@ -554,6 +554,24 @@ __global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__
 
 // end of step bookkeeping, echo.f90:211-247 (time, metric, dts interval,
 // vnewstep -> vold, tout cadence)
+// vnewstep / unewstep of echo.f90:192-201 for the restart dump: components
+// krh..kvold_end of v and u at the start of the step go to the buffer of the
+// step's parity (iter & 1); between steps the buffer of parity
+// (iter - 1) & 1 is vold / uold (echo.f90:240-241).  Each buffer holds the
+// kend v components, then the kend u components.
+static __global__ __launch_bounds__(UPDATE_BLOCK) void k_dump_copy(const DevScalars* S,
+                                                                  const double* __restrict__ V,
+                                                                  const double* __restrict__ U,
+                                                                  double* __restrict__ B0,
+                                                                  double* __restrict__ B1, long long n) {
+  if (S->temp_check || S->run_crashed) return;
+  const long long i = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double* B = (S->iter & 1) ? B1 : B0;
+  B[i] = V[i];
+  B[n + i] = U[i];
+}
+
 static __global__ void k_end_step(DevScalars* S, int coordinates, int flip_vs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (S->run_crashed || S->temp_check) return;

@@ -35,6 +35,7 @@ EXPORTS = (
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
     "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
     "echo_fo_setup", "echo_fo_slice", "echo_fo_width", "echo_fo_elements", "echo_fo_write",
+    "echo_set_dump", "echo_dump", "echo_restart",
 )
 
 # out_vars field flags (type out_type of the reference), in file order
@@ -142,6 +143,9 @@ def load_library(path=LIB_PATH):
         "echo_write_frame": (C.c_int, [P, _ip, C.c_int32, C.c_int32, C.c_char_p]),
         "echo_wait_frames": (C.c_int, [P]),
         "echo_get_stop": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+        "echo_set_dump": (C.c_int, [P, C.c_int32]),
+        "echo_dump": (C.c_int, [P, C.c_char_p, C.c_int32, C.c_double]),
+        "echo_restart": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "echo_fo_setup": (C.c_int, [P, C.c_int32, C.c_double, _dp]),
         "echo_fo_slice": (C.c_int, [P, C.POINTER(C.c_int64)]),
         "echo_fo_width": (C.c_int, [P]),
@@ -308,6 +312,20 @@ class EchoGrid:
         st, mx = C.c_int32(), C.c_double()
         self._check(self.lib.echo_get_stop(self.h, C.byref(st), C.byref(mx)))
         return st.value, mx.value
+
+    # -- restart dump (out_dump / out_restart) ---------------------------
+    def set_dump(self, on=True):
+        """keep vold / uold on the device for echo_dump"""
+        self._check(self.lib.echo_set_dump(self.h, int(bool(on))))
+
+    def dump(self, path, iout, thyp=0.0):
+        self._check(self.lib.echo_dump(self.h, os.fsencode(path), int(iout), float(thyp)))
+
+    def restart(self, path):
+        """read a dump.dat into the grid; returns (iout, thyp)"""
+        io, th = C.c_int32(0), C.c_double(0.0)
+        self._check(self.lib.echo_restart(self.h, os.fsencode(path), C.byref(io), C.byref(th)))
+        return io.value, th.value
 
     # -- freeze-out hypersurface (out_hyp / find_hypersurface_*) ----------
     def fo_setup(self, freeze_type, freeze_value):

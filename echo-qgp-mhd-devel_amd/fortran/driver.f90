@@ -16,19 +16,24 @@
 ! hypersurface.txt) at the cadence of echo.f90:176-177,252-258; with
 ! HYP_COMPU=1 the time loop advances one step per device call, as the
 ! reference checks the cadence after every step.
+! Restart: RESTART..=1 writes DUMP_DIR..= (default .)/dump.dat at every
+! output time (echo.f90:243-248); the driver flag RESUME...= names a
+! dump.dat to start from instead of the Gubser initial state
+! (out_restart, echo.f90:162-172).
 program echo_amd_driver
   use iso_c_binding
   use echo_amd_f
   implicit none
   type(echo_config) :: cfg
   type(c_ptr) :: grid
-  character(len=256) :: pfile, eos_file, dump_file, hyp_file
+  character(len=256) :: pfile, eos_file, dump_file, hyp_file, dump_dir, resume_file
   character(len=10) :: flag
   character(len=120) :: val
   integer :: ios, nsteps, chunk, dev, ierr, nv, iter, n, k
   integer(c_int32_t) :: nrows
-  real(8) :: t0gub, s(10), ttot, freeze_value, freeze_dt, thyp, hdr(2)
-  integer :: hyp_compu, freeze_kind
+  real(8) :: t0gub, s(10), ttot, freeze_value, freeze_dt, thyp, hdr(2), tout_prev
+  integer :: hyp_compu, freeze_kind, restart_type
+  integer(c_int32_t) :: iout
   integer(c_int64_t) :: nfo
   real(8), allocatable :: v(:,:,:,:), u(:,:,:,:), rows(:,:)
   integer(8) :: c0, c1, crate
@@ -47,6 +52,10 @@ program echo_amd_driver
   eos_file = 'qcdIEOS0.dat'
   dump_file = ' '
   hyp_file = 'hypersurface.txt'
+  dump_dir = '.'
+  resume_file = ' '
+  restart_type = 0
+  iout = 0
   hyp_compu = 0
   freeze_kind = 1
   freeze_value = 0.5d0
@@ -126,6 +135,9 @@ program echo_amd_driver
     case ('FREEZEVAL='); read(val, *) freeze_value
     case ('HYPSURFTI='); read(val, *) freeze_dt
     case ('HYP_FILE.='); hyp_file = adjustl(val)
+    case ('RESTART..='); read(val, *) restart_type
+    case ('DUMP_DIR.='); dump_dir = adjustl(val)
+    case ('RESUME...='); resume_file = adjustl(val)
     end select
   end do
   close(29)
@@ -143,19 +155,41 @@ program echo_amd_driver
   if (ierr /= 0) call fail('echo_create', ierr)
   nv = echo_nv(grid)
   allocate(v(cfg%nx, cfg%ny, cfg%nz, nv), u(cfg%nx, cfg%ny, cfg%nz, nv))
-  call gubser_init(v)
-  ierr = echo_init_from_prim(grid, v)
-  if (ierr /= 0) call fail('echo_init_from_prim', ierr)
-  ierr = echo_set_tout(grid, cfg%tmin + cfg%dtout)
+  if (restart_type /= 0) then
+    ierr = echo_set_dump(grid, 1_c_int32_t)
+    if (ierr /= 0) call fail('echo_set_dump', ierr)
+    chunk = 1                        ! dumps follow the step that reaches tout
+  end if
   if (hyp_compu == 1) then
-    ! get_hysufile_ready (hypersurface.f90:1137-1200)
+    ! get_hysufile_ready (hypersurface.f90:1137-1200): a restarted run appends
+    ! to the existing file; set up before the restart, which reads the
+    ! freeze-out part of the dump
     ierr = echo_fo_setup(grid, int(freeze_kind, c_int32_t), freeze_value, hdr)
     if (ierr /= 0) call fail('echo_fo_setup', ierr)
-    ierr = echo_fo_write(grid, trim(hyp_file)//c_null_char, 1_c_int32_t)
-    if (ierr /= 0) call fail('echo_fo_write', ierr)
+    if (len_trim(resume_file) == 0) then
+      ierr = echo_fo_write(grid, trim(hyp_file)//c_null_char, 1_c_int32_t)
+      if (ierr /= 0) call fail('echo_fo_write', ierr)
+    end if
     chunk = 1
   end if
-  thyp = cfg%tmin
+  if (len_trim(resume_file) > 0) then
+    ! out_restart; tout = t + dtout is set by echo_restart (echo.f90:164-172)
+    ierr = echo_restart(grid, trim(resume_file)//c_null_char, iout, thyp)
+    if (ierr /= 0) call fail('echo_restart', ierr)
+    iout = iout + 1
+    write(*,'(a,a)') ' restarted from ', trim(resume_file)
+  else
+    call gubser_init(v)
+    ierr = echo_init_from_prim(grid, v)
+    if (ierr /= 0) call fail('echo_init_from_prim', ierr)
+    ierr = echo_set_tout(grid, cfg%tmin + cfg%dtout)
+  end if
+  ierr = echo_get_scalars(grid, s)
+  if (s(1) == cfg%tmin) then
+    thyp = cfg%tmin
+  else
+    thyp = s(1) + freeze_dt          ! echo.f90:174-178 after a restart
+  end if
 
   write(*,'(a,3i6,a,i3,a,i2)') ' grid', cfg%nx, cfg%ny, cfg%nz, '  nv', nv, '  RK', cfg%nrk
   iter = 0
@@ -163,6 +197,7 @@ program echo_amd_driver
   ierr = echo_get_scalars(grid, s)
   do while (s(1) < cfg%tmax .and. iter < nsteps)
     n = min(chunk, nsteps - iter)
+    tout_prev = s(7)
     call system_clock(c0, crate)
     ierr = echo_steps(grid, int(n, c_int32_t))
     if (ierr /= 0) call fail('echo_steps', ierr)
@@ -171,9 +206,14 @@ program echo_amd_driver
     iter = iter + n
     ierr = echo_get_scalars(grid, s)
     write(*,'(a,i8,a,es14.6,a,es12.4)') ' iteration', iter, '  t', s(1), '  dt', s(2)
-    if (s(1) >= s(7)) then
-      ! output time reached (echo.f90:243-247): next tout
-      ierr = echo_set_tout(grid, s(7) + cfg%dtout)
+    if (s(7) /= tout_prev) then
+      ! an output time was reached (echo.f90:243-248; the grid advanced tout
+      ! itself): the restart dump
+      if (restart_type /= 0) then
+        ierr = echo_dump(grid, trim(dump_dir)//'/dump.dat'//c_null_char, iout, thyp)
+        if (ierr /= 0) call fail('echo_dump', ierr)
+      end if
+      iout = iout + 1
     end if
     if (hyp_compu == 1) then
       ! echo.f90:252-258

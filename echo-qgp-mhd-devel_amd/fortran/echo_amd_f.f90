@@ -15,7 +15,7 @@ module echo_amd_f
             echo_get_scalars, echo_set_tout, echo_cons2prim_batch, echo_set_timing, &
             echo_get_timing, echo_last_error, echo_error_message, echo_set_stop, echo_get_stop, &
             echo_write_frame, echo_wait_frames, echo_fo_setup, echo_fo_slice, echo_fo_width, &
-            echo_fo_elements, echo_fo_write
+            echo_fo_elements, echo_fo_write, echo_set_dump, echo_dump, echo_restart
 
   ! struct echo_config: 16 int32 then 19 double (param.dat keys in brackets)
   type, bind(C) :: echo_config
@@ -149,6 +149,26 @@ module echo_amd_f
     integer(c_int) function echo_wait_frames(grid) bind(C, name="echo_wait_frames")
       import :: c_int, c_ptr
       type(c_ptr), value :: grid
+    end function
+    ! restart dump: out_dump / out_restart (out.f90:1313-1483); path as a C string
+    integer(c_int) function echo_set_dump(grid, on) bind(C, name="echo_set_dump")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function echo_dump(grid, path, iout, thyp) bind(C, name="echo_dump")
+      import :: c_int, c_int32_t, c_ptr, c_char, c_double
+      type(c_ptr), value :: grid
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: iout
+      real(c_double), value :: thyp
+    end function
+    integer(c_int) function echo_restart(grid, path, iout, thyp) bind(C, name="echo_restart")
+      import :: c_int, c_int32_t, c_ptr, c_char, c_double
+      type(c_ptr), value :: grid
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), intent(out) :: iout
+      real(c_double), intent(out) :: thyp
     end function
     ! freeze-out hypersurface: get_hysufile_ready + out_hyp (out.f90:1568-1645)
     integer(c_int) function echo_fo_setup(grid, freeze_type, freeze_value, hdr) bind(C, name="echo_fo_setup")

@@ -167,6 +167,26 @@ int echo_wait_frames(echo_grid *g);
 int echo_set_stop(echo_grid *g, double temp_end, int32_t freeze_type);
 int echo_get_stop(echo_grid *g, int32_t *stopped, double *max_value);
 
+/* Restart dump: out_dump / out_restart (out.f90:1313-1483) in the
+ * reference's dump.dat format (a Fortran stream file: 4-byte iout, t,
+ * timeold, timeinterval, v, u, vold, uold -- components krh..kvold_end,
+ * 5 for hydro, 8 for MHD --, g_cov_3_old, and with echo_fo_setup active:
+ * thyp, time_arr, a primitive slice, field(2,:,:,:)).
+ * echo_set_dump(g, 1) keeps vold / uold on the device (one copy of
+ * 2 * kvold_end fields per step, at the step's start, as echo.f90:192-201);
+ * call it before the steps to be dumped (until a step has run, vold / uold
+ * are the current state, as init.f90:3271-3272).  echo_dump writes the
+ * file between steps; echo_restart reads one into the grid (t, timeold,
+ * timeinterval, g_cov_3_old, v, u, vold; tout = t + dtout as
+ * echo.f90:164-172) and returns iout and thyp.  In the freeze-out part the
+ * primitive slice is the one the next echo_fo_slice compares against (the
+ * reference stores its older slice there, which its own restart cannot
+ * use: the next out_hyp overwrites it with primitives, which the file does
+ * not hold). */
+int echo_set_dump(echo_grid *g, int32_t on);
+int echo_dump(echo_grid *g, const char *path, int32_t iout, double thyp);
+int echo_restart(echo_grid *g, const char *path, int32_t *iout, double *thyp);
+
 /* Freeze-out hypersurface: the out_freeze path of the reference.
  * echo_fo_setup replaces allocate_hypersurface_arrays
  * (hypersurface.f90:93-168) and get_hysufile_ready (1137-1200): freeze-out

@@ -89,3 +89,45 @@ def test_fortran_driver_freezeout(oracle, tmp_path):
     for la, lb in zip(a[1:], b[1:]):
         va, vb = np.array(la.split(), dtype=float), np.array(lb.split(), dtype=float)
         assert np.all(np.abs(va - vb) <= 1.5e-7 * np.abs(vb) + 1e-300), (la, lb)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
+def test_fortran_driver_restart(oracle, tmp_path):
+    """RESTART..=1 writes dump.dat at the output times; RESUME...= starts a
+    second run from it.  The oracle, restarted from the same file, takes the
+    same steps to 1e-10."""
+    from libs import oracle_restart
+    n, dtout = 32, 0.05
+    ic, rc, v0 = gubser2d(n=n)
+    half = n * 0.1 / 2
+    base = ["INIT_TYPE=4", "COORD....=2", f"NX.......={n}", f"NY.......={n}", "NZ.......=1",
+            f"XMIN.....={-half!r}", f"XMAX.....={half!r}", f"YMIN.....={-half!r}",
+            f"YMAX.....={half!r}", "ZMIN.....=-0.5", "ZMAX.....=0.5", "TSTART...=1.0",
+            "CFL......=0.4", "MAXDT....=1.0", "REC_ALGO.=WENO5", "INT_ALGO.=RK3",
+            "EOS......=1", "bound_con=2", f"GUB_T0...={GUBSER_THAT0!r}", f"DTOUT....={dtout!r}"]
+    run1 = tmp_path / "run1.dat"
+    run1.write_text("\n".join(base + ["NSTEPS...=6", "RESTART..=1", f"DUMP_DIR.={tmp_path}"]) + "\n")
+    r = subprocess.run([DRIVER, str(run1)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dump = tmp_path / "dump.dat"
+    assert dump.exists()
+    final = tmp_path / "final.bin"
+    run2 = tmp_path / "run2.dat"
+    run2.write_text("\n".join(base + ["NSTEPS...=5", f"RESUME...={dump}", f"DUMP_FILE={final}"]) + "\n")
+    r = subprocess.run([DRIVER, str(run2)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "restarted from" in r.stdout
+    raw = np.fromfile(final, dtype=np.float64)
+    nv = 5
+    v = raw[:nv * n * n].reshape(nv, 1, n, n)
+    u = raw[nv * n * n:].reshape(nv, 1, n, n)
+    rc2 = rc.copy()
+    rc2[16] = dtout
+    oracle.setup(ic, rc2)
+    e, iout, _ = oracle_restart(oracle, dump)
+    assert e == 0 and iout >= 0
+    assert oracle.steps(5) == 0
+    vo, uo = oracle.state()
+    assert rel_linf(v, vo) <= 1e-10
+    assert rel_linf(u, uo) <= 1e-10
