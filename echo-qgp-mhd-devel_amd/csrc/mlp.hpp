@@ -82,25 +82,32 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // Network inputs of one de-densitised state (u[0..7] = D, S_i, U, B^i),
-// 8 scale-free features of S^2/U^2, B^2/U, (S.B)^2/(S^2 B^2) and D/U.
+// 8 scale-free features of S^2/U^2, B^2/U, (S.B)^2/(S^2 B^2) and D/U.  The
+// ratios and 1 - S^2/U^2 (which cancels as v -> 1) are formed in FP64; the
+// logarithms, the square root and the two quotients that follow only feed
+// bfloat16 inputs and are evaluated in FP32 with the hardware log2,
+// sqrt and reciprocal (the FP64 library calls took a third of the kernel).
 __device__ __forceinline__ void features(const double* u, double g33, float* x) {
   const double S2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3] / g33;
   const double B2 = u[5] * u[5] + u[6] * u[6] + u[7] * u[7] * g33;
   const double SB = u[1] * u[5] + u[2] * u[6] + u[3] * u[7];
-  const double U = u[4];
-  const double s2 = S2 / (U * U), b2 = B2 / U;
+  const double iU = rcp_nr(u[4]);
+  const double s2 = S2 * (iU * iU), b2 = B2 * iU;
   const double den = S2 * B2;
-  const double q = den > 0.0 ? (SB * SB) / den : 0.0;
-  const double d = u[0] / U;
+  const double q = den > 0.0 ? (SB * SB) * rcp_nr(den) : 0.0;
+  const double d = u[0] * iU;
   const double oms = dmax(1.0 - s2, 1e-30);
-  x[0] = (float)(log(oms) / 3.0 + 1.0);
-  x[1] = (float)(2.0 * s2 - 1.0);
-  x[2] = (float)(log(dmax(b2, 1e-30)) / 4.0 + 1.0);
-  x[3] = (float)(2.0 * b2 / (1.0 + b2) - 1.0);
-  x[4] = (float)(2.0 * q - 1.0);
-  x[5] = (float)(log(dmax(d, 1e-30)) / 8.0);
-  x[6] = (float)(2.0 * sqrt(q) - 1.0);
-  x[7] = (float)(2.0 * oms / (oms + b2) - 1.0);
+  const float LN2 = 0.69314718f;
+  const float omsf = (float)oms, b2f = (float)dmax(b2, 1e-30), df = (float)dmax(d, 1e-30);
+  const float s2f = (float)s2, qf = (float)q;
+  x[0] = __builtin_amdgcn_logf(omsf) * (LN2 / 3.0f) + 1.0f;
+  x[1] = 2.0f * s2f - 1.0f;
+  x[2] = __builtin_amdgcn_logf(b2f) * (LN2 / 4.0f) + 1.0f;
+  x[3] = 2.0f * b2f * __builtin_amdgcn_rcpf(1.0f + b2f) - 1.0f;
+  x[4] = 2.0f * qf - 1.0f;
+  x[5] = __builtin_amdgcn_logf(df) * (LN2 / 8.0f);
+  x[6] = 2.0f * __builtin_amdgcn_sqrtf(qf) - 1.0f;
+  x[7] = 2.0f * omsf * __builtin_amdgcn_rcpf(omsf + b2f) - 1.0f;
 }
 
 // Baseline of the W head: W0 = U' + p0, p0 the conformal hydro closed form
@@ -137,9 +144,9 @@ __device__ __forceinline__ void targets(const double* u, const double* v, double
 // alone (c3 = (3 + x)/4, c2 = B^2 (1 + x)/2 - U), from the network's W.
 __device__ __forceinline__ void e3p_w(double W, double S2, double B2, double SB2, double U, double& f,
                                       double& df, double& x, double& dxdw) {
-  const double iw = 1.0 / W;
+  const double iw = rcp_nr(W);
   const double wb = W + B2;
-  const double iwb = 1.0 / wb;
+  const double iwb = rcp_nr(wb);
   const double sbw2 = SB2 * iw * iw;
   x = (S2 + (2.0 * W + B2) * sbw2) * iwb * iwb;
   dxdw = (2.0 * sbw2 - 2.0 * (2.0 * W + B2) * sbw2 * iw) * iwb * iwb - 2.0 * x * iwb;
@@ -159,21 +166,22 @@ __device__ __forceinline__ bool refine(const double* u, float y1, double g33, in
   const double B2 = u[5] * u[5] + u[6] * u[6] + u[7] * u[7] * g33;
   const double SB = u[1] * u[5] + u[2] * u[6] + u[3] * u[7];
   const double SB2 = SB * SB, U = u[4];
-  double W = w_baseline(S2, B2, U) * exp((double)y1 / 25.0);
+  // the guess needs no more than float precision: exp2 on the hardware
+  double W = w_baseline(S2, B2, U) * (double)__builtin_amdgcn_exp2f(y1 * (1.44269504f / 25.0f));
   double f, df, x, dxdw;
   for (int it = 0; it < iters; it++) {
     e3p_w(W, S2, B2, SB2, U, f, df, x, dxdw);
-    W -= f / df;
+    W -= f * rcp_nr(df);
   }
   e3p_w(W, S2, B2, SB2, U, f, df, x, dxdw);
-  const double dW = f / df;
+  const double dW = f * rcp_nr(df);
   // the correction's effect on W and on 1 - v^2 (p and rho scale with it)
   const bool ok = (W > 0.0) && (x >= 0.0) && (x < 1.0) && (fabs(dW) <= tol * W) &&
                   (fabs(dxdw * dW) <= tol * (1.0 - x));
   if (!ok) return false;
   // primitives as at the end of algorithm 5 (system.f90:990-998)
-  const double vb = SB / W;
-  const double iwb = 1.0 / (W + B2);
+  const double vb = SB * rcp_nr(W);
+  const double iwb = rcp_nr(W + B2);
   v[0] = u[0] * sqrt(1.0 - x);
   v[4] = 0.25 * (1.0 - x) * W;
   v[1] = (u[1] + vb * u[5]) * iwb;

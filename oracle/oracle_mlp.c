@@ -86,7 +86,10 @@ static void invariants(const double *u, double g33, double *S2, double *B2, doub
 }
 
 /* Network input: 8 scale-free features of u (u already divided by sqrt(-g)).
- * With s2 = S^2/U^2, b2 = B^2/U, q = (S.B)^2/(S^2 B^2), d = D/U. */
+ * With s2 = S^2/U^2, b2 = B^2/U, q = (S.B)^2/(S^2 B^2), d = D/U: the ratios
+ * and 1 - s2 in FP64, the logarithms, square root and the two quotients in
+ * float, as the kernel does (its hardware log2 / sqrt / reciprocal agree
+ * with the float library to an ulp or two). */
 void orc_mlp_features(const double *u, double g33, float *x) {
   double S2, B2, SB;
   invariants(u, g33, &S2, &B2, &SB);
@@ -96,14 +99,16 @@ void orc_mlp_features(const double *u, double g33, float *x) {
   const double q = den > 0.0 ? (SB * SB) / den : 0.0;
   const double d = u[0] / U;
   const double oms = fmax(1.0 - s2, 1e-30);
-  x[0] = (float)(log(oms) / 3.0 + 1.0);
-  x[1] = (float)(2.0 * s2 - 1.0);
-  x[2] = (float)(log(fmax(b2, 1e-30)) / 4.0 + 1.0);
-  x[3] = (float)(2.0 * b2 / (1.0 + b2) - 1.0);
-  x[4] = (float)(2.0 * q - 1.0);
-  x[5] = (float)(log(fmax(d, 1e-30)) / 8.0);
-  x[6] = (float)(2.0 * sqrt(q) - 1.0);
-  x[7] = (float)(2.0 * oms / (oms + b2) - 1.0);
+  const float omsf = (float)oms, b2f = (float)fmax(b2, 1e-30), df = (float)fmax(d, 1e-30);
+  const float s2f = (float)s2, qf = (float)q;
+  x[0] = logf(omsf) / 3.0f + 1.0f;
+  x[1] = 2.0f * s2f - 1.0f;
+  x[2] = logf(b2f) / 4.0f + 1.0f;
+  x[3] = 2.0f * b2f / (1.0f + b2f) - 1.0f;
+  x[4] = 2.0f * qf - 1.0f;
+  x[5] = logf(df) / 8.0f;
+  x[6] = 2.0f * sqrtf(qf) - 1.0f;
+  x[7] = 2.0f * omsf / (omsf + b2f) - 1.0f;
 }
 
 /* Baseline of the W head: W0 = U' + p0 with p0 the conformal hydro closed
@@ -274,7 +279,8 @@ int orc_mlp_refine(const double *u_in, const float *y, double g33, double gm, in
   double S2, B2, SB;
   invariants(u, g33, &S2, &B2, &SB);
   const double SB2 = SB * SB, U = u[4];
-  double W = orc_mlp_w0(u, g33) * exp((double)y[1] / 25.0);
+  /* the guess in float precision, as the kernel's hardware exp2 */
+  double W = orc_mlp_w0(u, g33) * (double)exp2f(y[1] * (1.44269504f / 25.0f));
   double f, df, x, dxdw;
   for (int it = 0; it < iters; it++) {
     e3p_w(W, S2, B2, SB2, U, &f, &df, &x, &dxdw);
