@@ -95,9 +95,14 @@ __host__ __device__ __forceinline__ long long flux_tiles(int nx, int ny, int nz,
 // Launch bounds (FLUX_BLOCK, 2): at most 256 VGPRs, so two waves share each
 // SIMD.  Unbounded, the viscous WENO5 kernel takes 320 VGPRs (one wave per
 // SIMD) and its FP64 chains and stencil loads stall with nothing to switch
-// to.  Bounded, that variant spills about 48 VGPRs per lane to scratch
-// (cache-resident) and still runs its sweeps 3-7% faster on MI355X; the
-// other variants fit in 256 VGPRs without spills.
+// to.  Under the bound the reconstructed states wl, wr (2 x NV doubles per
+// lane) would not fit beside the side states and the compiler would spill
+// about 48 VGPRs per lane to scratch, which lives in HBM (about 1.7 GB of
+// extra traffic per sweep on 192 x 192 x 128).  They are parked in LDS
+// instead (2 x NV x FLUX_BLOCK doubles, 48 KB per block for NV = 12: two
+// blocks per CU fit) and re-read per flux component; each lane touches only
+// its own column, so no barrier is needed.  The viscous WENO5 kernel then
+// takes 210 VGPRs with no spills.
 template <int DIR, int REC, int SYS, bool ZZ, int TY = 1>
 __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
@@ -219,9 +224,28 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
       wl[KB] = 0.5 * (wl[KB] + wr[KB]) - (wr[KGLM] - wl[KGLM]) / (2 * ch);
       wr[KB] = wl[KB];
     }
+    // Park wl, wr in LDS.  side_state completes the pi^{tau mu} components
+    // of its input (derived_pi), so each side is stored after its own
+    // side_state.  The empty asm statements with a memory clobber keep the
+    // compiler from forwarding the stored values in registers across the
+    // side-state computations, which is what would spill.
+    __shared__ double sw[2][NV][FLUX_BLOCK];
+#pragma unroll
+    for (int k = 0; k < NV; k++) sw[1][k][threadIdx.x] = wr[k];
     SideState sl, sr;
     int e = side_state<DIR, SYS, ZZ>(a.eos, M, wl, sl);
-    if (e == 0) e = side_state<DIR, SYS, ZZ>(a.eos, M, wr, sr);
+#pragma unroll
+    for (int k = 0; k < NV; k++) sw[0][k][threadIdx.x] = wl[k];
+    asm volatile("" ::: "memory");
+    {
+      double wv[NV];
+#pragma unroll
+      for (int k = 0; k < NV; k++) wv[k] = sw[1][k][threadIdx.x];
+      if (e == 0) e = side_state<DIR, SYS, ZZ>(a.eos, M, wv, sr);
+#pragma unroll
+      for (int k = 0; k < NV; k++) sw[1][k][threadIdx.x] = wv[k];
+    }
+    asm volatile("" ::: "memory");
     if (e != 0) {
       // the reference stops the run (run_crashed) on a flux error
       atomicMax(&a.S->errcode, e);
@@ -241,10 +265,13 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
       double a1 = (ap == 0.0 && am == 0.0) ? 1. : rcp_nr(ap + am);
 #pragma unroll
       for (int k = 0; k < NV; k++) {
-        double ulk = u_comp<SYS>(k, M, sl, wl) * M.gp;
-        double urk = u_comp<SYS>(k, M, sr, wr) * M.gp;
-        double flk = f_comp<DIR, SYS>(k, M, sl, wl) * M.gp;
-        double frk = f_comp<DIR, SYS>(k, M, sr, wr) * M.gp;
+        double ql[NV], qr[NV];
+#pragma unroll
+        for (int j = 0; j < NV; j++) { ql[j] = sw[0][j][threadIdx.x]; qr[j] = sw[1][j][threadIdx.x]; }
+        double ulk = u_comp<SYS>(k, M, sl, ql) * M.gp;
+        double urk = u_comp<SYS>(k, M, sr, qr) * M.gp;
+        double flk = f_comp<DIR, SYS>(k, M, sl, ql) * M.gp;
+        double frk = f_comp<DIR, SYS>(k, M, sr, qr) * M.gp;
         F[(long long)k * nf + f] = a1 * (ap * flk + am * frk - ap * am * (urk - ulk));
       }
     }
