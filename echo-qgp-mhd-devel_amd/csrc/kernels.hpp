@@ -92,19 +92,20 @@ __host__ __device__ __forceinline__ long long flux_tiles(int nx, int ny, int nz,
   return ntx * nti * (DIR == 2 ? nz : ny);
 }
 
-// Launch bounds (FLUX_BLOCK, 2): at most 256 VGPRs, so two waves share each
-// SIMD.  Unbounded, the viscous WENO5 kernel takes 320 VGPRs (one wave per
-// SIMD) and its FP64 chains and stencil loads stall with nothing to switch
-// to.  Under the bound the reconstructed states wl, wr (2 x NV doubles per
-// lane) would not fit beside the side states and the compiler would spill
-// about 48 VGPRs per lane to scratch, which lives in HBM (about 1.7 GB of
-// extra traffic per sweep on 192 x 192 x 128).  They are parked in LDS
-// instead (2 x NV x FLUX_BLOCK doubles, 48 KB per block for NV = 12: two
-// blocks per CU fit) and re-read per flux component; each lane touches only
-// its own column, so no barrier is needed.  The viscous WENO5 kernel then
-// takes 210 VGPRs with no spills.
+// Launch bounds (FLUX_BLOCK, 3): at most 168 VGPRs, so three waves share
+// each SIMD.  Unbounded, the viscous WENO5 kernel takes 320 VGPRs (one wave
+// per SIMD) and its FP64 chains and stencil loads stall with nothing to
+// switch to.  The reconstructed states are what does not fit: 2 x NV doubles
+// per lane beside the stencils in flight and the side states.  They wait in
+// LDS instead, sw[2][NV][FLUX_BLOCK] (48 KB per block for NV = 12, so three
+// blocks per CU fit).  In viscous runs the pi^{mu nu} components go there as
+// soon as they are reconstructed; rho, v and p stay in registers for the
+// repair.  Each lane touches only its own column of sw, so no barrier is
+// needed.  Every variant then fits without scratch spills; before this, the
+// viscous WENO5 kernel spilled ~48 VGPRs per lane to scratch, which lives in
+// HBM (about 1.7 GB of extra traffic per sweep on 192 x 192 x 128).
 template <int DIR, int REC, int SYS, bool ZZ, int TY = 1>
-__global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double* __restrict__ V,
+__global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
                                                      double* __restrict__ partial) {
   constexpr bool VISC = SysT<SYS>::VISC;
@@ -176,7 +177,10 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
         L[q * nfl + f] = l + r;
       }
     }
-    // c0: the cell left of the interface, used by the repair (KVX..KPR)
+    // c0: the cell left of the interface, used by the repair (KVX..KPR).
+    // Components k >= KREG of wl, wr go straight to sw.
+    __shared__ double sw[2][NV][FLUX_BLOCK];
+    constexpr int KREG = VISC ? KPR + 1 : NV;
     double wl[NV], wr[NV], c0[5];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -184,7 +188,9 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
       const double* Vk = V + (long long)k * a.N;
 #pragma unroll
       for (int j = 0; j < 6; j++) s[j] = Vk[off[j]];
-      rec_face<REC>(s, wl[k], wr[k]);
+      double l, r;
+      rec_face<REC>(s, l, r);
+      if (k < KREG) { wl[k] = l; wr[k] = r; } else { sw[0][k][threadIdx.x] = l; sw[1][k][threadIdx.x] = r; }
       if (k <= KPR) c0[k] = s[2];
     }
     // repair of reconstructed states with the cell left of the interface
@@ -224,14 +230,16 @@ __global__ __launch_bounds__(FLUX_BLOCK, 2) void k_flux(GridArgs a, const double
       wl[KB] = 0.5 * (wl[KB] + wr[KB]) - (wr[KGLM] - wl[KGLM]) / (2 * ch);
       wr[KB] = wl[KB];
     }
-    // Park wl, wr in LDS.  side_state completes the pi^{tau mu} components
-    // of its input (derived_pi), so each side is stored after its own
-    // side_state.  The empty asm statements with a memory clobber keep the
-    // compiler from forwarding the stored values in registers across the
-    // side-state computations, which is what would spill.
-    __shared__ double sw[2][NV][FLUX_BLOCK];
+    // Park wr in LDS while the left side state is formed, then wl while the
+    // right one is.  side_state completes the pi^{tau mu} components of its
+    // input (derived_pi), so each side is stored after its own side_state.
+    // The empty asm statements with a memory clobber keep the compiler from
+    // forwarding the stored values in registers across the side-state
+    // computations, which is what would spill.
 #pragma unroll
-    for (int k = 0; k < NV; k++) sw[1][k][threadIdx.x] = wr[k];
+    for (int k = 0; k < KREG; k++) sw[1][k][threadIdx.x] = wr[k];
+#pragma unroll
+    for (int k = KREG; k < NV; k++) wl[k] = sw[0][k][threadIdx.x];
     SideState sl, sr;
     int e = side_state<DIR, SYS, ZZ>(a.eos, M, wl, sl);
 #pragma unroll
