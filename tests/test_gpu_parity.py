@@ -71,6 +71,16 @@ def test_fireball_viscous(oracle, eqstate, smooth, coords):
     assert rel_linf(ug, uo) <= TOL
 
 
+@pytest.mark.parametrize("rec", ["TVD2", "MPE5", "WENO3", "CENO3", "MPE3"])
+def test_fireball_viscous_recon(oracle, rec):
+    """The viscous flux sweep keeps the reconstructed pi components apart
+    from rho, v, p: every reconstruction goes through that path."""
+    ic, rc, v0 = fireball3d(viscosity=1, rec=rec)
+    (vg, ug), (vo, uo) = _run_both(oracle, ic, rc, v0, 4)
+    assert rel_linf(vg, vo) <= TOL
+    assert rel_linf(ug, uo) <= TOL
+
+
 def test_fireball_viscous_no_zz(oracle):
     ic, rc, v0 = fireball3d(viscosity=1, obtained_zz=0, rec="PPM4", nrk=2)
     (vg, ug), (vo, uo) = _run_both(oracle, ic, rc, v0, 6)
