@@ -174,7 +174,9 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
       for (int q = 0; q < 4; q++) {
         double l, r;
         rec_face<REC>(wd[q], l, r);
-        L[q * nfl + f] = l + r;
+        // outputs are streamed: non-temporal stores keep them from
+        // evicting the stencil rows that neighbouring blocks still read
+        __builtin_nontemporal_store(l + r, &L[q * nfl + f]);
       }
     }
     // c0: the cell left of the interface, used by the repair (KVX..KPR).
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
         double urk = u_comp<SYS>(k, M, sr, qr) * M.gp;
         double flk = f_comp<DIR, SYS>(k, M, sl, ql) * M.gp;
         double frk = f_comp<DIR, SYS>(k, M, sr, qr) * M.gp;
-        F[(long long)k * nf + f] = a1 * (ap * flk + am * frk - ap * am * (urk - ulk));
+        __builtin_nontemporal_store(a1 * (ap * flk + am * frk - ap * am * (urk - ulk)),
+                                    &F[(long long)k * nf + f]);
       }
     }
   }
@@ -448,7 +451,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   }
   double v[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = V[k * N + c];
+  for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(&V[k * N + c]);   // read once
   Deriv D{};
   const int par = S->vs_parity;
   double* VSn = par ? VS1 : VS0;
@@ -466,7 +469,10 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   const double dt = S->dt;
   double u0[NV], uc[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) { u0[k] = U0[k * N + c]; uc[k] = Ucur[k * N + c]; }
+  for (int k = 0; k < NV; k++) {
+    u0[k] = __builtin_nontemporal_load(&U0[k * N + c]);
+    uc[k] = __builtin_nontemporal_load(&Ucur[k * N + c]);
+  }
   if constexpr (SysT<SYS>::GLM) {
     if (first_stage) {
       // evolve_glm_source(dt/2) before u0 = u (evolve.f90:196): the damped
@@ -501,12 +507,19 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
       prim2cons<SYS, ZZ>(a.eos, M, v, un);
     }
   }
+  // streamed outputs (read again only by the next kernel): non-temporal
+  // stores, as in k_flux
 #pragma unroll
-  for (int k = 0; k < NV; k++) { Uout[k * N + c] = un[k]; V[k * N + c] = v[k]; }
+  for (int k = 0; k < NV; k++) {
+    __builtin_nontemporal_store(un[k], &Uout[k * N + c]);
+    __builtin_nontemporal_store(v[k], &V[k * N + c]);
+  }
   if (VISC && last_stage) {
     // velocities at the start of the next step (vnewstep, echo.f90:192)
     double* VSnext = VSo;
-    VSnext[c] = v[KVX]; VSnext[N + c] = v[KVY]; VSnext[2 * N + c] = v[KVZ];
+    __builtin_nontemporal_store(v[KVX], &VSnext[c]);
+    __builtin_nontemporal_store(v[KVY], &VSnext[N + c]);
+    __builtin_nontemporal_store(v[KVZ], &VSnext[2 * N + c]);
   }
 }
 
