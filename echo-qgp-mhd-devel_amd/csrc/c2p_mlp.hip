@@ -16,9 +16,9 @@ struct echo_mlp {
   float* m1 = nullptr;      // Adam moments
   float* m2 = nullptr;
   float* G = nullptr;       // gradient
-  float* Wq = nullptr;      // bfloat16-rounded weights (backward pass)
+  float* Wq = nullptr;      // half-precision weights as float (backward pass)
   float* bias = nullptr;    // accumulator biases
-  __bf16* packed = nullptr; // MFMA A operands
+  _Float16* packed = nullptr; // MFMA A operands
   // training scratch for `cap` states per batch
   long long cap = 0;
   float *ybuf = nullptr, *xbuf = nullptr, *zbuf = nullptr, *hbuf = nullptr, *dzbuf = nullptr;
@@ -142,7 +142,7 @@ int echo_mlp_create(int32_t device, echo_mlp** out) {
   MALLOC(m->G, pb);
   MALLOC(m->Wq, pb);
   MALLOC(m->bias, sizeof(float) * mlp::BS_N);
-  MALLOC(m->packed, sizeof(__bf16) * mlp::PK_N);
+  MALLOC(m->packed, sizeof(_Float16) * mlp::PK_N);
 #undef MALLOC
   HIPCHECK(hipMemset(m->P, 0, pb));
   HIPCHECK(hipMemset(m->m1, 0, pb));

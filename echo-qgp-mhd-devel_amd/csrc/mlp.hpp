@@ -1,5 +1,5 @@
 // Learned con2prim for ideal MHD with p = e/3: an 8 -> 64 -> 64 -> 64 -> 5
-// network evaluated on MFMA (bfloat16 operands, FP32 accumulation) gives the
+// network evaluated on MFMA (half-precision operands, FP32 accumulation) gives the
 // initial guess of the inversion (W = (e+p) gamma^2, with v^2 following
 // from W exactly); FP64 Newton iterations refine it and a residual gate
 // sends every state it does not accept to the plain solver (algorithm 5 of system_cons2prim,
@@ -28,24 +28,29 @@ namespace mlp {
 constexpr int NI = 8, NH = 64, NO = 5, NPARAM = 9221;
 constexpr int OFF_W1 = 0, OFF_B1 = 512, OFF_W2 = 576, OFF_B2 = 4672, OFF_W3 = 4736, OFF_B3 = 8832,
               OFF_W4 = 8896, OFF_B4 = 9216;
-// packed bf16 A operands: layer 1 4 x 64 x 4, layers 2-3 8 x 64 x 8, layer 4 2 x 64 x 8
+// packed half-precision A operands: layer 1 4 x 64 x 4, layers 2-3 8 x 64 x 8, layer 4 2 x 64 x 8
 constexpr int PK_L1 = 0, PK_L2 = 1024, PK_L3 = 5120, PK_L4 = 9216, PK_N = 10240;
 // biases as the accumulator initial values: 64, 64, 64, 16 (rows 5..15 zero)
 constexpr int BS_L1 = 0, BS_L2 = 64, BS_L3 = 128, BS_L4 = 192, BS_N = 208;
 constexpr int BLOCK = 256;            // 4 waves, each 64 states per tile
 constexpr int BWD_BLOCK = 128;
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // activation: hard-swish h(x) = x * clamp(x / 6 + 1/2, 0, 1) -- 0 below -3,
-// x above 3, x^2/6 + x/2 between.  The clamp folds into the clamp bit of
-// the FMA, so an activation is two VALU instructions (FMA with clamp,
-// multiply) plus half a packed bf16 conversion; the degree-9 polynomial
-// GELU it replaced took nine.  The network is VALU-bound on its
-// activations (192 per state against 18 kflop of MFMA work).
+// x above 3, x^2/6 + x/2 between.  act and act_d are the FP32 function and
+// its derivative (training backward pass).  The forward pass evaluates it in
+// half precision on pairs of values, as the operands of the next MFMA:
+// x = half(z); t = clamp(fma(x, half(c6), 1/2), 0, 1) (one rounding, the
+// clamp bit of the packed FMA); h = half(x * t).  That is one packed
+// conversion, one packed FMA and one packed multiply per two values (1.5
+// VALU instructions per activation; 2.5 in FP32 with a bfloat16 conversion).
+// The network is VALU-bound on its activations (192 per state against
+// 18 kflop of MFMA work).
 #define ACT_C6 0.16666667f
 
 __device__ __forceinline__ float act(float x) {
@@ -59,33 +64,41 @@ __device__ __forceinline__ float act_d(float x) {
   return (a > 0.0f && a < 1.0f) ? __builtin_fmaf(x, ACT_C6, sv) : sv;
 }
 
-__device__ __forceinline__ float bfround(float x) { return (float)(__bf16)x; }
+__device__ __forceinline__ float h16round(float x) { return (float)(_Float16)x; }
 
-__device__ __forceinline__ bf16x4 act4(f32x4 a) {
-  bf16x4 r;
-#pragma unroll
-  for (int j = 0; j < 4; j++) r[j] = (__bf16)act(a[j]);
-  return r;
+// two activations in half precision (see above)
+__device__ __forceinline__ h16x2 act2h(f32x2 z) {
+  const h16x2 x = __builtin_convertvector(z, h16x2);
+  const h16x2 c6 = {(_Float16)ACT_C6, (_Float16)ACT_C6};
+  const h16x2 hf = {(_Float16)0.5f, (_Float16)0.5f};
+  h16x2 t = __builtin_elementwise_fma(x, c6, hf);
+  t = __builtin_elementwise_min(__builtin_elementwise_max(t, (h16x2){0, 0}), (h16x2){1, 1});
+  return x * t;
 }
 
-__device__ __forceinline__ bf16x8 cat(bf16x4 a, bf16x4 b) {
+__device__ __forceinline__ h16x4 act4(f32x4 a) {
+  const h16x2 lo = act2h((f32x2){a[0], a[1]});
+  const h16x2 hi = act2h((f32x2){a[2], a[3]});
+  return (h16x4){lo[0], lo[1], hi[0], hi[1]};
+}
+
+__device__ __forceinline__ h16x8 cat(h16x4 a, h16x4 b) {
   return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma32(h16x8 a, h16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, a),
-                                                   __builtin_bit_cast(s16x4, b), c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma16(h16x4 a, h16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
 // Network inputs of one de-densitised state (u[0..7] = D, S_i, U, B^i),
 // 8 scale-free features of S^2/U^2, B^2/U, (S.B)^2/(S^2 B^2) and D/U.  The
 // ratios and 1 - S^2/U^2 (which cancels as v -> 1) are formed in FP64; the
 // logarithms, the square root and the two quotients that follow only feed
-// bfloat16 inputs and are evaluated in FP32 with the hardware log2,
+// half-precision inputs and are evaluated in FP32 with the hardware log2,
 // sqrt and reciprocal (the FP64 library calls took a third of the kernel).
 __device__ __forceinline__ void features(const double* u, double g33, float* x) {
   const double S2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3] / g33;
@@ -196,7 +209,7 @@ __device__ __forceinline__ bool refine(const double* u, float y1, double g33, in
 enum { MODE_C2P = 0, MODE_FWD = 1, MODE_TRAIN = 2 };
 
 struct NetArgs {
-  const __bf16* packed;   // PK_N bf16 A operands
+  const _Float16* packed;   // PK_N half-precision A operands
   const float* bias;      // BS_N
 };
 
@@ -209,22 +222,25 @@ struct C2pArgs {
   int* fail_list;
 };
 
+// Launch bounds (BLOCK, 3): three waves per SIMD (at most 168 VGPRs; the
+// inference variant takes 160 with no spills).  With two, the FP64 Newton
+// steps and the MFMA chains stall with too little to switch to.
 // One kernel for inference (MODE_C2P), plain forward (MODE_FWD, outputs y)
 // and the training forward (MODE_TRAIN: y, network inputs h0, pre-activations
 // z1..z3 and rounded activations h1..h3, all [feature][state]).
 // Ub: conserved states [8 or 9][stride] starting at state `off`; n states.
 template <int MODE, int NVU>
-__global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const double* __restrict__ Ub,
+__global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const double* __restrict__ Ub,
                                                long long stride, long long off, long long n,
                                                double* __restrict__ Vb, int* __restrict__ status,
                                                float* __restrict__ ybuf, float* __restrict__ xbuf,
                                                float* __restrict__ zbuf, float* __restrict__ hbuf) {
-  __shared__ bf16x4 sA1[4 * 64];
-  __shared__ bf16x8 sA2[8 * 64];
-  __shared__ bf16x8 sA3[8 * 64];
-  __shared__ bf16x8 sA4[2 * 64];
+  __shared__ h16x4 sA1[4 * 64];
+  __shared__ h16x8 sA2[8 * 64];
+  __shared__ h16x8 sA3[8 * 64];
+  __shared__ h16x8 sA4[2 * 64];
   __shared__ float sBias[BS_N];
-  __shared__ bf16x8 sX[4][64];
+  __shared__ h16x8 sX[4][64];
   __shared__ float2 sY[4][64];
 
   {  // weights to LDS
@@ -268,9 +284,9 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
     {
       float x[8];
       features(u, g33, x);
-      bf16x8 xb;
+      h16x8 xb;
 #pragma unroll
-      for (int k = 0; k < 8; k++) xb[k] = (__bf16)x[k];
+      for (int k = 0; k < 8; k++) xb[k] = (_Float16)x[k];
       sX[w][lane] = xb;
       if (MODE == MODE_TRAIN && valid) {
 #pragma unroll
@@ -280,22 +296,22 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
     __syncthreads();
 
     // layer 1 (K = 8, padded to 16)
-    bf16x4 H[4][4];
+    h16x4 H[4][4];
     f32x4 acc[4][4];
     {
-      bf16x4 B0[4];
+      h16x4 B0[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const bf16x8 xv = sX[w][16 * q + c];
-        bf16x4 lo = __builtin_shufflevector(xv, xv, 0, 1, 2, 3);
-        bf16x4 hi = __builtin_shufflevector(xv, xv, 4, 5, 6, 7);
-        const bf16x4 zero = {(__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f};
+        const h16x8 xv = sX[w][16 * q + c];
+        h16x4 lo = __builtin_shufflevector(xv, xv, 0, 1, 2, 3);
+        h16x4 hi = __builtin_shufflevector(xv, xv, 4, 5, 6, 7);
+        const h16x4 zero = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
         B0[q] = g == 0 ? lo : (g == 1 ? hi : zero);
       }
 #pragma unroll
       for (int rb = 0; rb < 4; rb++) {
         const f32x4 b = *reinterpret_cast<const f32x4*>(&sBias[BS_L1 + 16 * rb + 4 * g]);
-        const bf16x4 a = sA1[rb * 64 + lane];
+        const h16x4 a = sA1[rb * 64 + lane];
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q][rb] = mfma16(a, B0[q], b);
       }
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
     // layers 2 and 3 (64 -> 64)
 #pragma unroll
     for (int L = 1; L <= 2; L++) {
-      const bf16x8* sA = (L == 1) ? sA2 : sA3;
+      const h16x8* sA = (L == 1) ? sA2 : sA3;
       const int bo = (L == 1) ? BS_L2 : BS_L3;
 #pragma unroll
       for (int rb = 0; rb < 4; rb++) {
@@ -336,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
       for (int ks = 0; ks < 2; ks++) {
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
-          const bf16x8 a = sA[(rb * 2 + ks) * 64 + lane];
+          const h16x8 a = sA[(rb * 2 + ks) * 64 + lane];
 #pragma unroll
           for (int q = 0; q < 4; q++) acc[q][rb] = mfma32(a, cat(H[q][2 * ks], H[q][2 * ks + 1]), acc[q][rb]);
         }
@@ -357,7 +373,7 @@ __global__ __launch_bounds__(BLOCK) void k_mlp(NetArgs net, C2pArgs ca, const do
       for (int q = 0; q < 4; q++) y4[q] = b;
 #pragma unroll
       for (int ks = 0; ks < 2; ks++) {
-        const bf16x8 a = sA4[ks * 64 + lane];
+        const h16x8 a = sA4[ks * 64 + lane];
 #pragma unroll
         for (int q = 0; q < 4; q++) y4[q] = mfma32(a, cat(H[q][2 * ks], H[q][2 * ks + 1]), y4[q]);
       }
@@ -437,7 +453,7 @@ static __global__ void k_zero_int(int* p, int n) {
 }
 
 // Backward pass, one thread per state: dL/dz of the four layers
-// ([feature][state]) and the loss sum.  Wq: the bfloat16-rounded weights
+// ([feature][state]) and the loss sum.  Wq: the half-precision weights
 // (as float, row-major [out][in]) that the forward pass used.
 static __global__ __launch_bounds__(BWD_BLOCK) void k_mlp_bwd(const float* __restrict__ Wq, const double* __restrict__ Ub,
                                                        const double* __restrict__ Vl, long long stride,
@@ -562,15 +578,15 @@ static __global__ void k_adam(float* __restrict__ P, float* __restrict__ m1, flo
   P[i] -= lr_t * a / (sqrtf(b) + eps);
 }
 
-// FP32 parameters -> packed bf16 A operands, accumulator biases and the
-// bfloat16-rounded weights used by the backward pass.
-static __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ packed,
+// FP32 parameters -> packed half-precision A operands, accumulator biases
+// and the rounded weights (as float) used by the backward pass.
+static __global__ void k_mlp_pack(const float* __restrict__ P, _Float16* __restrict__ packed,
                            float* __restrict__ bias, float* __restrict__ Wq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < NPARAM) {
     const bool isb = (i >= OFF_B1 && i < OFF_W2) || (i >= OFF_B2 && i < OFF_W3) ||
                      (i >= OFF_B3 && i < OFF_W4) || (i >= OFF_B4);
-    Wq[i] = isb ? P[i] : bfround(P[i]);
+    Wq[i] = isb ? P[i] : h16round(P[i]);
   }
   if (i < BS_N) {
     float b;
@@ -597,7 +613,7 @@ static __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restric
       const int k = 32 * ks + (m < 4 ? 4 * gg + m : 16 + 4 * gg + (m - 4));
       w = row < nrows ? P[offw + row * NH + k] : 0.0f;
     }
-    packed[i] = (__bf16)w;
+    packed[i] = (_Float16)w;
   }
 }
 

@@ -260,7 +260,7 @@ int echo_cons2prim_batch_dev(echo_grid *g, const double *d_u, double *d_v, int32
                              int64_t n);
 
 /* ---- learned con2prim (no counterpart in the reference) ----------------
- * An 8 -> 64 -> 64 -> 64 -> 5 network (bfloat16 MFMA, FP32 accumulation)
+ * An 8 -> 64 -> 64 -> 64 -> 5 network (half-precision MFMA, FP32 accumulation)
  * supplies the initial guess (v^2, W) of the ideal-MHD p = e/3 inversion
  * that the reference does with algorithm 5 of system_cons2prim
  * (system.f90:945-998, rtsafe + solver8); `iters` FP64 Newton iterations on

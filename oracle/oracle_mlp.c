@@ -14,12 +14,14 @@
  *   - the derivative of the refinement's residual by finite differences.
  *
  * Network: 8 -> 64 -> 64 -> 64 -> 5, activation hard-swish
- * h(x) = x * clamp(x * c6 + 1/2, 0, 1) with c6 the float nearest 1/6 (the
- * FMA is single-rounded in float, as the kernel's FMA with clamp).  With
- * round = 1 the forward pass
- * rounds the network input and every hidden activation to bfloat16 (round
- * to nearest even) and uses bfloat16-rounded weights, as the MFMA kernel
- * does; sums are accumulated in FP64 here and in FP32 on the GPU.
+ * h(x) = x * clamp(x * c6 + 1/2, 0, 1) with c6 the float nearest 1/6.  With
+ * round = 1 the forward pass follows the kernel's half-precision arithmetic
+ * (IEEE binary16, round to nearest even): the network input and the weights
+ * are rounded to half, and each hidden activation is evaluated in half from
+ * the pre-activation z as x = half(z), t = clamp(half(x * half(c6) + 1/2),
+ * 0, 1) (a fused multiply-add, one rounding), h = half(x * t).  Sums are
+ * accumulated in FP64 here and in FP32 on the GPU.  orc_f16 and orc_act_h
+ * are pinned against numpy's float16 in tests/test_mlp_oracle.py.
  *
  * Parameter block (float, MLP_NPARAM = 9221), row-major W[out][in]:
  *   W1[64][8] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[5][64] b4[5]
@@ -44,16 +46,33 @@ static const double LAMBDA[NO] = {1.0, 1.0, 0.25, 0.25, 0.25};
 
 #define ACT_C6 0.16666667f
 
-float orc_bf16(float x) {
-  uint32_t u;
-  memcpy(&u, &x, 4);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  u &= 0xFFFF0000u;
-  memcpy(&x, &u, 4);
-  return x;
+/* x rounded to IEEE binary16 (nearest, ties to even; subnormals down to
+ * 2^-24; +-inf beyond 65504 + half an ulp), returned exactly as a double.
+ * A binary16 value with exponent e (2^(e-1) <= |x| < 2^e) has an ulp of
+ * 2^(e-11), never below the subnormal ulp 2^-24. */
+double orc_f16(double x) {
+  if (x == 0.0 || !isfinite(x)) return x;
+  int e;
+  frexp(x, &e);
+  int q = e - 11;
+  if (q < -24) q = -24;
+  const double y = ldexp(nearbyint(ldexp(x, -q)), q);   /* ties to even */
+  return fabs(y) > 65504.0 ? copysign(INFINITY, x) : y;
 }
 
-/* float evaluation, explicit fma, the same sequence as the HIP kernel */
+/* The kernel's activation in half precision (see the header).  The
+ * products of two binary16 values and the sum with 1/2 are exact in
+ * double, so each orc_f16 below is the single rounding of the hardware
+ * operation. */
+double orc_act_h(float z) {
+  const double c6 = orc_f16(ACT_C6);
+  const double x = orc_f16(z);
+  const double t = fmin(fmax(orc_f16(x * c6 + 0.5), 0.0), 1.0);
+  return orc_f16(x * t);
+}
+
+/* the activation in float with an explicit fma (the function that the
+ * half-precision forward pass approximates) */
 float orc_act(float x) {
   const float a = fmaf(x, ACT_C6, 0.5f);
   return x * fminf(fmaxf(a, 0.0f), 1.0f);
@@ -141,22 +160,22 @@ void orc_mlp_targets(const double *u, const double *v, double g33, double *t) {
   t[4] = 2.0 * v2 - 1.0;
 }
 
-/* Forward pass of one state.  round = 1: bfloat16 input, weights and hidden
- * activations (float activation function), outputs rounded to float, as on
- * the GPU; round = 0: FP64 throughout.  zs (optional): the 3x64 hidden
+/* Forward pass of one state.  round = 1: half-precision input, weights and
+ * hidden activations (orc_act_h), outputs rounded to float, as on the GPU;
+ * round = 0: FP64 throughout.  zs (optional): the 3x64 hidden
  * pre-activations; hs (optional): the layer inputs h0 (8) then h1..h3 (64
  * each), as used by the next layer. */
 static void forward(const float *P, const float *x, int round, double *y, double *zs, double *hs) {
   double h[NH], z[NH];
   int n_in = NI;
-  for (int i = 0; i < NI; i++) h[i] = round ? orc_bf16(x[i]) : x[i];
+  for (int i = 0; i < NI; i++) h[i] = round ? orc_f16(x[i]) : x[i];
   if (hs) for (int i = 0; i < NI; i++) hs[i] = h[i];
   for (int l = 0; l < 4; l++) {
     const float *W = P + OFF_W[l], *b = P + OFF_B[l];
     for (int o = 0; o < DOUT[l]; o++) {
       double acc = b[o];
       for (int i = 0; i < n_in; i++) {
-        const double w = round ? orc_bf16(W[o * n_in + i]) : W[o * n_in + i];
+        const double w = round ? orc_f16(W[o * n_in + i]) : W[o * n_in + i];
         acc += w * h[i];
       }
       z[o] = acc;
@@ -167,7 +186,7 @@ static void forward(const float *P, const float *x, int round, double *y, double
     }
     for (int o = 0; o < NH; o++) {
       if (zs) zs[l * NH + o] = z[o];
-      h[o] = round ? orc_bf16(orc_act((float)z[o])) : act64(z[o]);
+      h[o] = round ? orc_act_h((float)z[o]) : act64(z[o]);
       if (hs) hs[NI + l * NH + o] = h[o];
     }
     n_in = NH;
@@ -181,9 +200,9 @@ void orc_mlp_forward(const float *P, const float *x, int round, float *y) {
 }
 
 /* Backward pass of one state: dz holds dL/dy on entry; adds the parameter
- * gradient to grad.  round = 1 uses the bfloat16 weights of the forward pass
- * and the float activation derivative (the bfloat16 roundings are passed
- * through as the identity). */
+ * gradient to grad.  round = 1 uses the half-precision weights of the
+ * forward pass and the float activation derivative (the half-precision
+ * roundings are passed through as the identity). */
 static void backward(const float *P, int round, const double *zs, const double *hs, double *dz,
                      double *grad) {
   double dh[NH];
@@ -200,7 +219,7 @@ static void backward(const float *P, int round, const double *zs, const double *
     for (int i = 0; i < ni; i++) {
       double acc = 0.0;
       for (int o = 0; o < no; o++)
-        acc += (round ? (double)orc_bf16(W[o * ni + i]) : (double)W[o * ni + i]) * dz[o];
+        acc += (round ? orc_f16(W[o * ni + i]) : (double)W[o * ni + i]) * dz[o];
       dh[i] = acc;
     }
     for (int i = 0; i < ni; i++) {

@@ -10,7 +10,8 @@ extern "C" {
 #define MLP_NOUT 5
 #define MLP_NPARAM 9221
 
-float orc_bf16(float x);
+double orc_f16(double x);
+double orc_act_h(float z);
 float orc_act(float x);
 float orc_act_d(float x);
 void orc_mlp_features(const double *u, double g33, float *x);

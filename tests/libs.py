@@ -366,8 +366,10 @@ class OracleMLP:
         lib = C.CDLL(ORACLE_SO)
         fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
-        lib.orc_bf16.restype = C.c_float
-        lib.orc_bf16.argtypes = [C.c_float]
+        lib.orc_f16.restype = C.c_double
+        lib.orc_f16.argtypes = [C.c_double]
+        lib.orc_act_h.restype = C.c_double
+        lib.orc_act_h.argtypes = [C.c_float]
         lib.orc_act.restype = C.c_float
         lib.orc_act.argtypes = [C.c_float]
         lib.orc_act_d.restype = C.c_float
@@ -388,8 +390,11 @@ class OracleMLP:
         lib.orc_mlp_refine.argtypes = [dp, fp, C.c_double, C.c_double, C.c_int, C.c_double, dp]
         self.lib = lib
 
-    def bf16(self, x):
-        return self.lib.orc_bf16(x)
+    def f16(self, x):
+        return self.lib.orc_f16(x)
+
+    def act_h(self, z):
+        return self.lib.orc_act_h(z)
 
     def act(self, x):
         return self.lib.orc_act(x)

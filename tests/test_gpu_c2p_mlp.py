@@ -1,9 +1,9 @@
 """GPU checks of the learned con2prim through the C-ABI.
 
-- forward pass (bfloat16 MFMA, FP32 accumulation) against the oracle's
-  bfloat16-emulating forward pass;
+- forward pass (half-precision MFMA, FP32 accumulation) against the
+  oracle's half-precision-emulating forward pass;
 - loss and gradient of the on-device trainer against the oracle's FP64
-  backward pass on the same bfloat16 forward;
+  backward pass on the same half-precision forward;
 - the trainer lowers the loss;
 - the inversion (network guess, FP64 Newton, gate, fallback) against the
   exact primitives and the plain solver, on Minkowski and Bjorken metrics,
@@ -86,7 +86,8 @@ def test_forward_matches_oracle(om, grid):
     yo = np.stack([om.forward(P, om.features(u[:, s]), 1) for s in range(n)], axis=1)
     d = np.abs(y - yo)
     # FP32 MFMA sums vs FP64 sums: |dz| ~ 1e-7 |z|, so a hidden activation
-    # occasionally rounds to the neighbouring bfloat16 value (2^-8 relative)
+    # occasionally rounds to the neighbouring half-precision value (2^-11
+    # relative)
     assert np.median(d) < 1e-5, np.median(d)
     assert np.quantile(d, 0.99) < 1e-3, np.quantile(d, 0.99)
     assert d.max() < 2e-2, d.max()
@@ -104,7 +105,7 @@ def test_gradient_matches_oracle(om, grid):
     L, g = net.grad(grid, du.ptr, dv.ptr, n)
     Lo, go = om.loss(P, u, vt, rnd=1, grad=True)
     assert abs(L - Lo) <= 1e-3 * Lo, (L, Lo)
-    # per parameter block: FP32 vs FP64 accumulation and rare bfloat16
+    # per parameter block: FP32 vs FP64 accumulation and rare half-precision
     # rounding flips of hidden activations
     blocks = [(0, 512), (512, 576), (576, 4672), (4672, 4736), (4736, 8832), (8832, 8896),
               (8896, 9216), (9216, 9221)]

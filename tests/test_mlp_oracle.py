@@ -48,13 +48,47 @@ def _mhd_oracle(oracle, coords=1, t=1.0):
     oracle.set_time(t)
 
 
-def test_bf16_rounding(om):
-    # round to nearest even on the 16 dropped bits
-    assert om.bf16(1.0) == 1.0
-    assert om.bf16(1.0 + 2 ** -8) == 1.0                 # tie -> even
-    assert om.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # tie -> even (up)
-    assert om.bf16(1.0 + 2 ** -8 + 2 ** -20) == 1.0 + 2 ** -7
-    assert om.bf16(-3.0) == -3.0
+def test_f16_rounding(om):
+    """IEEE binary16, nearest with ties to even: known cases and numpy's
+    float16 conversion (an independent implementation) on random values
+    over the normal, subnormal and overflow ranges."""
+    assert om.f16(1.0) == 1.0
+    assert om.f16(1.0 + 2 ** -11) == 1.0                  # tie -> even
+    assert om.f16(1.0 + 3 * 2 ** -11) == 1.0 + 2 ** -9    # tie -> even (up)
+    assert om.f16(1.0 + 2 ** -11 + 2 ** -30) == 1.0 + 2 ** -10
+    assert om.f16(-3.0) == -3.0
+    assert om.f16(2.0 ** -25) == 0.0                      # half the smallest subnormal
+    assert om.f16(3 * 2.0 ** -26) == 2.0 ** -24
+    assert om.f16(65504.0) == 65504.0
+    assert om.f16(65519.9) == 65504.0
+    assert om.f16(65520.0) == np.inf
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-8, 8, 20000),
+                        np.exp(rng.uniform(np.log(1e-9), np.log(7e4), 20000)) * rng.choice([-1, 1], 20000)])
+    x = x.astype(np.float32)
+    mine = np.array([om.f16(float(a)) for a in x])
+    ref = x.astype(np.float16).astype(np.float64)
+    assert np.array_equal(mine, ref)
+
+
+def test_half_precision_activation(om):
+    """orc_act_h (the kernel's activation in half precision) against the
+    same three operations in numpy float16: each product of two float16
+    values and its sum with 1/2 are exact in float64, so rounding them once
+    to float16 is the hardware's fused multiply-add and multiply."""
+    c6 = np.float64(np.float16(np.float32(0.16666667)))
+    rng = np.random.default_rng(4)
+    z = np.concatenate([rng.uniform(-5, 5, 20000), rng.normal(0, 0.05, 5000),
+                        [-3.0, 3.0, -1.5, 0.0, 1e-6, -1e-6, 100.0, -100.0]]).astype(np.float32)
+    x = z.astype(np.float16).astype(np.float64)
+    t = np.clip(np.float16(x * c6 + 0.5).astype(np.float64), 0.0, 1.0)
+    ref = np.float16(x * t).astype(np.float64)
+    mine = np.array([om.act_h(float(a)) for a in z])
+    assert np.array_equal(mine, ref)
+    # and it is the float activation to half-precision accuracy: rounding z,
+    # t and h each to 11 bits moves h by at most ~3 ulps of max(|z|, 1)
+    hf = np.array([om.act(float(a)) for a in z])
+    assert np.all(np.abs(mine - hf) <= 4 * 2.0 ** -11 * np.maximum(np.abs(z), 1.0))
 
 
 def test_activation_shape(om):
@@ -101,7 +135,7 @@ def test_backward_matches_finite_differences(om):
         assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g).max()) + 1e-4 * abs(g[i]), (i, fd, g[i])
 
 
-def test_bf16_gradient_close_to_fp64_gradient(om):
+def test_f16_gradient_close_to_fp64_gradient(om):
     from echoamd.surrogate import init_params
     u, vt, _ = _states(64)
     P = init_params(6)
@@ -229,7 +263,7 @@ def test_generator_ranges():
 
 
 def test_trained_network_gain_cpu(om):
-    """The committed network (run through the oracle's bfloat16 forward pass)
+    """The committed network (run through the oracle's half-precision forward pass)
     against the method without it: the share of states whose two Newton
     iterations pass the 1e-11 gate, from the network's W versus from the
     state at rest (W = 4p, p = (U - B^2/2)/3)."""
