@@ -84,11 +84,15 @@ __host__ __device__ __forceinline__ int tab_bracket(const double* __restrict__ a
 // c = #{j : a[j] < x}, clamped to n-1, whatever path it takes.  c is found
 // here by a short lower-bound search inside one bin of a monotone bin
 // function: the bit pattern of the float nearest to x, shifted right by
-// TAB_SHIFT (256 bins per octave).  Because the bin function is
+// TAB_SHIFT (2048 bins per octave).  Because the bin function is
 // monotone, entries in lower bins are < x and entries in higher bins are
-// > x, so only the entries of x's own bin (at most 13 for qcdIEOS0.dat) are
-// searched.  The result is the same index as tab_bracket.
-constexpr int TAB_SHIFT = 15;
+// > x, so only the entries of x's own bin are searched: at most 2 for
+// qcdIEOS0.dat, whose bin starts take 350 KB per column (any float range
+// needs at most 2 MB).  The chain of dependent table loads is then the bin
+// start and one or two entries (at 256 bins per octave, up to 13 entries
+// and four search steps: 2% slower flux sweeps).  The result is the same
+// index as tab_bracket.
+constexpr int TAB_SHIFT = 12;
 
 __host__ __device__ __forceinline__ int tab_bin(double x, int base, int nb) {
   float f = (float)x;
@@ -173,6 +177,34 @@ __host__ __device__ __forceinline__ int eos_sound(const Eos& E, double rh, doubl
   int l = bracket_e(E, en);
   double s2 = E.tc[l], s1 = E.tc[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
   cs2 = s1 + (s2 - s1) * (en - e1) / (e2 - e1);
+  return 0;
+}
+
+// eos_energy, then eos_sound at the energy it returns (the pair side_state
+// evaluates).  For the table, en lies between te[l-1] and te[l] of the
+// pressure bracket l; strictly inside, l is also the energy bracket (the
+// count of te[j] < en), so that search is skipped; on a node it runs as in
+// eos_sound.  Returns the energy error; the sound error (returned by
+// eos_sound at the same point) goes to err_sound.
+__host__ __device__ __forceinline__ int eos_energy_sound(const Eos& E, double rh, double pr, double& en,
+                                                         double& cs2, int& err_sound) {
+  err_sound = 0;
+  if (E.eqstate != 3) {
+    const int err = eos_energy(E, rh, pr, en);
+    if (err > 0) return err;
+    err_sound = eos_sound(E, rh, en, pr, cs2);
+    return err;
+  }
+  if (pr < E.tp[0]) return 125;
+  if (pr > E.tp[E.n - 1]) return 126;
+  const int l = bracket_p(E, pr);
+  const double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  en = e1 + (e2 - e1) * (pr - p1) / (p2 - p1);
+  if (en < E.te[0]) { err_sound = 131; return 0; }
+  if (en > E.te[E.n - 1]) { err_sound = 132; return 0; }
+  const int m = (en > e1 && en < e2) ? l : bracket_e(E, en);
+  const double s2 = E.tc[m], s1 = E.tc[m - 1], f2 = E.te[m], f1 = E.te[m - 1];
+  cs2 = s1 + (s2 - s1) * (en - f1) / (f2 - f1);
   return 0;
 }
 
@@ -466,8 +498,9 @@ struct SideState {
 template <int DIR, int SYS, bool ZZ>
 __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double* v, SideState& s) {
   constexpr bool VISC = SysT<SYS>::VISC;
-  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
-  int err = eos_energy(E, rh, pr, en);
+  double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en, cs2;
+  int err_sound;
+  int err = eos_energy_sound(E, rh, pr, en, cs2, err_sound);
   if (err > 0) return err;
   double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
   if (v2 > 1) return 333;
@@ -489,7 +522,8 @@ __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double*
   }
   s.vdir = DIR == 1 ? vx : (DIR == 2 ? vy : vz);
   s.wwdir = s.ww * s.vdir;
-  return eos_sound(E, rh, en, pr, s.cs2);
+  s.cs2 = cs2;
+  return err_sound;
 }
 
 // fast speeds lambda+ (lp) and lambda- (lm), system.f90:1557-1580

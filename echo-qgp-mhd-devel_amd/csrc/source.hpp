@@ -64,20 +64,38 @@ __device__ __forceinline__ void get_sigma(int coordinates, const Metric& M, cons
   }
 }
 
-// get_viscous_parameters, viscous.f90:172-199
-__device__ __forceinline__ int get_viscous_parameters(const Eos& E, const ViscParams& P, double rh,
-                                                      double pr, double& eta, double& zeta,
-                                                      double& tau_pi) {
-  const double HBAR = 0.197326;   // common.f90:47
-  double en, temp, s, cs2;
-  get_derived_data(E, rh, pr, en, temp, s);
-  int err = eos_sound(E, rh, en, pr, cs2);
-  eta = HBAR * P.eta_over_s * s;
-  if (P.init_type == SHEAR_VISCOUS_1D) eta = P.eta_over_s;
-  zeta = P.bulkvis ? 2. * eta * (1. / 3. - cs2) : 0.;
-  tau_pi = (temp == 0 || s == 0) ? 0. : P.tau_pi_coeff * eta / (temp * s);
-  if (P.init_type == GUBSER_VISCOUS) tau_pi = 5. * eta / (temp * s);
-  return err;
+// The EoS values of the viscous source at (rh, pr), with one pressure search
+// and, off the table's nodes, no energy search: en as eos_energy returns it;
+// temp, s as get_derived_data returns them, and cs2 from eos_sound at that
+// function's energy (the steps of get_viscous_parameters).  Returns the
+// energy error; the sound error goes to err_sound.
+__device__ __forceinline__ int visc_eos(const Eos& E, double rh, double pr, double& en, double& temp,
+                                        double& s, double& cs2, int& err_sound) {
+  err_sound = 0;
+  if (E.eqstate != 3) {
+    const int err = eos_energy(E, rh, pr, en);
+    if (err > 0) return err;
+    double en2;
+    get_derived_data(E, rh, pr, en2, temp, s);
+    err_sound = eos_sound(E, rh, en2, pr, cs2);
+    return err;
+  }
+  if (pr < E.tp[0]) return 125;
+  if (pr > E.tp[E.n - 1]) return 126;
+  const int l = bracket_p(E, pr);
+  const double p2 = E.tp[l], p1 = E.tp[l - 1], e2 = E.te[l], e1 = E.te[l - 1];
+  const double t2 = E.tt[l], t1 = E.tt[l - 1];
+  en = e1 + (e2 - e1) * (pr - p1) / (p2 - p1);
+  const double f = (pr - p1) / (p2 - p1);
+  temp = t1 + (t2 - t1) * f;
+  const double en2 = e1 + (e2 - e1) * f;
+  s = (en2 + pr) / temp;
+  if (en2 < E.te[0]) { err_sound = 131; return 0; }
+  if (en2 > E.te[E.n - 1]) { err_sound = 132; return 0; }
+  const int m = (en2 > e1 && en2 < e2) ? l : bracket_e(E, en2);
+  const double c2 = E.tc[m], c1 = E.tc[m - 1], g2 = E.te[m], g1 = E.te[m - 1];
+  cs2 = c1 + (c2 - c1) * (en2 - g1) / (g2 - g1);
+  return 0;
 }
 
 // system_source, system.f90:1587-1842 for HD (VISC=false) and viscous HD.
@@ -91,7 +109,9 @@ __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coo
 #pragma unroll
   for (int k = 0; k < NV; k++) src[k] = 0.;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
-  int err = eos_energy(E, rh, pr, en);
+  double vtemp = 0., vs = 0., vcs2 = 0.;
+  int err_sound = 0;
+  int err = VISC ? visc_eos(E, rh, pr, en, vtemp, vs, vcs2, err_sound) : eos_energy(E, rh, pr, en);
   if (err > 0) return err;
   const double t = M.t;
   double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
@@ -117,7 +137,16 @@ __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coo
     const double lambda0 = 1., lambda1 = 1., lambda2 = 0.;
     double sigma[6], eta, zeta, tau_pi;
     get_sigma(coordinates, M, v, D, sigma);
-    err = get_viscous_parameters(E, P, v[KRH], v[KPR], eta, zeta, tau_pi);
+    {
+      // get_viscous_parameters, viscous.f90:172-199, on the values of visc_eos
+      const double HBAR = 0.197326;   // common.f90:47
+      eta = HBAR * P.eta_over_s * vs;
+      if (P.init_type == SHEAR_VISCOUS_1D) eta = P.eta_over_s;
+      zeta = P.bulkvis ? 2. * eta * (1. / 3. - vcs2) : 0.;
+      tau_pi = (vtemp == 0 || vs == 0) ? 0. : P.tau_pi_coeff * eta / (vtemp * vs);
+      if (P.init_type == GUBSER_VISCOUS) tau_pi = 5. * eta / (vtemp * vs);
+      err = err_sound;
+    }
     if (err > 0) return err;
     const double tau_pi_big = tau_pi;
     PiDer q = derived_pi<ZZ>(v, M);
