@@ -30,4 +30,4 @@ wait $P3
 "$HIPCC" --offload-arch="$ARCH" -shared -fPIC -o "$HERE/libechoamd.so" "$OBJ/echo_capi.o" "$OBJ/c2p_mlp.o" \
   "$OBJ/freezeout.o"
 echo "build: $HERE/libechoamd.so"
-"$HIPCC" --offload-arch="$ARCH" -O3 -o "$HERE/copy_calib" "$HERE/csrc/copy_calib.hip"
+"$HIPCC" "${FLAGS[@]}" -o "$HERE/copy_calib" "$HERE/csrc/copy_calib.hip"
