@@ -491,8 +491,9 @@ int echo_init_from_prim(echo_grid* g, const double* v) {
   return run_init(g);
 }
 
-int echo_init_glauber(echo_grid* g, const echo_glauber_params* p, const double* part_a, int32_t n_a,
-                      const double* part_b, int32_t n_b, const double* coll, int32_t n_coll) {
+// primitives of the Glauber-MC initialisation in V (no prim2cons yet)
+static int glauber_primitives(echo_grid* g, const echo_glauber_params* p, const double* part_a, int32_t n_a,
+                              const double* part_b, int32_t n_b, const double* coll, int32_t n_coll) {
   if (!g || !p) return fail(ECHO_E_ARG, "null argument");
   if (n_a < 0 || n_b < 0 || n_coll < 0 || (n_a && !part_a) || (n_b && !part_b) || (n_coll && !coll))
     return fail(ECHO_E_ARG, "invalid source arrays");
@@ -546,7 +547,67 @@ int echo_init_glauber(echo_grid* g, const echo_glauber_params* p, const double* 
   hipFree(dsrc);
   hipFree(dsum);
   if (e != hipSuccess) return fail(ECHO_E_HIP, "glauber initial state: %s", hipGetErrorString(e));
+  return st;
+}
+
+int echo_init_glauber(echo_grid* g, const echo_glauber_params* p, const double* part_a, int32_t n_a,
+                      const double* part_b, int32_t n_b, const double* coll, int32_t n_coll) {
+  const int st = glauber_primitives(g, p, part_a, n_a, part_b, n_b, coll, n_coll);
   if (st) return st;
+  return run_init(g);
+}
+
+int echo_init_glauber_mhd(echo_grid* g, const echo_glauber_params* p, const double* part_a, int32_t n_a,
+                          const double* part_b, int32_t n_b, const double* coll, int32_t n_coll,
+                          const echo_bfield_params* bp, const echo_nucleons* lists) {
+  if (!g || !p || !bp || !lists) return fail(ECHO_E_ARG, "null argument");
+  if (!g->cfg.mhd) return fail(ECHO_E_ARG, "the grid has no magnetic field (mhd = 0)");
+  if (!(bp->gamma_col > 1.)) return fail(ECHO_E_ARG, "need gamma_col > 1");
+  if (bp->magfield_type < 1 || bp->magfield_type > 3) return fail(ECHO_E_ARG, "magfield_type must be 1, 2 or 3");
+  // each list up to its first neutron, as glaubermc.f90:1026-1058 reads it
+  std::vector<double> xy;
+  int cnt[4];
+  for (int l = 0; l < 4; l++) {
+    const echo_nucleons& L = lists[l];
+    if (L.n < 0 || (L.n && (!L.xy || !L.type))) return fail(ECHO_E_ARG, "invalid nucleon list %d", l);
+    int k = 0;
+    while (k < L.n && L.type[k] == 0) k++;
+    cnt[l] = k;
+    xy.insert(xy.end(), L.xy, L.xy + 2 * k);
+  }
+  const int ns = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+  if (ns > BFIELD_MAX) return fail(ECHO_E_ARG, "at most %d protons, got %d", BFIELD_MAX, ns);
+  int st = glauber_primitives(g, p, part_a, n_a, part_b, n_b, coll, n_coll);
+  if (st) return st;
+  const echo_config& c = g->cfg;
+  BFieldArgs b;
+  b.gam = bp->gamma_col;
+  b.vel = sqrt(1. - 1. / (bp->gamma_col * bp->gamma_col));
+  b.sel = c.sigma_el;
+  b.sch = c.sigma_chi;
+  b.tmin = c.tmin;
+  b.w[0] = bp->b_amplify; b.w[1] = bp->b_amplify * bp->pw;
+  b.w[2] = bp->b_amplify; b.w[3] = bp->b_amplify * bp->pw;
+  for (int l = 0; l < 4; l++) b.n[l] = cnt[l];
+  b.magfield_type = bp->magfield_type;
+  double* dxy = nullptr;
+  if (ns > 0 && hipMalloc(&dxy, sizeof(double) * xy.size()) != hipSuccess)
+    return fail(ECHO_E_NOMEM, "proton buffer");
+  hipError_t e = hipSuccess;
+  if (ns > 0) e = hipMemcpyAsync(dxy, xy.data(), sizeof(double) * xy.size(), hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) {
+    const double dx = (c.x2 - c.x1) / c.nx, dy = (c.y2 - c.y1) / c.ny, dz = (c.z2 - c.z1) / c.nz;
+    dispatch_sys(c, [&](auto sys, auto) {
+      constexpr int SYS = decltype(sys)::value;
+      if constexpr (SysT<SYS>::MHD)
+        k_glauber_bfield<SYS><<<nblocks(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+            g->ga, b, c.x1, dx, c.y1, dy, c.z1, dz, dxy, g->V);
+    });
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (dxy) hipFree(dxy);
+  if (e != hipSuccess) return fail(ECHO_E_HIP, "initial magnetic field: %s", hipGetErrorString(e));
   return run_init(g);
 }
 

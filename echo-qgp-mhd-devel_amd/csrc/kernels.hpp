@@ -851,6 +851,98 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_glauber_prim(GridArgs a, Glaub
   if (err) atomicMax(status, err);
 }
 
+// ---------------------------------------------------------------------------
+// Initial magnetic field of generate_B_field_profile (glaubermc.f90:
+// 1009-1168): per cell, the field of the protons among the spectators and
+// participants of nuclei A and B (participants weighted by pw), added to
+// B^x, B^y, B^eta charge by charge in the reference's order (spectators A,
+// participants A, spectators B, participants B).  The charges are staged in
+// LDS; each list has already been cut at its first neutron.
+// ---------------------------------------------------------------------------
+struct BFieldArgs {
+  double gam, vel, sel, sch, tmin;
+  double w[4];   // prefactor per list: B_amplify_factor (times pw for participants)
+  int n[4];      // protons per list
+  int magfield_type;
+};
+
+constexpr int BFIELD_MAX = 2048;   // LDS: 32 KiB of (x, y) pairs
+
+// one proton: the classical azimuthal field Bphi and the chiral fields Br,
+// Bz of glaubermc.f90:1028-1056 (nucleus A, a = true) and 1099-1127 (B)
+__device__ __forceinline__ void proton_field(bool a, double w, double xc, double yc, double zpos,
+                                             const BFieldArgs& b, double tom, double tomz, double& bx,
+                                             double& by, double& bz) {
+  const double hbar = 0.197326, sqrh = 0.444214, qbig = 0.30282212;   // common.f90:51-55
+  const double pi = 3.141592653589793;
+  const double gam = b.gam, vel = b.vel, sel = b.sel, sch = b.sch;
+  const double phi = atan2(yc, xc);
+  const double xt2 = xc * xc + yc * yc;
+  const double xt = sqrt(xt2);
+  const double D = (gam * zpos) * (gam * zpos) + xt2;
+  const double sqrD = sqrt(D);
+  const double A = (sel * vel * gam / 2) * (gam * zpos - sqrD) / hbar;
+  const double D15 = pow(D, 1.5);
+  const double eA = exp(A);
+  double bphi = 0., br = 0., bzz = 0.;
+  if (b.magfield_type != 3)
+    bphi = w * qbig / (4 * pi) * sqrh * (vel * gam * xt / D15) * (1 + sel * vel * gam * sqrD / (2 * hbar)) * eA;
+  if (b.magfield_type != 2) {
+    br = w * (-sch) * qbig / (8 * pi) * (vel * (gam * gam) * xt / D15) * (gam * zpos + A * sqrD) * eA / sqrh;
+    bzz = w * sch * qbig / (8 * pi) * (vel * gam / D15) *
+          ((gam * zpos) * (gam * zpos) * (1 + sel * vel * gam * sqrD / (2 * hbar)) +
+           D * (1 - sel * vel * gam * sqrD / (2 * hbar))) *
+          eA / sqrh;
+  }
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  if (a) {
+    bx = bx + (bphi * (-sp) + br * cp) / tom;
+    by = by + (bphi * cp + br * sp) / tom;
+    bz = bz + bzz / tomz;
+  } else {
+    bx = bx + (bphi * sp + br * cp) / tom;
+    by = by + (bphi * (-cp) + br * sp) / tom;
+    bz = bz + (-bzz) / tomz;
+  }
+}
+
+template <int SYS>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_glauber_bfield(GridArgs a, BFieldArgs b, double x1, double dx,
+                                                                 double y1, double dy, double z1, double dz,
+                                                                 const double* __restrict__ xy,
+                                                                 double* __restrict__ V) {
+  __shared__ double2 s[BFIELD_MAX];
+  const int ns = b.n[0] + b.n[1] + b.n[2] + b.n[3];
+  for (int i = threadIdx.x; i < ns; i += UPDATE_BLOCK) s[i] = make_double2(xy[2 * i], xy[2 * i + 1]);
+  __syncthreads();
+  const long long c = (long long)blockIdx.x * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N) return;
+  const long long N = a.N;
+  const int ix = (int)(c % a.nx);
+  const long long r = c / a.nx;
+  const int iy = (int)(r % a.ny), iz = (int)(r / a.ny);
+  const double x = x1 + ((double)(ix + 1) - 0.5) * dx;
+  const double y = y1 + ((double)(iy + 1) - 0.5) * dy;
+  const double z = z1 + ((double)(iz + 1) - 0.5) * dz;
+  double tt, zcoord, tom, tomz;
+  if (a.coordinates == MINKOWSKI) {
+    zcoord = z; tt = b.tmin; tom = 1.; tomz = 1.;
+  } else {
+    tt = b.tmin * cosh(z); zcoord = b.tmin * sinh(z); tom = cosh(z); tomz = b.tmin;
+  }
+  double bx = V[KBX * N + c], by = V[KBY * N + c], bz = V[KBZ * N + c];
+  int i = 0;
+  for (int l = 0; l < 4; l++) {
+    const double zpos = l < 2 ? tt * b.vel + zcoord : tt * b.vel - zcoord;
+    const int end = i + b.n[l];
+    for (; i < end; i++) proton_field(l < 2, b.w[l], x - s[i].x, y - s[i].y, zpos, b, tom, tomz, bx, by, bz);
+  }
+  V[KBX * N + c] = bx;
+  V[KBY * N + c] = by;
+  V[KBZ * N + c] = bz;
+}
+
 // batched con2prim over n independent states (system_cons2prim)
 template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_c2p_batch(GridArgs a, const double* __restrict__ Ub,

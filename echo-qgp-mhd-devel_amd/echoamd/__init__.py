@@ -35,7 +35,7 @@ EXPORTS = (
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
     "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
     "echo_fo_setup", "echo_fo_slice", "echo_fo_width", "echo_fo_elements", "echo_fo_write",
-    "echo_set_dump", "echo_dump", "echo_restart",
+    "echo_set_dump", "echo_dump", "echo_restart", "echo_init_glauber_mhd",
 )
 
 # out_vars field flags (type out_type of the reference), in file order
@@ -77,6 +77,17 @@ class GlauberParams(C.Structure):
     """Mirror of struct echo_glauber_params."""
     _fields_ = [(n, C.c_double) for n in ("kappa", "ah", "sig_mc", "ybeam", "deta", "sigeta",
                                           "rhocenter")]
+
+
+class BFieldParams(C.Structure):
+    """Mirror of struct echo_bfield_params."""
+    _fields_ = [("gamma_col", C.c_double), ("b_amplify", C.c_double), ("pw", C.c_double),
+                ("magfield_type", C.c_int32)]
+
+
+class Nucleons(C.Structure):
+    """Mirror of struct echo_nucleons."""
+    _fields_ = [("xy", C.POINTER(C.c_double)), ("type", C.POINTER(C.c_int32)), ("n", C.c_int32)]
 
 
 class TrainConfig(C.Structure):
@@ -139,6 +150,9 @@ def load_library(path=LIB_PATH):
         "echo_ensemble_last_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
         "echo_init_glauber": (C.c_int, [P, C.POINTER(GlauberParams), _dp, C.c_int32, _dp,
                                         C.c_int32, _dp, C.c_int32]),
+        "echo_init_glauber_mhd": (C.c_int, [P, C.POINTER(GlauberParams), _dp, C.c_int32, _dp,
+                                            C.c_int32, _dp, C.c_int32, C.POINTER(BFieldParams),
+                                            C.POINTER(Nucleons)]),
         "echo_set_stop": (C.c_int, [P, C.c_double, C.c_int32]),
         "echo_write_frame": (C.c_int, [P, _ip, C.c_int32, C.c_int32, C.c_char_p]),
         "echo_wait_frames": (C.c_int, [P]),
@@ -369,6 +383,31 @@ class EchoGrid:
         return self._check(self.lib.echo_init_glauber(self.h, C.byref(gp), flat[0], arrs[0].shape[0],
                                                       flat[1], arrs[1].shape[0], flat[2],
                                                       arrs[2].shape[0]))
+
+    def init_glauber_mhd(self, params, part_a, part_b, coll, bparams, nucleons):
+        """Glauber-MC initial state of an ideal-MHD grid with the initial
+        magnetic field of the protons.  bparams: dict of BFieldParams fields;
+        nucleons: four (xy (n, 2), type (n,)) pairs for the spectators of A,
+        the participants of A, the spectators of B and the participants of
+        B (type 0 proton, 1 neutron)."""
+        gp = GlauberParams(**params)
+        bp = BFieldParams(**bparams)
+        arrs = [np.ascontiguousarray(np.reshape(a, (-1, 2)), dtype=np.float64) for a in
+                (part_a, part_b, coll)]
+        flat = [a.ravel() if a.size else np.zeros(2) for a in arrs]
+        keep = []
+        lists = (Nucleons * 4)()
+        for l, (xy, ty) in enumerate(nucleons):
+            xy = np.ascontiguousarray(np.reshape(xy, (-1, 2)), dtype=np.float64)
+            ty = np.ascontiguousarray(ty, dtype=np.int32)
+            assert ty.shape[0] == xy.shape[0]
+            keep += [xy, ty]
+            lists[l].xy = xy.ctypes.data_as(C.POINTER(C.c_double))
+            lists[l].type = ty.ctypes.data_as(C.POINTER(C.c_int32))
+            lists[l].n = xy.shape[0]
+        return self._check(self.lib.echo_init_glauber_mhd(
+            self.h, C.byref(gp), flat[0], arrs[0].shape[0], flat[1], arrs[1].shape[0], flat[2],
+            arrs[2].shape[0], C.byref(bp), lists))
 
     def upload(self, v, u):
         v = np.ascontiguousarray(v, dtype=np.float64)

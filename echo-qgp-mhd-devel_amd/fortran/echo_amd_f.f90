@@ -15,7 +15,9 @@ module echo_amd_f
             echo_get_scalars, echo_set_tout, echo_cons2prim_batch, echo_set_timing, &
             echo_get_timing, echo_last_error, echo_error_message, echo_set_stop, echo_get_stop, &
             echo_write_frame, echo_wait_frames, echo_fo_setup, echo_fo_slice, echo_fo_width, &
-            echo_fo_elements, echo_fo_write, echo_set_dump, echo_dump, echo_restart
+            echo_fo_elements, echo_fo_write, echo_set_dump, echo_dump, echo_restart, &
+            echo_glauber_params, echo_bfield_params, echo_nucleons, echo_init_glauber, &
+            echo_init_glauber_mhd
 
   ! struct echo_config: 16 int32 then 19 double (param.dat keys in brackets)
   type, bind(C) :: echo_config
@@ -48,6 +50,25 @@ module echo_amd_f
     real(c_double) :: smooth_temp               ! CUT_TEMP
     real(c_double) :: tmax                      ! TSTOP
   end type echo_config
+
+  ! struct echo_glauber_params: parameters of generate_energy_density_profile
+  type, bind(C) :: echo_glauber_params
+    real(c_double) :: kappa, ah, sig_mc, ybeam, deta, sigeta, rhocenter
+  end type echo_glauber_params
+
+  ! struct echo_bfield_params: generate_B_field_profile (gamma_col, B_amp_fac, PW, MAGFIELDT)
+  type, bind(C) :: echo_bfield_params
+    real(c_double) :: gamma_col, b_amplify, pw
+    integer(c_int32_t) :: magfield_type
+  end type echo_bfield_params
+
+  ! struct echo_nucleons: (x, y) pairs and types (0 proton, 1 neutron) of one list,
+  ! e.g. c_loc of xspecA/yspecA packed in pairs and of spec_typeA
+  type, bind(C) :: echo_nucleons
+    type(c_ptr) :: xy
+    type(c_ptr) :: ptype
+    integer(c_int32_t) :: n
+  end type echo_nucleons
 
   interface
     integer(c_int) function echo_create(cfg, eos_rows, n_rows, device, grid) bind(C, name="echo_create")
@@ -198,6 +219,24 @@ module echo_amd_f
       type(c_ptr), value :: grid
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: header
+    end function
+    integer(c_int) function echo_init_glauber(grid, p, part_a, n_a, part_b, n_b, coll, n_coll) &
+        bind(C, name="echo_init_glauber")
+      import :: c_int, c_int32_t, c_double, c_ptr, echo_glauber_params
+      type(c_ptr), value :: grid
+      type(echo_glauber_params), intent(in) :: p
+      real(c_double), intent(in) :: part_a(*), part_b(*), coll(*)
+      integer(c_int32_t), value :: n_a, n_b, n_coll
+    end function
+    integer(c_int) function echo_init_glauber_mhd(grid, p, part_a, n_a, part_b, n_b, coll, n_coll, &
+        bp, lists) bind(C, name="echo_init_glauber_mhd")
+      import :: c_int, c_int32_t, c_double, c_ptr, echo_glauber_params, echo_bfield_params, echo_nucleons
+      type(c_ptr), value :: grid
+      type(echo_glauber_params), intent(in) :: p
+      real(c_double), intent(in) :: part_a(*), part_b(*), coll(*)
+      integer(c_int32_t), value :: n_a, n_b, n_coll
+      type(echo_bfield_params), intent(in) :: bp
+      type(echo_nucleons), intent(in) :: lists(4)
     end function
   end interface
 

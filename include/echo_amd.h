@@ -114,6 +114,29 @@ typedef struct echo_glauber_params {
 int echo_init_glauber(echo_grid *g, const echo_glauber_params *p, const double *part_a, int32_t n_a,
                       const double *part_b, int32_t n_b, const double *coll, int32_t n_coll);
 
+/* The same for an ideal-MHD grid, with the initial magnetic field of
+ * generate_B_field_profile (glaubermc.f90:877-1174, called from init.f90:
+ * 2796-2811 when no B-field file is given; one event, avg_events = 0): the field of the protons among
+ * the spectators and participants of nuclei A and B (participants weighted
+ * by pw) in a medium of the grid's sigma_el and sigma_chi, added to B^x,
+ * B^y, B^eta before prim2cons.  lists[4]: spectators A, participants A,
+ * spectators B, participants B; each is read up to its first neutron, as
+ * the reference does (its files list protons first); at most 2048 protons. */
+typedef struct echo_bfield_params {
+  double gamma_col;       /* beam Lorentz factor, rads/(2 am) (common.f90:2261) */
+  double b_amplify;       /* [B_amp_fac] B_amplify_factor */
+  double pw;              /* [PW.......] weight of the participants */
+  int32_t magfield_type;  /* [MAGFIELDT] 1 classical + chiral, 2 classical, 3 chiral */
+} echo_bfield_params;
+typedef struct echo_nucleons {
+  const double *xy;       /* (x, y) pairs in fm, host memory */
+  const int32_t *type;    /* 0 proton, 1 neutron */
+  int32_t n;
+} echo_nucleons;
+int echo_init_glauber_mhd(echo_grid *g, const echo_glauber_params *p, const double *part_a, int32_t n_a,
+                          const double *part_b, int32_t n_b, const double *coll, int32_t n_coll,
+                          const echo_bfield_params *bp, const echo_nucleons *lists);
+
 /* Upload / download primitive and conserved state (host memory, Fortran
  * layout).  Replaces the reference's direct access to module arrays v, u
  * (common.f90:203-214). */

@@ -63,6 +63,9 @@ void orc_get_stop(int *stopped, double *max_value);
 /* Glauber-MC initial primitives (oracle_init.c) */
 int orc_glauber_prim(const double *par, const double *pa, int na, const double *pb, int nb,
                      const double *pc, int nc, double *vout);
+int orc_glauber_bfield(const double *par, const double *sa, const int *ta, int nsa, const double *pa,
+                       const int *tpa, int npa, const double *sb, const int *tb, int nsb,
+                       const double *pb, const int *tpb, int npb, double *v);
 
 /* freeze-out hypersurface (oracle_fo.c) */
 int  orc_fo_setup(int freeze_type, double freeze_value, double *hdr);

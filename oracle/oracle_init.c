@@ -13,8 +13,11 @@
  *     2662-2730): rho = 1, v = 0, p = eos_pressure(rhocenter, e + enezero);
  *   - viscous runs: the Navier-Stokes initial stresses of viscous_initio
  *     (viscous.f90:101-136) with get_viscous_parameters_0 (140-167).
- * The participant and collision positions are inputs (the reference draws
- * them in generate_events with its own random-number stream).
+ *   - the initial magnetic field of generate_B_field_profile
+ *     (glaubermc.f90:877-1174, nucleus-nucleus branch): orc_glauber_bfield.
+ * The participant, collision and spectator positions are inputs (the
+ * reference draws them in generate_events with its own random-number
+ * stream).
  */
 #include <math.h>
 #include <string.h>
@@ -92,5 +95,95 @@ int orc_glauber_prim(const double *par, const double *pa, int na, const double *
           vout[KPIBU * n + c] = -zeta / tau0;
         }
       }
+  return 0;
+}
+
+/* The field of one proton of nucleus A (a = 1) or B (a = 0) at transverse
+ * offset (xc, yc) from the cell and longitudinal distance zpos, scaled by
+ * the prefactor w (B_amplify_factor, times pw for participants), added to
+ * (bx, by, bz) as glaubermc.f90:1028-1056 (A) and 1099-1127 (B) do: the
+ * classical azimuthal field Bphi and the chiral radial and longitudinal
+ * fields Br, Bz of a charge moving through a medium of electric
+ * conductivity sigma_el and chiral conductivity sigma_chi.  pow(D, 1.5) is
+ * D**1.5 (a real exponent). */
+static void proton_field(int a, double w, double xc, double yc, double zpos, double gam, double vel,
+                         int magfield_type, double tom, double tomz, double *bx, double *by,
+                         double *bz) {
+  const double hbar = 0.197326, sqrh = 0.444214, qbig = 0.30282212;   /* common.f90:51-55 */
+  const double pi = acos(-1.);
+  const double sel = G.sigma_el, sch = G.sigma_chi;
+  const double phi = atan2(yc, xc);
+  const double xt2 = xc * xc + yc * yc;
+  const double xt = sqrt(xt2);
+  const double D = (gam * zpos) * (gam * zpos) + xt2;
+  const double sqrD = sqrt(D);
+  const double A = (sel * vel * gam / 2) * (gam * zpos - sqrD) / hbar;
+  const double D15 = pow(D, 1.5);
+  double bphi = 0., br = 0., bzz = 0.;
+  if (magfield_type != 3)
+    bphi = w * qbig / (4 * pi) * sqrh * (vel * gam * xt / D15) * (1 + sel * vel * gam * sqrD / (2 * hbar)) *
+           exp(A);
+  if (magfield_type != 2) {
+    br = w * (-sch) * qbig / (8 * pi) * (vel * (gam * gam) * xt / D15) * (gam * zpos + A * sqrD) * exp(A) /
+         sqrh;
+    bzz = w * sch * qbig / (8 * pi) * (vel * gam / D15) *
+          ((gam * zpos) * (gam * zpos) * (1 + sel * vel * gam * sqrD / (2 * hbar)) +
+           D * (1 - sel * vel * gam * sqrD / (2 * hbar))) *
+          exp(A) / sqrh;
+  }
+  if (a) {
+    *bx = *bx + (bphi * (-sin(phi)) + br * cos(phi)) / tom;
+    *by = *by + (bphi * cos(phi) + br * sin(phi)) / tom;
+    *bz = *bz + bzz / tomz;
+  } else {
+    *bx = *bx + (bphi * sin(phi) + br * cos(phi)) / tom;
+    *by = *by + (bphi * (-cos(phi)) + br * sin(phi)) / tom;
+    *bz = *bz + (-bzz) / tomz;
+  }
+}
+
+/* generate_B_field_profile, glaubermc.f90:1009-1168: adds the field of the
+ * protons of the spectators and participants of nuclei A and B to B^x, B^y,
+ * B^eta of the primitives v (nv x cells, variable-major), cell by cell and
+ * charge by charge in the reference's order (spectators A, participants A,
+ * spectators B, participants B).  Each list is read up to its first
+ * neutron (type != 0), as the reference does.  par: gamma_col,
+ * B_amplify_factor, pw, magfield_type (1 classical + chiral, 2 classical,
+ * 3 chiral).  Returns 1 when the grid has no magnetic field. */
+int orc_glauber_bfield(const double *par, const double *sa, const int *ta, int nsa, const double *pa,
+                       const int *tpa, int npa, const double *sb, const int *tb, int nsb,
+                       const double *pb, const int *tpb, int npb, double *v) {
+  if (!G.mhd) return 1;
+  const double gam = par[0], amp = par[1], pw = par[2];
+  const int magfield_type = (int)par[3];
+  const double vel = sqrt(1. - 1. / (gam * gam));
+  const size_t n = (size_t)G.nx * G.ny * G.nz;
+  const double *xy[4] = {sa, pa, sb, pb};
+  const int *ty[4] = {ta, tpa, tb, tpb};
+  const int cnt[4] = {nsa, npa, nsb, npb};
+  const double wt[4] = {amp, amp * pw, amp, amp * pw};
+  for (int iz = 0; iz < G.nz; iz++) {
+    double tt, zcoord, tom, tomz;
+    if (G.coordinates == 1) {
+      zcoord = G.zc[iz]; tt = G.tmin; tom = 1.; tomz = 1.;
+    } else {
+      tt = G.tmin * cosh(G.zc[iz]); zcoord = G.tmin * sinh(G.zc[iz]);
+      tom = cosh(G.zc[iz]); tomz = G.tmin;
+    }
+    for (int iy = 0; iy < G.ny; iy++)
+      for (int ix = 0; ix < G.nx; ix++) {
+        const size_t c = ((size_t)iz * G.ny + iy) * G.nx + ix;
+        double bx = v[G.kbx * n + c], by = v[G.kby * n + c], bz = v[G.kbz * n + c];
+        for (int l = 0; l < 4; l++) {
+          const double zpos = l < 2 ? tt * vel + zcoord : tt * vel - zcoord;
+          for (int i = 0; i < cnt[l] && ty[l][i] == 0; i++)
+            proton_field(l < 2, wt[l], G.xc[ix] - xy[l][2 * i], G.yc[iy] - xy[l][2 * i + 1], zpos, gam,
+                         vel, magfield_type, tom, tomz, &bx, &by, &bz);
+        }
+        v[G.kbx * n + c] = bx;
+        v[G.kby * n + c] = by;
+        v[G.kbz * n + c] = bz;
+      }
+  }
   return 0;
 }

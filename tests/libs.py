@@ -210,6 +210,29 @@ def oracle_glauber_prim(oracle, params, part_a, part_b, coll):
     return e, v.reshape(oracle.nv, oracle.nz, oracle.ny, oracle.nx)
 
 
+def oracle_glauber_bfield(oracle, bparams, nucleons, v):
+    """Adds the oracle's initial magnetic field (oracle_init.c,
+    orc_glauber_bfield) to the primitives v (nv, nz, ny, nx), in place.
+    nucleons: four (xy (n, 2), type (n,)) pairs: spectators A, participants
+    A, spectators B, participants B."""
+    f = oracle.lib.orc_glauber_bfield
+    f.restype = C.c_int
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    f.argtypes = [_dp] + [_dp, ip, C.c_int] * 4 + [_dp]
+    par = np.array([bparams["gamma_col"], bparams["b_amplify"], bparams["pw"],
+                    float(bparams["magfield_type"])], dtype=np.float64)
+    args = []
+    for xy, ty in nucleons:
+        xy = np.ascontiguousarray(np.reshape(xy, (-1, 2)), dtype=np.float64)
+        ty = np.ascontiguousarray(ty, dtype=np.int32)
+        args += [xy.ravel() if xy.size else np.zeros(2), ty if ty.size else np.zeros(1, np.int32),
+                 xy.shape[0]]
+    flat = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    e = f(par, *args, flat)
+    v[...] = flat.reshape(v.shape)
+    return e
+
+
 class OracleFO:
     """The oracle's freeze-out hypersurface finder (oracle_fo.c)."""
 
