@@ -240,7 +240,6 @@ def ensemble_leg(n_events, steps, eos_rows, seed=2026):
     ens = Ensemble(cfgs, eos_rows=eos_rows)
     for g, v in zip(ens.grids, states):
         g.init_from_prim(v)
-    del states
     ens.steps(1)
     ens.tune(reps=2)
     ens.steps(1)
@@ -249,8 +248,25 @@ def ensemble_leg(n_events, steps, eos_rows, seed=2026):
     cells = sum(int(ic[0]) * int(ic[1]) * int(ic[2]) for ic, _ in cfgs)
     tiles = [g.tile().tolist() for g in ens.grids]
     ens.close()
+    # the same events one at a time (one grid and one graph replay each)
+    from echoamd import EchoGrid
+    grids = [EchoGrid(ic, rc, eos_rows=eos_rows) for ic, rc in cfgs]
+    for g, v in zip(grids, states):
+        g.init_from_prim(v)
+        g.steps(1)
+        g.tune(reps=2)
+        g.steps(1)
+    del states
+    t0 = time.perf_counter()
+    for g in grids:
+        g.steps(steps)
+    single_ms = (time.perf_counter() - t0) * 1e3
+    for g in grids:
+        g.close()
     return {"events": n_events, "cells": cells, "steps": steps, "ms": ms,
             "zone_updates_per_s": cells * steps / (ms * 1e-3), "crashed": crashed,
+            "one_at_a_time_zone_updates_per_s": cells * steps / (single_ms * 1e-3),
+            "speedup_vs_one_at_a_time": single_ms / ms,
             "grid_sizes": sorted({f"{int(ic[0])}x{int(ic[1])}x{int(ic[2])}" for ic, _ in cfgs}),
             "tiles_chosen": {str(t): tiles.count(t) for t in map(list, {tuple(t) for t in tiles})}}
 
