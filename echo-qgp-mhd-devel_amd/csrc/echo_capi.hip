@@ -243,6 +243,9 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
     return fail(ECHO_E_ARG, "MHD needs ALG_SOLV 1 or 2 with EQ_STATE 1 or 3, 4 with EQ_STATE 2, "
                             "or 5 (6 in Minkowski) with EQ_STATE 1 (got %d, %d)", c.algo_solver, c.eqstate);
   if (c.coordinates != 1 && c.coordinates != 2) return fail(ECHO_E_ARG, "COORD must be 1 or 2");
+  if (c.derivatives_out)
+    return fail(ECHO_E_ARG, "derivativ = 1: the derivative output of out_derivatives (out.f90:860-1250) "
+                            "is not built");
   if (c.nrk != 2 && c.nrk != 3) return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2) or RK3 (3)");
   if (c.boundary_cond != 0 && c.boundary_cond != 2)
     return fail(ECHO_E_ARG, "boundary_cond must be 0 (periodic) or 2 (outflow)");

@@ -56,7 +56,7 @@ typedef struct echo_config {
                                   Minkowski quartic */
   int32_t obtained_zz;         /* [TRACE_IMP] 0 'no', 1 'zz' */
   int32_t init_type;           /* [INIT_TYPE] (selects viscous tau_pi rules) */
-  int32_t derivatives_out;     /* [DER_OUT] */
+  int32_t derivatives_out;     /* [derivativ] must be 0: out_derivatives is not built */
   double x1, x2, y1, y2, z1, z2; /* [XMIN..ZMAX] */
   double tmin;                 /* [TSTART] */
   double cfl;                  /* [CFL] */

@@ -99,6 +99,15 @@ def test_glauber_mhd_initial_field(oracle, coordinates, magfield_type, divclean)
     g.close()
 
 
+def test_create_rejects_derivative_output():
+    """The derivative output (out_derivatives) is not built: a configuration
+    asking for it fails at creation instead of running without it."""
+    from echoamd import EchoError, EchoGrid
+    ic, rc = make_config(nx=8, ny=8, nz=4, coordinates=2, derivatives_out=1)
+    with pytest.raises(EchoError):
+        EchoGrid(ic, rc)
+
+
 def test_glauber_rejects_bad_input():
     from echoamd import EchoError, EchoGrid
     ic, rc = make_config(nx=8, ny=8, nz=4, coordinates=2)
