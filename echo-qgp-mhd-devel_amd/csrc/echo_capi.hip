@@ -95,12 +95,13 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   double* Uout = last ? g->U[0] : g->U[1];
   k_update<SYS, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
       g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
-      g->VS[0], g->VS[1], g->faillist, c1, c2, irk == 1 ? 1 : 0, last ? 1 : 0);
+      g->VS[0], g->VS[1], g->fix, c1, c2, irk == 1 ? 1 : 0, last ? 1 : 0);
   if (g->timing) hipEventRecord(g->ev[5], g->stream);
-  // con2prim repair of the cells k_update could not invert (rare); it
-  // writes the next step's vnewstep too, before k_end_step flips the parity
-  k_fixup<SYS, ZZ><<<1, 64, 0, g->stream>>>(g->ga, g->V, Uout, g->faillist, g->VS[0],
-                                             g->VS[1], last ? 1 : 0);
+  // con2prim repair of the cells k_update could not invert (rare; returns at
+  // once when there are none); it writes the next step's vnewstep too,
+  // before k_end_step flips the parity
+  k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, Uout, g->fix, g->VS[0],
+                                                    g->VS[1], last ? 1 : 0);
 }
 
 static void enqueue_stage(echo_grid* g, int irk) {
@@ -178,13 +179,18 @@ static void enqueue_check(echo_grid* g) {
 
 static int dump_kend(const echo_grid* g) { return g->cfg.mhd ? 8 : 5; }   // common.f90:379-386
 
+// vnewstep / unewstep of the restart dump, at the start of every step
+// (all three ways of stepping: captured steps, timed steps, single stages)
+static void enqueue_dump_copy(echo_grid* g) {
+  if (!g->dump_track) return;
+  const long long n = (long long)dump_kend(g) * g->N;
+  k_dump_copy<<<nblocks(n, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(g->S, g->V, g->U[0], g->dump_buf[0],
+                                                                        g->dump_buf[1], n);
+}
+
 static void enqueue_step(echo_grid* g) {
   enqueue_check(g);
-  if (g->dump_track) {
-    const long long n = (long long)dump_kend(g) * g->N;
-    k_dump_copy<<<nblocks(n, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(g->S, g->V, g->U[0], g->dump_buf[0],
-                                                                          g->dump_buf[1], n);
-  }
+  enqueue_dump_copy(g);
   for (int irk = 1; irk <= g->cfg.nrk; irk++) enqueue_stage(g, irk);
   enqueue_end_step(g);
 }
@@ -210,7 +216,11 @@ static void release(echo_grid* g) {
     hipFree(g->partial);
   }
   hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->tab); hipFree(g->tabix);
-  hipFree(g->S); hipFree(g->faillist); hipFree(g->status);
+  if (g->owns_scratch) {
+    hipFree(g->fix.Pold); hipFree(g->fix.Upre); hipFree(g->fix.failbits);
+    hipFree(g->fix.list); hipFree(g->fix.level);
+  }
+  hipFree(g->S); hipFree(g->status);
   hipFree(g->fo_f[0]); hipFree(g->fo_f[1]); hipFree(g->fo_f[2]); hipFree(g->fo_vb); hipFree(g->fo_mask);
   hipFree(g->fo_bsum); hipFree(g->fo_status); hipFree(g->fo_rec);
   hipFree(g->dump_buf[0]); hipFree(g->dump_buf[1]);
@@ -224,6 +234,9 @@ extern "C" {
 const char* echo_last_error(void) { return echo::g_err.c_str(); }
 
 }  // extern "C"
+
+// words of the failure bitmap of a grid of n cells
+static long long fix_words(long long n) { return (n + 31) / 32; }
 
 // Set-up of one grid.  stream: launch stream to use (not owned) or null for
 // a stream of its own; share_scratch: leave F, L and partial unallocated
@@ -298,7 +311,14 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   }
   if (!share_scratch) ALLOC(g->partial, sizeof(double) * (size_t)nb_cap);
   ALLOC(g->S, sizeof(DevScalars));
-  ALLOC(g->faillist, sizeof(int) * FAIL_CAP);
+  g->fix_upre_cap = (c.viscosity && c.smooth_temp > 0.) ? (long long)g->nv * g->N : 0;
+  if (!share_scratch) {
+    ALLOC(g->fix.Pold, sizeof(double) * (size_t)g->N);
+    if (g->fix_upre_cap) ALLOC(g->fix.Upre, sizeof(double) * (size_t)g->fix_upre_cap);
+    ALLOC(g->fix.failbits, sizeof(unsigned) * (size_t)fix_words(g->N));
+    ALLOC(g->fix.list, sizeof(int) * (size_t)g->N);
+    ALLOC(g->fix.level, sizeof(int) * (size_t)g->N);
+  }
   ALLOC(g->status, sizeof(int));
   if (stream) {
     g->stream = stream;
@@ -377,6 +397,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
     S0.dt = c.maxdt; S0.tout = c.tmin + c.dtout; S0.dtout = c.dtout; S0.tmax = c.tmax;
     S0.glm_ch_old = 0.5;
     S0.glm_ch = 0.5;
+    S0.crash_cell = 0x7fffffff;
     S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? pressure_from_temperature(Eh, c.smooth_temp) : 0.;
     if (hipMemcpy(g->S, &S0, sizeof S0, hipMemcpyHostToDevice) != hipSuccess) {
       release(g);
@@ -405,7 +426,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   a.divclean = c.mhd && c.divclean;
   a.glm_alpha = c.glm_alpha;
   a.S = g->S;
-  if (hipMemset(g->faillist, 0, sizeof(int) * FAIL_CAP) != hipSuccess) {
+  if (!share_scratch && hipMemset(g->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(g->N)) != hipSuccess) {
     release(g);
     return fail(ECHO_E_HIP, "memset failed");
   }
@@ -427,6 +448,7 @@ struct echo_ensemble {
   double* F[3] = {nullptr, nullptr, nullptr};
   double* L[3] = {nullptr, nullptr, nullptr};
   double* partial = nullptr;
+  FixBufs fix{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool dirty = true;
@@ -442,6 +464,8 @@ static void release_ensemble(echo_ensemble* e) {
   for (echo_grid* g : e->g) release(g);
   for (int d = 0; d < 3; d++) { hipFree(e->F[d]); hipFree(e->L[d]); }
   hipFree(e->partial);
+  hipFree(e->fix.Pold); hipFree(e->fix.Upre); hipFree(e->fix.failbits);
+  hipFree(e->fix.list); hipFree(e->fix.level);
   for (auto& v : e->ev) if (v) hipEventDestroy(v);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
@@ -632,12 +656,30 @@ static int run_init(echo_grid* g) {
   return st;
 }
 
+// the run's crash state, with a message saying what crashed it
 static int read_crash(echo_grid* g) {
   DevScalars S;
   HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
   HIPCHECK(hipStreamSynchronize(g->stream));
-  if (S.run_crashed) return S.errcode > 0 ? S.errcode : ECHO_E_CRASHED;
-  return 0;
+  if (!S.run_crashed) return 0;
+  const int code = S.errcode > 0 ? S.errcode : ECHO_E_CRASHED;
+  const int nx = g->cfg.nx, ny = g->cfg.ny;
+  switch (S.crash_kind) {
+    case CRASH_REPAIR: {
+      const long long c = S.crash_cell;
+      return fail(code, "cons2prim failed and its neighbour-average repair failed too (evolve_u2v, "
+                        "evolve.f90:836-840) at cell ix=%lld iy=%lld iz=%lld (1-based), t=%.17g",
+                  c % nx + 1, (c / nx) % ny + 1, c / ((long long)nx * ny) + 1, S.t);
+    }
+    case CRASH_FLUX:
+      return fail(code, "flux computation failed with error code %d (system_flux, system.f90:1316-1583), "
+                        "t=%.17g", S.errcode, S.t);
+    case CRASH_SOURCE:
+      return fail(code, "source terms failed with error code %d (system_source, system.f90:1587-1842), "
+                        "t=%.17g", S.errcode, S.t);
+    default:
+      return fail(code, "the run has crashed (error code %d), t=%.17g", S.errcode, S.t);
+  }
 }
 
 int echo_rk_stage(echo_grid* g, int32_t irk) {
@@ -645,7 +687,10 @@ int echo_rk_stage(echo_grid* g, int32_t irk) {
   if (irk != g->next_stage || irk < 1 || irk > g->cfg.nrk)
     return fail(ECHO_E_ARG, "stages must be called in order 1..nrk (expected %d)", g->next_stage);
   HIPCHECK(hipSetDevice(g->dev));
-  if (irk == 1) enqueue_check(g);
+  if (irk == 1) {
+    enqueue_check(g);
+    enqueue_dump_copy(g);
+  }
   enqueue_stage(g, irk);
   HIPCHECK(hipGetLastError());
   g->next_stage = irk + 1;
@@ -672,6 +717,7 @@ int echo_steps(echo_grid* g, int32_t nsteps) {
     int nst = 0;
     for (int s = 0; s < nsteps; s++) {
       enqueue_check(g);
+      enqueue_dump_copy(g);
       for (int irk = 1; irk <= g->cfg.nrk; irk++) {
         enqueue_stage(g, irk);
         HIPCHECK(hipEventSynchronize(g->ev[5]));
@@ -711,6 +757,16 @@ int echo_get_scalars(echo_grid* g, double* s) {
   HIPCHECK(hipStreamSynchronize(g->stream));
   s[0] = S.t; s[1] = S.dt; s[2] = S.timeold; s[3] = S.timeinterval; s[4] = S.g_cov_3_old;
   s[5] = S.glm_ch_old; s[6] = S.tout; s[7] = S.iter; s[8] = S.run_crashed; s[9] = S.errcode;
+  return 0;
+}
+
+int echo_get_repairs(echo_grid* g, int64_t* n) {
+  if (!g || !n) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  *n = S.nrepair;
   return 0;
 }
 
@@ -889,6 +945,7 @@ int echo_restart(echo_grid* g, const char* path, int32_t* iout, double* thyp) {
   S.t = t; S.timeold = timeold; S.timeinterval = timeinterval; S.g_cov_3_old = g3old;
   S.tout = t + S.dtout;
   S.run_crashed = 0; S.errcode = 0; S.temp_check = 0;
+  S.crash_kind = CRASH_NONE; S.crash_cell = 0x7fffffff;
   HIPCHECK(hipMemcpyAsync(g->S, &S, sizeof S, hipMemcpyHostToDevice, g->stream));
   HIPCHECK(hipMemcpyAsync(g->V, v.data(), sizeof(double) * nvN, hipMemcpyHostToDevice, g->stream));
   HIPCHECK(hipMemcpyAsync(g->U[0], u.data(), sizeof(double) * nvN, hipMemcpyHostToDevice, g->stream));
@@ -964,7 +1021,7 @@ int echo_ensemble_create(const echo_config* cfgs, int32_t n_events, const double
     return fail(ECHO_E_HIP, "stream creation failed");
   }
   size_t szF[3] = {0, 0, 0}, szL[3] = {0, 0, 0};
-  long long szP = 0;
+  long long szP = 0, maxN = 0, szUpre = 0;
   for (int i = 0; i < n_events; i++) {
     echo_grid* g = nullptr;
     int rc = grid_create(&cfgs[i], eos_rows, n_rows, device, e->stream, true, &g);
@@ -981,6 +1038,8 @@ int echo_ensemble_create(const echo_config* cfgs, int32_t n_events, const double
       if (g->cfg.viscosity) szL[d] = std::max(szL[d], (size_t)(4LL * g->nf[d]));
     }
     szP = std::max(szP, g->partial_cap);
+    maxN = std::max(maxN, g->N);
+    szUpre = std::max(szUpre, g->fix_upre_cap);
   }
   for (int d = 0; d < 3; d++) {
     if (szF[d] && hipMalloc(&e->F[d], sizeof(double) * szF[d]) != hipSuccess) {
@@ -996,12 +1055,24 @@ int echo_ensemble_create(const echo_config* cfgs, int32_t n_events, const double
     release_ensemble(e);
     return fail(ECHO_E_NOMEM, "partial buffer allocation failed");
   }
+  // the con2prim repair scratch, one set for all events (they run one
+  // after the other on one stream)
+  if (hipMalloc(&e->fix.Pold, sizeof(double) * (size_t)maxN) != hipSuccess ||
+      (szUpre && hipMalloc(&e->fix.Upre, sizeof(double) * (size_t)szUpre) != hipSuccess) ||
+      hipMalloc(&e->fix.failbits, sizeof(unsigned) * (size_t)fix_words(maxN)) != hipSuccess ||
+      hipMalloc(&e->fix.list, sizeof(int) * (size_t)maxN) != hipSuccess ||
+      hipMalloc(&e->fix.level, sizeof(int) * (size_t)maxN) != hipSuccess ||
+      hipMemset(e->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(maxN)) != hipSuccess) {
+    release_ensemble(e);
+    return fail(ECHO_E_NOMEM, "repair scratch allocation failed");
+  }
   for (echo_grid* g : e->g) {
     for (int d = 0; d < 3; d++) {
       g->F[d] = g->active[d] ? e->F[d] : nullptr;
       g->L[d] = (g->active[d] && g->cfg.viscosity) ? e->L[d] : nullptr;
     }
     g->partial = e->partial;
+    g->fix = e->fix;
   }
   for (auto& v : e->ev) hipEventCreate(&v);
   *out = e;

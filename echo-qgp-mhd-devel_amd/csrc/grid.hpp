@@ -56,7 +56,8 @@ struct echo_grid {
   int* tabix = nullptr;
   int n_tab = 0;
   DevScalars* S = nullptr;
-  int* faillist = nullptr;
+  echo::FixBufs fix{};          // con2prim repair scratch (shared inside an ensemble)
+  long long fix_upre_cap = 0;   // doubles of fix.Upre (0: no viscous smoothing)
   int* status = nullptr;
   GridArgs ga;
   hipGraph_t graph = nullptr;

@@ -23,11 +23,32 @@ struct DevScalars {
   double temp_end;         // check_temp threshold (<= 0: no check)
   double aflux[3];         // max(ap, am) at the last interface of each sweep
   double check_max;        // max temperature (or energy density) of the last check
+  long long nrepair;       // cells repaired by k_fixup since set-up
   int iter, run_crashed, errcode, nfail;
   int vs_parity;
   int freeze_type;         // 0: temperature, 1: energy density
   int temp_check;          // 1 once the check has stopped the run
+  int crash_kind;          // what crashed the run: CRASH_* below
+  int crash_cell;          // first cell (x fastest) whose repair failed
   int pad;
+};
+
+enum { CRASH_NONE = 0, CRASH_REPAIR = 1, CRASH_FLUX = 2, CRASH_SOURCE = 3 };
+
+// Scratch of the con2prim repair (k_update -> k_fixup), valid within one
+// stage; an ensemble shares one set sized for its largest event.
+//   Pold     N        the pressure each cell held before the stage
+//   Upre     nv * N   only with the viscous smoothing: u of the smoothed cells
+//                     before the smoothing
+//   failbits N / 32   one bit per cell whose con2prim failed (cleared by k_fixup)
+//   list     N        the failed cells in increasing order (k_fixup)
+//   level    N        dependency level of each failed cell (k_fixup)
+struct FixBufs {
+  double* Pold;
+  double* Upre;
+  unsigned* failbits;
+  int* list;
+  int* level;
 };
 
 struct GridArgs {
@@ -48,7 +69,7 @@ struct GridArgs {
 // the tail of a sweep (about 1% faster sweeps than 256)
 constexpr int FLUX_BLOCK = 128;
 constexpr int UPDATE_BLOCK = 256;
-constexpr int FAIL_CAP = 4096;
+constexpr int FIX_BLOCK = 256;
 constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
 
 // XCD-aware block order.  Workgroups are dispatched round-robin over the
@@ -263,6 +284,7 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
       // the reference stops the run (run_crashed) on a flux error
       atomicMax(&a.S->errcode, e);
       atomicExch(&a.S->run_crashed, 1);
+      atomicCAS(&a.S->crash_kind, CRASH_NONE, CRASH_FLUX);
 #pragma unroll
       for (int k = 0; k < NV; k++) F[(long long)k * nf + f] = 0.;
     } else {
@@ -412,14 +434,16 @@ __device__ __forceinline__ Deriv cell_deriv(const GridArgs& a, const Metric& M, 
 // combination u = c1*u0 + c2*(u - dt*du) (evolve.f90:199), con2prim
 // (evolve_u2v, evolve.f90:769-868) and, for viscous runs, the low-pressure
 // smoothing (evolve_smooth_viscosity, evolve.f90:1159-1207).  Cells whose
-// con2prim fails are appended to a list for k_fixup.
+// con2prim fails are marked in fb.failbits for k_fixup; what the repair
+// reads of the other cells is kept in fb: every cell's pressure before the
+// stage, and u before the smoothing of the smoothed cells.
 // ---------------------------------------------------------------------------
 template <int SYS, bool ZZ>
 __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     GridArgs a, double* __restrict__ V, double* U0, const double* Ucur, double* Uout,
     const double* __restrict__ FX, const double* __restrict__ FY, const double* __restrict__ FZ,
     const double* __restrict__ LX, const double* __restrict__ LY, const double* __restrict__ LZ,
-    double* __restrict__ VS0, double* __restrict__ VS1, int* __restrict__ faillist, double c1,
+    double* __restrict__ VS0, double* __restrict__ VS1, FixBufs fb, double c1,
     double c2, int first_stage, int last_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
@@ -455,6 +479,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(&V[k * N + c]);   // read once
+  __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
   Deriv D{};
   const int par = S->vs_parity;
   double* VSn = par ? VS1 : VS0;
@@ -465,7 +490,12 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 #pragma unroll
     for (int k = 0; k < NV; k++) vv[k] = v[k];
     int e = sys_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
-    if (e > 0) { atomicMax(&S->errcode, e); atomicExch(&S->run_crashed, 1); return; }
+    if (e > 0) {
+      atomicMax(&S->errcode, e);
+      atomicExch(&S->run_crashed, 1);
+      atomicCAS(&S->crash_kind, CRASH_NONE, CRASH_SOURCE);
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < NV; k++) du[k] = du[k] - src[k];
   }
@@ -498,11 +528,13 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   }
   int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);
   if (e != 0) {
-    int slot = atomicAdd(&S->nfail, 1);
-    if (slot < FAIL_CAP) faillist[slot] = (int)c;
+    atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+    atomicAdd(&S->nfail, 1);
   } else if (VISC && a.enable_smooth) {
     const double ps = S->smooth_pressure;
     if (v[KPR] < ps) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) fb.Upre[k * N + c] = un[k];
       double q = v[KPR] / ps;
 #pragma unroll
       for (int k = KPIBU; k <= KPIZZ; k++) v[k] = v[k] * (q * q);
@@ -528,78 +560,211 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 
 // ---------------------------------------------------------------------------
 // k_fixup: the con2prim repair of evolve_u2v (evolve.f90:795-842) for the
-// cells listed by k_update, in increasing cell order, by one thread.  The
-// neighbour averages read the six face neighbours, clamped to the grid
-// (the reference reads outside its arrays on boundary cells).
+// cells that k_update could not invert, with the reference's sequential
+// semantics.  The reference repairs cell by cell in loop order (x fastest)
+// while it inverts the grid, so a failed cell sees:
+//   - u of its face neighbours after their inversion (and repair) when they
+//     come earlier in the loop, the RK result when they come later; both
+//     before the viscous smoothing, which runs after the whole loop;
+//   - the pressure of its neighbours from the new primitives when they come
+//     earlier, from the old ones (before the stage) when they come later.
+// Here the failed cells are taken in dependency levels: a failed cell waits
+// for its earlier failed face neighbours (lower levels), and is done before
+// its later ones (higher levels), so within a level the cells are
+// independent and run in parallel.  The neighbour values come from U and V
+// (new), fb.Pold (old pressure) and fb.Upre (u of smoothed cells before the
+// smoothing).  Boundary neighbours are clamped to the cell itself (the
+// reference reads outside its arrays there).  The smoothing of the repaired
+// cells follows the repairs.  One block.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool fix_failed(const unsigned* __restrict__ bits, long long q) {
+  return (bits[q >> 5] >> (q & 31)) & 1u;
+}
+
 template <int SYS, bool ZZ>
-__global__ void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
-                        int* __restrict__ faillist, double* __restrict__ VS0,
-                        double* __restrict__ VS1, int last_stage) {
+__device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, double* __restrict__ V,
+                            double* __restrict__ U, const FixBufs& fb, double* __restrict__ VSnext,
+                            int last_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevScalars* S = a.S;
-  if (S->temp_check) return;
-  // glm_ch_old = maxval(aflux) of this stage; the next stage starts with
-  // glm_ch = glm_ch_old (evolve.f90:157-163)
-  S->glm_ch_old = dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
-  S->glm_ch = S->glm_ch_old;
-  int n = S->nfail;
-  if (n == 0) return;
-  S->nfail = 0;
-  if (n > FAIL_CAP) { S->errcode = 1; S->run_crashed = 1; return; }
-  for (int i = 1; i < n; i++) {   // insertion sort: n is small
-    int key = faillist[i], j = i - 1;
-    while (j >= 0 && faillist[j] > key) { faillist[j + 1] = faillist[j]; j--; }
-    faillist[j + 1] = key;
-  }
-  const Metric M = make_metric(a.coordinates, S->t);
   const long long N = a.N;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x = (int)(c % nx);
+  const long long r = c / nx;
+  const int y = (int)(r % ny), z = (int)(r / ny);
+  const long long sxy = (long long)nx * ny;
+  const long long nb[6] = {c - (x > 0 ? 1 : 0), c + (x < nx - 1 ? 1 : 0),
+                           c - (y > 0 ? nx : 0), c + (y < ny - 1 ? nx : 0),
+                           c - (z > 0 ? sxy : 0), c + (z < nz - 1 ? sxy : 0)};
+  // u of a neighbour as the reference's loop sees it: a smoothed cell's u
+  // before its smoothing (fb.Upre), else U (the RK result of later cells,
+  // failed ones included, and the repaired u of earlier failed cells)
+  const double* us[6];
+  const bool smooth = VISC && a.enable_smooth;
+  const double ps = S->smooth_pressure;
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    const long long q = nb[j];
+    const bool pre = smooth && q != c && !fix_failed(fb.failbits, q) && V[KPR * N + q] < ps;
+    us[j] = pre ? fb.Upre : U;
+  }
+  double uu[NV], vv[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const long long o = (long long)k * N;
+    uu[k] = (us[0][o + nb[0]] + us[1][o + nb[1]] + us[2][o + nb[2]] + us[3][o + nb[3]] + us[4][o + nb[4]] +
+             us[5][o + nb[5]]) / 6.;
+    vv[k] = V[o + c];   // the primitives k_update's failed inversion left
+  }
+  cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
+  double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
+  if (v2 > 1) {
+    const double sc = a.maxspeed / sqrt(v2);
+    vv[KVX] = sc * vv[KVX]; vv[KVY] = sc * vv[KVY]; vv[KVZ] = sc * vv[KVZ];
+  }
+  if (vv[KPR] < 0) {
+    double p[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) p[j] = nb[j] < c ? V[KPR * N + nb[j]] : fb.Pold[nb[j]];
+    vv[KPR] = (p[0] + p[1] + p[2] + p[3] + p[4] + p[5]) / 6.;
+  }
+  prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
+  cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
+  v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
+  // u is stored before the check (evolve.f90:848), pi^{eta eta} too with 'zz'
+#pragma unroll
+  for (int k = 0; k < NV; k++) U[(long long)k * N + c] = uu[k];
+  if constexpr (VISC && ZZ) V[KPIZZ * N + c] = vv[KPIZZ];
+  if (!((vv[KPR] > 0) && (v2 < 1))) {
+    atomicMax(&S->errcode, 1);
+    atomicExch(&S->run_crashed, 1);
+    atomicCAS(&S->crash_kind, CRASH_NONE, CRASH_REPAIR);
+    atomicMin(&S->crash_cell, (int)c);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) V[(long long)k * N + c] = vv[k];
+  if (VISC && last_stage) {
+    VSnext[c] = vv[KVX]; VSnext[N + c] = vv[KVY]; VSnext[2 * N + c] = vv[KVZ];
+  }
+}
+
+template <int SYS, bool ZZ>
+__global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
+                                                     FixBufs fb, double* __restrict__ VS0,
+                                                     double* __restrict__ VS1, int last_stage) {
+  constexpr bool VISC = SysT<SYS>::VISC;
+  constexpr int NV = SysT<SYS>::NV;
+  DevScalars* S = a.S;
+  const int t = threadIdx.x;
+  __shared__ int sh_go, sh_maxlev, sh_stop;
+  __shared__ int scan[FIX_BLOCK + 1];
+  if (t == 0) {
+    sh_go = 0;
+    if (!S->temp_check) {
+      // glm_ch_old = maxval(aflux) of this stage; the next stage starts with
+      // glm_ch = glm_ch_old (evolve.f90:157-163)
+      S->glm_ch_old = dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
+      S->glm_ch = S->glm_ch_old;
+      sh_go = S->nfail > 0;
+    }
+  }
+  __syncthreads();
+  if (!sh_go) return;
+  const long long N = a.N;
+  const long long words = (N + 31) >> 5;
+  const long long per = (words + FIX_BLOCK - 1) / FIX_BLOCK;
+  const long long w0 = t * per < words ? t * per : words;
+  const long long w1 = w0 + per < words ? w0 + per : words;
+  // 1. the failed cells in increasing order: popcounts per thread chunk,
+  //    exclusive scan, then each thread lists its own chunk
+  int cnt = 0;
+  for (long long w = w0; w < w1; w++) cnt += __popc(fb.failbits[w]);
+  scan[t + 1] = cnt;
+  __syncthreads();
+  if (t == 0) {
+    scan[0] = 0;
+    for (int i = 1; i <= FIX_BLOCK; i++) scan[i] += scan[i - 1];
+  }
+  __syncthreads();
+  const int n = scan[FIX_BLOCK];
+  {
+    int o = scan[t];
+    for (long long w = w0; w < w1; w++) {
+      unsigned b = fb.failbits[w];
+      while (b) {
+        const int i = __ffs(b) - 1;
+        b &= b - 1;
+        fb.list[o++] = (int)(w * 32 + i);
+      }
+    }
+  }
+  __syncthreads();
+  // 2. dependency levels, in cell order: one more than the highest level of
+  //    the failed face neighbours that come earlier (x-1, y-1, z-1)
+  if (t == 0) {
+    int maxlev = 0;
+    const int nx = a.nx, ny = a.ny;
+    const long long sxy = (long long)nx * ny;
+    for (int i = 0; i < n; i++) {
+      const long long c = fb.list[i];
+      const int x = (int)(c % nx);
+      const long long r = c / nx;
+      const int y = (int)(r % ny), z = (int)(r / ny);
+      int lev = 0;
+      if (x > 0 && fix_failed(fb.failbits, c - 1)) lev = max(lev, fb.level[c - 1] + 1);
+      if (y > 0 && fix_failed(fb.failbits, c - nx)) lev = max(lev, fb.level[c - nx] + 1);
+      if (z > 0 && fix_failed(fb.failbits, c - sxy)) lev = max(lev, fb.level[c - sxy] + 1);
+      fb.level[c] = lev;
+      maxlev = max(maxlev, lev);
+    }
+    sh_maxlev = maxlev;
+    sh_stop = 0;
+  }
+  __syncthreads();
+  // 3. the repairs, one level after the other.  A failed repair crashes the
+  //    run; the reference returns from evolve_u2v there, leaving the later
+  //    cells as they were, while here every level is still repaired: the
+  //    cells before the crashing one end as in the reference, the state of
+  //    the later ones is not meaningful after a crash either way.
+  const Metric M = make_metric(a.coordinates, S->t);
   double* VSnext = S->vs_parity ? VS0 : VS1;
-  for (int i = 0; i < n; i++) {
-    long long c = faillist[i];
-    int x = (int)(c % nx); long long r = c / nx; int y = (int)(r % ny), z = (int)(r / ny);
-    long long nb[6] = {
-        c - (x > 0 ? 1 : 0), c + (x < nx - 1 ? 1 : 0),
-        c - (y > 0 ? nx : 0), c + (y < ny - 1 ? nx : 0),
-        c - (z > 0 ? (long long)nx * ny : 0), c + (z < nz - 1 ? (long long)nx * ny : 0)};
-    double uu[NV], vv[NV];
+  for (int lev = 0; lev <= sh_maxlev; lev++) {
+    for (int i = t; i < n; i += FIX_BLOCK) {
+      const long long c = fb.list[i];
+      if (fb.level[c] == lev) repair_cell<SYS, ZZ>(a, M, c, V, U, fb, VSnext, last_stage);
+    }
+    __syncthreads();
+  }
+  if (t == 0) sh_stop = S->run_crashed;
+  __syncthreads();
+  // 4. evolve_smooth_viscosity (evolve.f90:1159-1207) of the repaired cells
+  //    (not after a crash: evolve_main returns before it, evolve.f90:276-281)
+  if (VISC && a.enable_smooth && !sh_stop) {
+    const double ps = S->smooth_pressure;
+    for (int i = t; i < n; i += FIX_BLOCK) {
+      const long long c = fb.list[i];
+      if (V[KPR * N + c] < ps) {
+        double vv[NV], uu[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-      const double* Uk = U + k * N;
-      uu[k] = (Uk[nb[0]] + Uk[nb[1]] + Uk[nb[2]] + Uk[nb[3]] + Uk[nb[4]] + Uk[nb[5]]) / 6.;
-      vv[k] = V[k * N + c];
-    }
-    cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
-    double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
-    if (v2 > 1) {
-      double s = a.maxspeed / sqrt(v2);
-      vv[KVX] = s * vv[KVX]; vv[KVY] = s * vv[KVY]; vv[KVZ] = s * vv[KVZ];
-    }
-    if (vv[KPR] < 0) {
-      const double* P = V + KPR * N;
-      vv[KPR] = (P[nb[0]] + P[nb[1]] + P[nb[2]] + P[nb[3]] + P[nb[4]] + P[nb[5]]) / 6.;
-    }
-    prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
-    cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
-    v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * M.g3;
-    if (!((vv[KPR] > 0) && (v2 < 1))) {
-      S->errcode = 1; S->run_crashed = 1; return;
-    }
-    if (VISC && a.enable_smooth && vv[KPR] < S->smooth_pressure) {
-      double q = vv[KPR] / S->smooth_pressure;
+        for (int k = 0; k < NV; k++) vv[k] = V[(long long)k * N + c];
+        const double q = vv[KPR] / ps;
 #pragma unroll
-      for (int k = KPIBU; k <= KPIZZ; k++) vv[k] = vv[k] * (q * q);
-      if constexpr (ZZ) derived_pi<true>(vv, M);
-      prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
-    }
+        for (int k = KPIBU; k <= KPIZZ; k++) vv[k] = vv[k] * (q * q);
+        if constexpr (ZZ) derived_pi<true>(vv, M);
+        prim2cons<SYS, ZZ>(a.eos, M, vv, uu);
 #pragma unroll
-    for (int k = 0; k < NV; k++) { U[k * N + c] = uu[k]; V[k * N + c] = vv[k]; }
-    if (VISC && last_stage) {
-      VSnext[c] = vv[KVX]; VSnext[N + c] = vv[KVY]; VSnext[2 * N + c] = vv[KVZ];
+        for (int k = 0; k < NV; k++) { U[(long long)k * N + c] = uu[k]; V[(long long)k * N + c] = vv[k]; }
+      }
     }
+  }
+  // 5. clear the marks for the next stage
+  for (long long w = w0; w < w1; w++)
+    if (fb.failbits[w]) fb.failbits[w] = 0u;
+  if (t == 0) {
+    S->nfail = 0;
+    S->nrepair += n;
   }
 }
 

@@ -1,8 +1,9 @@
 // Device physics for the ECHO-QGP grid update on gfx950: equation of state,
 // reconstruction, fluxes, con2prim and sources, one cell or one interface per
-// thread, FP64 throughout.  Operation order follows the reference so that the
-// FP64 results agree with it to rounding (the library is built with FP
-// contraction off).  Reference file:line citations on each function.
+// thread, FP64 throughout.  Operation order follows the reference; the
+// library is built with -ffp-contract=fast, so multiply-adds become FMAs and
+// the FP64 results agree with the reference to rounding, not bit for bit.
+// Reference file:line citations on each function.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -1052,7 +1053,11 @@ __device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const do
   double sz = gm * u[KVZ] / M.g3;
   double et = gm * u[KPR];
   double s2 = M.g1 * sx * sx + M.g2 * sy * sy + M.g3 * sz * sz;
-  double pr, w = 0., v2, glf, rh = 0., en, drhdpr, dprdrh, dprden, f, df, dpr, pr_eos = 0.;
+  // eos_pressure leaves its outputs unset on an error (out of table range)
+  // and the Newton step still uses them (system.f90:534-538, 627-631);
+  // before the first successful call the reference reads uninitialised
+  // locals there.  They start at 0 here.
+  double pr, w = 0., v2, glf, rh = 0., en, drhdpr, dprdrh = 0., dprden = 0., f, df, dpr, pr_eos = 0.;
   if constexpr (!VISC) {
     pr = v[KPR];
     for (int iter = 1; iter <= 100; iter++) {

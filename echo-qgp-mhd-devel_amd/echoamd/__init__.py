@@ -35,7 +35,7 @@ EXPORTS = (
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
     "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
     "echo_fo_setup", "echo_fo_slice", "echo_fo_width", "echo_fo_elements", "echo_fo_write",
-    "echo_set_dump", "echo_dump", "echo_restart", "echo_init_glauber_mhd",
+    "echo_set_dump", "echo_dump", "echo_restart", "echo_init_glauber_mhd", "echo_get_repairs",
 )
 
 # out_vars field flags (type out_type of the reference), in file order
@@ -122,6 +122,7 @@ def load_library(path=LIB_PATH):
         "echo_rk_stage": (C.c_int, [P, C.c_int32]),
         "echo_end_step": (C.c_int, [P]),
         "echo_get_scalars": (C.c_int, [P, _dp]),
+        "echo_get_repairs": (C.c_int, [P, C.POINTER(C.c_int64)]),
         "echo_set_tout": (C.c_int, [P, C.c_double]),
         "echo_cons2prim_batch": (C.c_int, [P, _dp, _dp, _ip, C.c_int64]),
         "echo_set_timing": (C.c_int, [P, C.c_int32]),
@@ -436,6 +437,12 @@ class EchoGrid:
         return dict(t=s[0], dt=s[1], timeold=s[2], timeinterval=s[3], g_cov_3_old=s[4],
                     glm_ch_old=s[5], tout=s[6], iter=int(s[7]), crashed=int(s[8]),
                     errcode=int(s[9]))
+
+    def repairs(self):
+        """Cells repaired by the con2prim repair (evolve_u2v) since set-up."""
+        n = C.c_int64()
+        self._check(self.lib.echo_get_repairs(self.h, C.byref(n)))
+        return n.value
 
     def set_tout(self, tout):
         self._check(self.lib.echo_set_tout(self.h, float(tout)))

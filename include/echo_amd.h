@@ -160,6 +160,11 @@ int echo_end_step(echo_grid *g);
  * s[5] glm_ch_old  s[6] tout  s[7] iterations  s[8] run_crashed  s[9] errcode */
 int echo_get_scalars(echo_grid *g, double *s);
 
+/* Number of cells repaired since set-up by the con2prim repair of
+ * evolve_u2v (evolve.f90:795-842): cells whose cons2prim failed and that
+ * were rebuilt from the average of their face neighbours. */
+int echo_get_repairs(echo_grid *g, int64_t *n);
+
 /* Next output time tout (echo.f90:172, 243-247); the dt clamps of
  * evolve.f90:179-188 use it. */
 int echo_set_tout(echo_grid *g, double tout);

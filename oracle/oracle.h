@@ -43,6 +43,16 @@ void orc_get_state(double *v, double *u);
 void orc_get_deriv(double *d);
 void orc_get_scalars(double *scal);
 int  orc_steps(int nsteps);
+/* one RK stage irk (evolve_main) without the step bookkeeping */
+int  orc_stage(int irk);
+/* evolve_u2v alone on the current v, u */
+int  orc_u2v(void);
+/* out[0..7]: how often each repair branch fired since orc_setup
+ * (REP_* of oracle_int.h) */
+void orc_get_repair_counts(long *out);
+/* the cells (x fastest) of the first min(n, 256) u2v repairs; returns how
+ * many were written */
+int  orc_get_repair_cells(long *out, int n);
 
 /* pointwise operators at the current metric */
 void orc_set_time(double t);

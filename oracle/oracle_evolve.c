@@ -112,6 +112,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.iter = 0; G.run_crashed = 0; G.errcode = 0;
   G.smooth_first = 1;
   G.temp_end = 0.; G.check_max = 0.; G.freeze_type = 0; G.temp_check = 0;
+  memset(G.nrep, 0, sizeof G.nrep);
   return G.nv;
 }
 
@@ -148,6 +149,15 @@ void orc_get_state(double *v, double *u) {
 }
 
 void orc_get_deriv(double *d) { memcpy(d, G.deriv, ncell() * NDERIV * sizeof(double)); }
+
+void orc_get_repair_counts(long *out) { memcpy(out, G.nrep, sizeof G.nrep); }
+
+int orc_get_repair_cells(long *out, int n) {
+  int m = G.nrep[REP_U2V] < REPLOG ? (int)G.nrep[REP_U2V] : REPLOG;
+  if (m > n) m = n;
+  memcpy(out, G.replog, sizeof(long) * m);
+  return m;
+}
 
 void orc_get_scalars(double *s) {
   s[0] = G.t; s[1] = G.dt; s[2] = G.timeold; s[3] = G.timeinterval;
@@ -257,22 +267,26 @@ static int sweep(int dir) {
 #define W0(k) c0[(k) * len]
 #define WR(k) wr[(k) * (m + 1) + i]
 #define WL(k) wl[(k) * (m + 1) + i]
-        if (WR(KPR) <= 0) WR(KPR) = W0(KPR);
+        if (WR(KPR) <= 0) { WR(KPR) = W0(KPR); G.nrep[REP_REC_PR]++; }
         double v2 = WR(KVX) * WR(KVX) * g[0] + WR(KVY) * WR(KVY) * g[1] + WR(KVZ) * WR(KVZ) * g[2];
         if (v2 >= 1.) {
+          G.nrep[REP_REC_V]++;
           WR(KVX) = W0(KVX); WR(KVY) = W0(KVY); WR(KVZ) = W0(KVZ);
           v2 = WR(KVX) * WR(KVX) * g[0] + WR(KVY) * WR(KVY) * g[1] + WR(KVZ) * WR(KVZ) * g[2];
           if (v2 >= 1) {
+            G.nrep[REP_REC_VMAX]++;
             double s = sqrt(v2);
             for (int k = KVX; k <= KVZ; k++) { W0(k) = W0(k) * G.maxspeed / s; WR(k) = W0(k); }
           }
         }
-        if (WL(KPR) <= 0) WL(KPR) = W0(KPR);
+        if (WL(KPR) <= 0) { WL(KPR) = W0(KPR); G.nrep[REP_REC_PR]++; }
         v2 = WL(KVX) * WL(KVX) * g[0] + WL(KVY) * WL(KVY) * g[1] + WL(KVZ) * WL(KVZ) * g[2];
         if (v2 >= 1.) {
+          G.nrep[REP_REC_V]++;
           WL(KVX) = W0(KVX); WL(KVY) = W0(KVY); WL(KVZ) = W0(KVZ);
           v2 = WL(KVX) * WL(KVX) * g[0] + WL(KVY) * WL(KVY) * g[1] + WL(KVZ) * WL(KVZ) * g[2];
           if (v2 >= 1) {
+            G.nrep[REP_REC_VMAX]++;
             double s = sqrt(v2);
             for (int k = KVX; k <= KVZ; k++) { W0(k) = W0(k) * G.maxspeed / s; WL(k) = W0(k); }
           }
@@ -438,6 +452,8 @@ static int evolve_u2v(void) {
         for (int k = 0; k < G.nv; k++) { vv[k] = G.v[k * nc + c]; uu[k] = G.u[k * nc + c]; }
         int e = sys_cons2prim(uu, vv);
         if (e != 0) {
+          if (G.nrep[REP_U2V] < REPLOG) G.replog[G.nrep[REP_U2V]] = (long)c;
+          G.nrep[REP_U2V]++;
           int xl = ix > 0 ? ix - 1 : ix, xr = ix < G.nx - 1 ? ix + 1 : ix;
           int yl = iy > 0 ? iy - 1 : iy, yr = iy < G.ny - 1 ? iy + 1 : iy;
           int zl = iz > 0 ? iz - 1 : iz, zr = iz < G.nz - 1 ? iz + 1 : iz;
@@ -452,10 +468,12 @@ static int evolve_u2v(void) {
           const double *g = G.g_cov;
           double v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * g[2];
           if (v2 > 1) {
+            G.nrep[REP_U2V_V]++;
             double s = G.maxspeed / sqrt(v2);
             vv[KVX] = s * vv[KVX]; vv[KVY] = s * vv[KVY]; vv[KVZ] = s * vv[KVZ];
           }
           if (vv[KPR] < 0) {
+            G.nrep[REP_U2V_PR]++;
             const double *p = &G.v[KPR * nc];
             vv[KPR] = (p[n6[0]] + p[n6[1]] + p[n6[2]] + p[n6[3]] + p[n6[4]] + p[n6[5]]) / 6.;
           }
@@ -463,8 +481,12 @@ static int evolve_u2v(void) {
           sys_cons2prim(uu, vv);
           v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * g[2];
           e = ((vv[KPR] > 0) && (v2 < 1)) ? 0 : 1;
+          if (e) G.nrep[REP_U2V_FAIL]++;
         }
         if (G.viscosity && G.obtained_zz) G.v[KPIZZ * nc + c] = vv[KPIZZ];
+        /* u is written before the error check (evolve.f90:848-849): a cell
+         * whose repair failed keeps its last attempt in u */
+        for (int k = 0; k < G.nu; k++) G.u[k * nc + c] = uu[k];
         if (e > 0) { G.errcode = e; G.run_crashed = 1; return e; }
         for (int k = 0; k < G.nv; k++) { G.v[k * nc + c] = vv[k]; G.u[k * nc + c] = uu[k]; }
       }
@@ -619,6 +641,26 @@ int orc_steps(int nsteps) {
       if ((fabs(G.tout - G.tmax) < 1.e-10) || (G.tout > G.tmax)) G.tout = G.tmax;
     }
   }
+  return 0;
+}
+
+/* evolve_u2v alone on the current v, u (host-forced repair states) */
+int orc_u2v(void) {
+  G.run_crashed = 0; G.errcode = 0;
+  int e = evolve_u2v();
+  return e ? e : (G.run_crashed ? 1 : 0);
+}
+
+/* one RK stage (evolve_main) without the step bookkeeping of orc_steps;
+ * irk = 1 also takes the step-start snapshot */
+int orc_stage(int irk) {
+  const size_t nc = ncell();
+  if (irk == 1) {
+    memcpy(G.vnewstep, G.v, G.kvold_end * nc * sizeof(double));
+    memcpy(G.unewstep, G.u, G.kvold_end * nc * sizeof(double));
+  }
+  int e = evolve_main(irk);
+  if (e || G.run_crashed) { G.run_crashed = 1; return e ? e : 1; }
   return 0;
 }
 

@@ -51,7 +51,26 @@ typedef struct {
   double *v, *u, *u0, *du, *du_stiff, *deriv, *vnewstep, *vold, *unewstep, *uold;
   int kvold_end;   /* components krh..kvold_end of vold/uold: 5 (hydro), 8 (MHD) */
   double *ax, *ay, *az;
+  /* how often each repair branch fired since orc_setup (REP_* below) */
+  long nrep[8];
+  /* cells (cell_index) of the first REPLOG u2v repairs since orc_setup */
+#define REPLOG 256
+  long replog[REPLOG];
 } orc_state;
+
+/* repair branches of evolve_df{x,y,z} (evolve.f90:334-369) and
+ * evolve_u2v (evolve.f90:795-842) */
+enum { REP_REC_PR = 0,    /* reconstructed p <= 0 replaced by the cell's */
+       REP_REC_V = 1,     /* reconstructed v^2 >= 1 replaced by the cell's */
+       REP_REC_VMAX = 2,  /* the cell's own v^2 >= 1: scaled to maxspeed */
+       REP_U2V = 3,       /* cons2prim failed: neighbour-average repair */
+       REP_U2V_V = 4,     /* repaired v^2 > 1 scaled to maxspeed */
+       REP_U2V_PR = 5,    /* repaired p < 0 replaced by the neighbour average */
+       REP_U2V_FAIL = 6,  /* repair failed: run crashed */
+       REP_C2P_UNDEF = 7 };/* an ideal or viscous cons2prim Newton step used
+                              eos_pressure outputs before any call of that
+                              cons2prim had set them: the reference then
+                              reads uninitialised locals (see sys_cons2prim) */
 
 extern orc_state G;
 

@@ -625,7 +625,14 @@ int sys_cons2prim(double *u, double *v) {
   double sz = gm * u[KVZ] / g[2];
   double et = gm * u[KPR];
   double s2 = g[0] * sx * sx + g[1] * sy * sy + g[2] * sz * sz;
-  double pr, w = 0, v2, glf, rh = 0, en, drhdpr, dendpr, pr_eos = 0, dprdrh, dprden, f, df, dpr;
+  /* pr_eos, dprdrh, dprden: when eos_pressure fails (out of table range)
+   * it leaves its outputs unset and the Newton step below still uses them
+   * (system.f90:534-538, 627-631: errcode is only checked after the loop).
+   * Before the first successful call the reference reads uninitialised
+   * locals there, whose values depend on the stack; this restatement (and
+   * the GPU build) start them at 0 and count such steps in REP_C2P_UNDEF. */
+  double pr, w = 0, v2, glf, rh = 0, en, drhdpr, dendpr, pr_eos = 0, dprdrh = 0, dprden = 0, f, df, dpr;
+  int eos_set = 0;
 
   if (!G.viscosity && !G.mhd) {
     pr = v[KPR];
@@ -639,6 +646,8 @@ int sys_cons2prim(double *u, double *v) {
       drhdpr = rh * v2 * glf / (en + pr);
       dendpr = v2;
       errcode = eos_pressure(rh, en, &pr_eos, &dprdrh, &dprden);
+      if (errcode == 0) eos_set = 1;
+      else if (!eos_set) G.nrep[REP_C2P_UNDEF]++;
       f = pr_eos - pr;
       df = dprdrh * drhdpr + dprden * dendpr - 1.;
       dpr = -f / df;
@@ -685,6 +694,8 @@ int sys_cons2prim(double *u, double *v) {
         drhdpr = rh * v2 * glf / (en + pr);
         dendpr = v2;
         errcode = eos_pressure(rh, en, &pr_eos, &dprdrh, &dprden);
+        if (errcode == 0) eos_set = 1;
+        else if (!eos_set) G.nrep[REP_C2P_UNDEF]++;
         f = pr_eos - pr;
         df = dprdrh * drhdpr + dprden * dendpr - 1.;
         dpr = -f / df;

@@ -285,6 +285,36 @@ contains
     end do
   end function ref_steps
 
+  ! evolve_u2v (evolve.f90:769-868) alone on the current v, u: pins the
+  ! con2prim repair on host-forced states.  Returns errcode if the run
+  ! crashed, else 0.
+  integer(c_int) function ref_u2v() bind(C, name="ref_u2v")
+    run_crashed = .false.
+    errcode = 0
+    call evolve_u2v
+    ref_u2v = 0
+    if (run_crashed) ref_u2v = max(errcode, 1)
+  end function ref_u2v
+
+  ! One RK stage, evolve_main(irk, dt) of evolve.f90:114-298, without the
+  ! step bookkeeping of ref_steps (for stage-by-stage comparisons).  irk = 1
+  ! also takes the step-start snapshot vnewstep / unewstep.
+  integer(c_int) function ref_stage(irk) bind(C, name="ref_stage")
+    integer(c_int), value :: irk
+    if (irk == 1) then
+      if (mhd) then
+        vnewstep = v(:, :, :, krh:kbz)
+        unewstep = u(:, :, :, krh:kbz)
+      else
+        vnewstep = v(:, :, :, krh:kpr)
+        unewstep = u(:, :, :, krh:kpr)
+      end if
+    end if
+    call evolve_main(irk, h_dt)
+    ref_stage = 0
+    if (run_crashed) ref_stage = max(errcode, 1)
+  end function ref_stage
+
   ! ---------------------------------------------------------------------
   ! Function-level entry points (pointwise operators of system.f90,
   ! holib.f90, eos.f90, viscous.f90), all at the current metric.

@@ -90,12 +90,52 @@ TRAJ = {
 }
 
 
+def repair_fixture(ref):
+    """The repair cases of tests/test_oracle_repair.py: inputs (initial
+    primitives, and the host-forced state where there is one) and the
+    reference's results."""
+    from test_oracle_repair import FORCED, ROUGH, forced_state, u2v_state
+    from scenarios import fast_eta
+    out = {}
+    cases = {}
+    for name, (make, steps, _) in ROUGH.items():
+        ic, rc, v0 = make()
+        cases[name] = (ic, rc, v0, None, steps)
+    for name, (kw, steps, _) in FORCED.items():
+        ic, rc, v0, v, u = forced_state(ref, kw)
+        cases[name] = (ic, rc, v0, (v, u), steps)
+    ic, rc, v0 = fast_eta(rec="TVD2")
+    cases["fast-eta-crash"] = (ic, rc, v0, None, 2)
+    for name, (ic, rc, v0, st, steps) in cases.items():
+        ref.setup(ic, rc)
+        assert ref.init_from_prim(v0) == 0
+        if st is not None:
+            ref.set_state(*st)
+            out[f"{name}/v_in"], out[f"{name}/u_in"] = st
+        e = ref.steps(steps)
+        v, u = ref.state()
+        out.update({f"{name}/ic": ic, f"{name}/rc": rc, f"{name}/v0": v0, f"{name}/steps": steps,
+                    f"{name}/err": e, f"{name}/v": v, f"{name}/u": u})
+    # evolve_u2v alone
+    from libs import Oracle
+    ic, rc, v0, v, u = u2v_state(Oracle())
+    ref.setup(ic, rc)
+    ref.init_from_prim(v0)
+    ref.set_state(v, u)
+    e = ref.u2v()
+    vr, ur = ref.state()
+    out.update({"u2v/ic": ic, "u2v/rc": rc, "u2v/v0": v0, "u2v/v_in": v, "u2v/u_in": u, "u2v/err": e,
+                "u2v/v": vr, "u2v/u": ur})
+    return out
+
+
 def main():
     ref = Ref()
     rng = np.random.default_rng(42)
     np.savez_compressed(os.path.join(HERE, "rec.npz"), **rec_fixture(ref, rng))
     np.savez_compressed(os.path.join(HERE, "eos.npz"), **eos_fixture(ref, rng))
     np.savez_compressed(os.path.join(HERE, "system.npz"), **system_fixture(ref, rng))
+    np.savez_compressed(os.path.join(HERE, "repair.npz"), **repair_fixture(ref))
     for name, (make, n) in TRAJ.items():
         ic, rc, v0 = make()
         ref.setup(ic, rc)

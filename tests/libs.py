@@ -59,6 +59,8 @@ class _Lib:
         self.c_eos_d = self._fn("eos_derived", C.c_int, [dbl, dbl, _dp])
         self.c_eos_tab = self._fn("eos_table", C.c_int, [C.c_int, _dp, _dp, _dp, _dp])
         self.c_przero = self._fn("przero", dbl, [])
+        self.c_stage = self._fn("stage", C.c_int, [C.c_int])
+        self.c_u2v = self._fn("u2v", C.c_int, [])
 
     # -- grid-level -------------------------------------------------------
     def setup(self, ic, rc):
@@ -102,6 +104,14 @@ class _Lib:
 
     def steps(self, n):
         return self.c_steps(int(n))
+
+    def stage(self, irk):
+        """one RK stage (evolve_main) without the step bookkeeping"""
+        return self.c_stage(int(irk))
+
+    def u2v(self):
+        """evolve_u2v alone on the current v, u"""
+        return self.c_u2v()
 
     # -- pointwise --------------------------------------------------------
     def set_time(self, t):
@@ -177,6 +187,23 @@ class Oracle(_Lib):
 
     def _setup(self, ic, rc):
         return self._csetup(ic, rc, EOS_TABLE.encode())
+
+    # names of the repair counters (REP_* of oracle/oracle_int.h)
+    REPAIRS = ("rec_pr", "rec_v", "rec_vmax", "u2v", "u2v_v", "u2v_pr", "u2v_fail", "c2p_undef")
+
+    def repair_counts(self):
+        """how often each repair branch fired since setup (dict by REPAIRS)"""
+        a = (C.c_long * 8)()
+        self.lib.orc_get_repair_counts(a)
+        return dict(zip(self.REPAIRS, list(a)))
+
+    def repair_cells(self):
+        """cells (x fastest) of the first 256 con2prim repairs since setup"""
+        a = (C.c_long * 256)()
+        f = self.lib.orc_get_repair_cells
+        f.restype = C.c_int
+        m = f(a, 256)
+        return list(a[:m])
 
 
 def oracle_set_stop(oracle, temp_end, freeze_type=0):

@@ -164,3 +164,127 @@ def mhd3d(nx=12, ny=10, nz=8, eqstate=3, algo=1, divclean=1, coordinates=2, B0=0
     v[6] = 0.5 * B0 * (1 + 0.1 * np.cos(np.pi * X))
     v[7] = 0.1 * B0 * (1 + 0.2 * np.cos(2 * np.pi * Z))
     return ic, rc, v
+
+
+# ---------------------------------------------------------------------------
+# States that drive the repair branches of evolve_df{x,y,z} (evolve.f90:
+# 334-369) and evolve_u2v (evolve.f90:795-842)
+# ---------------------------------------------------------------------------
+def rough_patch(nx=12, ny=10, nz=8, eqstate=1, rec="WENO5", seed=1, amp=0.9, pr_lo=-6.0):
+    """fireball3d with a seeded rough patch inside: velocities in random
+    directions up to |v| = amp and pressures spread over 10**pr_lo .. 1 times
+    the smooth value.  Its reconstructions overshoot (p <= 0, v^2 >= 1) and
+    its cons2prim fails in places, so both repairs run."""
+    rng = np.random.default_rng(seed)
+    ic, rc, v = fireball3d(nx, ny, nz, eqstate=eqstate, rec=rec, seed=seed)
+    sl = (slice(2, nz - 2), slice(3, ny - 3), slice(3, nx - 3))
+    shp = v[0][sl].shape
+    d = rng.standard_normal((3,) + shp)
+    d /= np.linalg.norm(d, axis=0)
+    sp = amp * rng.uniform(0.0, 1.0, shp) ** 0.3
+    v[1][sl] = d[0] * sp
+    v[2][sl] = d[1] * sp
+    v[3][sl] = d[2] * sp
+    v[4][sl] = v[4][sl] * 10 ** rng.uniform(pr_lo, 0, shp)
+    return ic, rc, v
+
+
+# interior cells (iz, iy, ix) whose conserved state is made superluminal; the
+# first two are x neighbours, the third is the first's y neighbour, so the
+# later cells' repairs read the earlier cells' repaired values
+FORCED_CELLS = ((4, 5, 5), (4, 5, 6), (4, 6, 5))
+
+
+def forced_background(eqstate=2, viscosity=0, rec="TVD2", pbg=1e-3, coordinates=1, seed=0,
+                      smooth_temp=0.0, nx=10, ny=10, nz=8):
+    """A smooth, cold background (p/rho ~ pbg) for host-forced failures."""
+    rng = np.random.default_rng(seed)
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=coordinates, viscosity=viscosity, bulk=1,
+                         eqstate=eqstate, rec=rec, nrk=3, bc=2, xlim=(-1, 1), ylim=(-1, 1),
+                         zlim=(-1, 1), tmin=1.0, maxdt=1.0, init_type=0, smooth_temp=smooth_temp)
+    nv = 12 if viscosity else 5
+    x = cell_centres(nx, -1, 1)
+    y = cell_centres(ny, -1, 1)
+    z = cell_centres(nz, -1, 1)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    v = np.zeros((nv, nz, ny, nx))
+    v[0] = 1.0 + 0.1 * np.cos(X)
+    v[1] = 0.1 * np.sin(2 * Y)
+    v[2] = 0.1 * np.sin(2 * X)
+    v[3] = 0.05 * np.sin(Z)
+    v[4] = pbg * (1.0 + 0.3 * np.sin(3 * X + Y)) * (1 + 0.05 * rng.standard_normal(X.shape))
+    if viscosity:
+        v[9] = 0.1 * v[4]
+        v[10] = 0.1 * v[4]
+        v[6] = 0.02 * v[4]
+    return ic, rc, v
+
+
+def force_superluminal(u, cells=FORCED_CELLS, factor=3.0):
+    """u with S_x = factor * E at the given cells (|S| > E: no primitive
+    state has these conserved variables, so cons2prim fails there)."""
+    u = u.copy()
+    for (z, y, x) in cells:
+        u[1, z, y, x] = factor * u[4, z, y, x]
+    return u
+
+
+def fast_eta(rec="TVD2", v_eta=0.99):
+    """Bjorken grid with two cells moving at v^eta = v_eta at tau = 1: as tau
+    grows, their own v^2 = tau^2 (v^eta)^2 passes 1, which drives the
+    maxspeed rescaling of the reconstruction repair (evolve.f90:342-346) and,
+    one step later, a failed con2prim repair."""
+    ic, rc, v = forced_background(eqstate=1, rec=rec, pbg=0.05, coordinates=2)
+    for (z, y, x) in ((4, 5, 5), (3, 4, 6)):
+        v[1, z, y, x] = 0.05
+        v[2, z, y, x] = 0.0
+        v[3, z, y, x] = v_eta
+    return ic, rc, v
+
+
+# ---------------------------------------------------------------------------
+# Glauber-MC events (test-side sampler, numpy's seeded generator)
+# ---------------------------------------------------------------------------
+# Glauber-MC deposition parameters (echo_glauber_params)
+GLAUBER_PAR = dict(kappa=19.0, ah=0.15, sig_mc=0.4, ybeam=5.36, deta=3.0, sigeta=0.8, rhocenter=1.0)
+
+
+def woods_saxon_nucleus(rng, A, R, a):
+    """(x, y) of A nucleons: radii from r^2 / (1 + exp((r - R) / a)) by
+    rejection, isotropic directions, centre of mass at the origin
+    (the sampling of glaubermc.f90:251-307 with another generator)."""
+    rmax = R + 10 * a
+    pts = []
+    while len(pts) < A:
+        r = rng.uniform(0, rmax, 8 * A)
+        keep = rng.uniform(0, 1, r.size) * rmax ** 2 < r ** 2 / (1 + np.exp((r - R) / a))
+        pts.extend(r[keep].tolist())
+    r = np.array(pts[:A])
+    cth = rng.uniform(-1, 1, A)
+    phi = rng.uniform(0, 2 * np.pi, A)
+    sth = np.sqrt(1 - cth ** 2)
+    p = np.stack([r * sth * np.cos(phi), r * sth * np.sin(phi)], 1)
+    return p - p.mean(0)
+
+
+def glauber_event(seed=5, A=208, R=6.62, a=0.546, sigma_nn=6.76, b=5.0):
+    """Participants of nuclei A (at +b/2) and B (at -b/2) and the binary
+    collision points (midpoints of colliding pairs, d^2 <= sigma_nn / pi as
+    glaubermc.f90:408-420); defaults: Pb-Pb, sigma_NN = 67.6 mb, b = 5 fm."""
+    rng = np.random.default_rng(seed)
+    na = woods_saxon_nucleus(rng, A, R, a) + [b / 2, 0]
+    nb = woods_saxon_nucleus(rng, A, R, a) - [b / 2, 0]
+    hit = ((na[:, None] - nb[None]) ** 2).sum(-1) <= sigma_nn / np.pi
+    ia, ib = np.nonzero(hit)
+    return na[hit.any(1)], nb[hit.any(0)], 0.5 * (na[ia] + nb[ib])
+
+
+def pbpb_config(nx=384, ny=384, nz=256, zc=0.0, dx=0.08, deta=0.05, eta_over_s=0.12):
+    """The large single-event grid: 384 x 384 x 256, dx = 0.08 fm,
+    deta = 0.05, tau0 = 0.4 fm, viscous hydro, tabulated EoS, WENO5 + RK3,
+    ENEZERO 0.001; nz and the centre zc select an eta slab of it."""
+    return make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=0, eqstate=3,
+                       rec="WENO5", nrk=3, bc=2, obtained_zz=1, init_type=3,
+                       xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
+                       zlim=(zc - nz * deta / 2, zc + nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=0.1,
+                       eta_over_s=eta_over_s, tau_pi_coeff=5.0, enezero=0.001)

@@ -15,6 +15,8 @@ it: after three steps the asymmetry is 4.8e-12 of the scale, so the bound is
 import numpy as np
 import pytest
 
+from scenarios import GLAUBER_PAR, glauber_event, pbpb_config
+
 pytestmark = pytest.mark.gpu
 
 # parity of the 12 viscous components under (x, eta) -> (-x, -eta):
@@ -62,3 +64,86 @@ def test_reflection_symmetry_384x384x256():
     assert np.abs(v[1]).max() > 1e-3 and np.abs(v[3]).max() > 1e-4
     print(f"max reflection asymmetry / scale: {worst:.2e}", flush=True)
     assert worst <= 1e-10
+
+
+# ---------------------------------------------------------------------------
+# The unfloored Glauber-MC state of the large grid (echo_init_glauber,
+# ENEZERO 0.001).  Near the beam-rapidity cut of the deposition (|eta| ->
+# ybeam = 5.36) the energy density falls to the vacuum value and the
+# Navier-Stokes stresses exceed the pressure; there the reference's own
+# scheme (the oracle, which matches it bit for bit on the repair paths)
+# repairs hundreds of cells per step and in the second step a con2prim
+# repair fails, which stops the run (evolve.f90:836-840).
+# ---------------------------------------------------------------------------
+def _glauber_state(ic, rc, pa, pb, pc):
+    from libs import Oracle, oracle_glauber_prim
+    o = Oracle()
+    o.setup(ic, rc)
+    e, v0 = oracle_glauber_prim(o, GLAUBER_PAR, pa, pb, pc)
+    assert e == 0
+    return o, v0
+
+
+def test_glauber_rapidity_tail_slab_vs_oracle():
+    """An eta slab of the large grid around eta = 5.2 (384 x 384 x 8 cells):
+    step 1 with its repairs against the oracle, then the failed repair of
+    step 2 with the same error code, the oracle's crashing cell in the
+    message, and the cells before it (in the reference's loop order) equal."""
+    from echoamd import EchoError, EchoGrid, read_eos_rows
+    from scenarios import rel_linf
+    ic, rc = pbpb_config(nz=8, zc=5.2)
+    pa, pb, pc = glauber_event(seed=5)
+    o, v0 = _glauber_state(ic, rc, pa, pb, pc)
+    assert o.init_from_prim(v0) == 0
+    g = EchoGrid(ic, rc, eos_rows=read_eos_rows())
+    assert g.init_glauber(GLAUBER_PAR, pa, pb, pc) == 0
+    print("\nslab 384x384x8 at eta 5.2: step 1", flush=True)
+    assert o.steps(1) == 0
+    g.steps(1)
+    n1 = o.repair_counts()["u2v"]
+    assert n1 > 0 and g.repairs() == n1
+    vg, ug = g.state()
+    vo, uo = o.state()
+    assert rel_linf(vg, vo) <= 1e-10 and rel_linf(ug, uo) <= 1e-10
+    print("step 2", flush=True)
+    assert o.steps(1) == 1
+    cnt = o.repair_counts()
+    assert cnt["u2v_fail"] == 1
+    crash = o.repair_cells()[cnt["u2v"] - 1] if cnt["u2v"] <= 256 else None
+    with pytest.raises(EchoError) as ei:
+        g.steps(1)
+    assert ei.value.code == 1 and "repair failed" in str(ei.value)
+    print(str(ei.value), flush=True)
+    if crash is not None:
+        nx, ny = int(ic[0]), int(ic[1])
+        assert f"ix={crash % nx + 1} iy={(crash // nx) % ny + 1} iz={crash // (nx * ny) + 1}" in str(ei.value)
+        vg, _ = g.state()
+        vo, _ = o.state()
+        before = np.arange(vg[0].size) < crash
+        a = vg.reshape(12, -1)[:, before].reshape(12, 1, 1, -1)
+        b = vo.reshape(12, -1)[:, before].reshape(12, 1, 1, -1)
+        assert rel_linf(a, b) <= 1e-10
+    g.close()
+
+
+def test_glauber_unfloored_384x384x256():
+    """The whole grid from echo_init_glauber's unfloored state: the first
+    step runs (with repairs), and within three steps the run stops on a
+    failed con2prim repair, reported as such, as the oracle's slab run of
+    the rapidity tail does."""
+    from echoamd import EchoError, EchoGrid, read_eos_rows
+    ic, rc = pbpb_config()
+    pa, pb, pc = glauber_event(seed=5)
+    g = EchoGrid(ic, rc, eos_rows=read_eos_rows())
+    assert g.init_glauber(GLAUBER_PAR, pa, pb, pc) == 0
+    print("\nfull size 384x384x256, unfloored Glauber state: step 1", flush=True)
+    g.steps(1)
+    assert g.scalars()["crashed"] == 0
+    n1 = g.repairs()
+    print(f"repaired cells in step 1: {n1}", flush=True)
+    assert n1 > 0
+    with pytest.raises(EchoError) as ei:
+        g.steps(2)
+    print(str(ei.value), f"repaired cells: {g.repairs()}", flush=True)
+    assert ei.value.code == 1 and "repair failed" in str(ei.value)
+    g.close()

@@ -73,3 +73,35 @@ def test_golden_trajectory(oracle, name):
     s = oracle.scalars()
     assert s["t"] == g["t"] and s["dt"] == g["dt"]
     assert np.array_equal(v, g["v"]) and np.array_equal(u, g["u"])
+
+
+REPAIR_CASES = ["rough-ideal-weno5", "rough-ideal-mpe5", "rough-table-weno5", "forced-idealgas-tvd2",
+                "forced-idealgas-weno5", "forced-viscous-weno5", "fast-eta-crash"]
+
+
+@pytest.mark.parametrize("name", REPAIR_CASES)
+def test_golden_repairs(oracle, name):
+    """the repair branches (evolve.f90:334-369, 795-842) on the states of
+    tests/test_oracle_repair.py, against the reference's results"""
+    g = load("repair.npz")
+    oracle.setup(g[f"{name}/ic"], g[f"{name}/rc"])
+    assert oracle.init_from_prim(g[f"{name}/v0"]) == 0
+    if f"{name}/v_in" in g:
+        oracle.set_state(g[f"{name}/v_in"], g[f"{name}/u_in"])
+    assert oracle.steps(int(g[f"{name}/steps"])) == int(g[f"{name}/err"])
+    cnt = oracle.repair_counts()
+    assert cnt["u2v"] > 0 and cnt["c2p_undef"] == 0
+    v, u = oracle.state()
+    assert np.array_equal(v, g[f"{name}/v"], equal_nan=True)
+    assert np.array_equal(u, g[f"{name}/u"], equal_nan=True)
+
+
+def test_golden_u2v_alone(oracle):
+    g = load("repair.npz")
+    oracle.setup(g["u2v/ic"], g["u2v/rc"])
+    oracle.init_from_prim(g["u2v/v0"])
+    oracle.set_state(g["u2v/v_in"], g["u2v/u_in"])
+    assert oracle.u2v() == int(g["u2v/err"])
+    assert oracle.repair_counts()["u2v_pr"] > 0
+    v, u = oracle.state()
+    assert np.array_equal(v, g["u2v/v"]) and np.array_equal(u, g["u2v/u"])
