@@ -12,11 +12,12 @@
 orc_state G;
 
 static void free_fields(void) {
-  free(G.v); free(G.u); free(G.u0); free(G.du); free(G.du_stiff);
+  free(G.v); free(G.u); free(G.u0); free(G.du_all); free(G.dus_all);
   free(G.deriv); free(G.vnewstep); free(G.vold); free(G.unewstep); free(G.uold);
   free(G.ax); free(G.ay); free(G.az);
   free(G.xc); free(G.yc); free(G.zc);
   G.v = G.u = G.u0 = G.du = G.du_stiff = G.deriv = G.vnewstep = G.vold = G.unewstep = G.uold = NULL;
+  G.du_all = G.dus_all = NULL;
   G.ax = G.ay = G.az = G.xc = G.yc = G.zc = NULL;
   fo_release();
 }
@@ -34,7 +35,13 @@ static void system_setup(void) {
   G.nov = 5; G.nou = 5;
   G.nv = G.viscosity ? 12 : 5;
   G.kbx = G.kby = G.kbz = G.kglm = -1;
-  if (G.mhd) {
+  G.kex = G.key = G.kez = -1;
+  if (G.mhd && G.rmhd) {
+    G.nov += 6; G.nou += 6; G.nv += 6;
+    G.kbx = G.nv - 6; G.kby = G.nv - 5; G.kbz = G.nv - 4;
+    G.kex = G.nv - 3; G.key = G.nv - 2; G.kez = G.nv - 1;
+    if (G.divclean) { G.nov++; G.nou++; G.nv++; G.kglm = G.nv - 1; }
+  } else if (G.mhd) {
     G.nov += 3; G.nou += 3; G.nv += 3;
     G.kbx = G.nv - 3; G.kby = G.nv - 2; G.kbz = G.nv - 1;
     if (G.divclean) { G.nov++; G.nou++; G.nv++; G.kglm = G.nv - 1; }
@@ -77,7 +84,12 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.enable_smooth_viscosity = G.smooth_temp > 0.;
   G.glm_ch_old = 0.5;
   G.aflux[0] = G.aflux[1] = G.aflux[2] = 0.;
-  if (G.rmhd || (G.mhd && G.viscosity)) return -11;   /* as the reference: no such system */
+  /* the resistive system (oracle_rmhd.c) with or without viscosity, with
+   * divergence cleaning; ideal MHD with viscosity does not exist (the
+   * reference refuses viscosity with MHD, common.f90:1904-1908) */
+  if (G.rmhd && !G.divclean) return -11;
+  if (G.mhd && G.viscosity && !G.rmhd) return -11;
+  if (G.nrk == SSP && !G.mhd) return -11;   /* common.f90:2076-2083 */
   system_setup();
   common_grid();
   if (G.eqstate == 3) {
@@ -93,8 +105,11 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.v = calloc(n * G.nv, sizeof(double));
   G.u = calloc(n * G.nv, sizeof(double));
   G.u0 = calloc(n * G.nv, sizeof(double));
-  G.du = calloc(n * G.nv, sizeof(double));
-  G.du_stiff = calloc(n * G.nv, sizeof(double));
+  /* du, du_stiff: one slot per IMEX stage (evolve.f90:49-79), one for RK */
+  G.du_all = calloc(n * G.nv * 4, sizeof(double));
+  G.dus_all = calloc(n * G.nv * 4, sizeof(double));
+  G.du = G.du_all;
+  G.du_stiff = G.dus_all;
   G.deriv = calloc(n * NDERIV, sizeof(double));
   /* common.f90:379-388: vold, uold, vnewstep, unewstep hold krh..kvold_end */
   G.kvold_end = G.mhd ? 8 : 5;
@@ -125,7 +140,7 @@ int orc_init_from_prim(const double *vin) {
       for (int ix = 0; ix < G.nx; ix++) {
         size_t c = cell_index(ix, iy, iz);
         for (int k = 0; k < G.nv; k++) vv[k] = vin[k * n + c];
-        int e = G.viscosity ? sys_prim2cons_0(vv, uu) : sys_prim2cons(vv, uu);
+        int e = G.rmhd ? rmhd_prim2cons(vv, uu, 1) : G.viscosity ? sys_prim2cons_0(vv, uu) : sys_prim2cons(vv, uu);
         if (e > 0) return e;
         if (!G.viscosity) {
           e = sys_cons2prim(uu, vv);
@@ -206,7 +221,14 @@ static int hll_face(double *wl, double *wr, int dir, double *w1) {
   double am = fmax2(fmax2(0., -vfl[1]), -vfr[1]);
   G.aflux[dir - 1] = fmax2(ap, am);
   double a1 = (ap == 0.0 && am == 0.0) ? 1. : 1. / (ap + am);
-  for (int k = 0; k < G.nv; k++) w1[k] = a1 * (ap * fl[k] + am * fr[k] - ap * am * (ur[k] - ul[k]));
+  /* resistive system: B, E and psi travel at the speed of light, whose
+   * coordinate value is 1/sqrt(g_dir); HLL with those speeds for them */
+  const int kem = G.rmhd ? G.kbx : G.nv;
+  for (int k = 0; k < kem; k++) w1[k] = a1 * (ap * fl[k] + am * fr[k] - ap * am * (ur[k] - ul[k]));
+  if (G.rmhd) {
+    const double c = 1. / sqrt(G.g_cov[dir - 1]);
+    for (int k = kem; k < G.nv; k++) w1[k] = 0.5 * (fl[k] + fr[k] - c * (ur[k] - ul[k]));
+  }
   w1[G.nv] = ap;
   w1[G.nv + 1] = am;
   return 0;
@@ -405,6 +427,14 @@ static void evolve_glm_source(double dt_glm) {
 
 /* evolve_dt, evolve.f90:723-764 */
 static double evolve_dt(void) {
+  if (G.rmhd) {
+    /* resistive system: the speed of light in each active direction */
+    double amax = 0.;
+    if (G.nx > 1) amax = fmax2(amax, G.ddx);
+    if (G.ny > 1) amax = fmax2(amax, G.ddy);
+    if (G.nz > 1) amax = fmax2(amax, G.ddz / sqrt(G.g_cov[2]));
+    return G.cfl / amax;
+  }
   if (G.divclean) return G.cfl / 1.;
   double amax = 0.;
   const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -450,7 +480,9 @@ static int evolve_u2v(void) {
       for (int ix = 0; ix < G.nx; ix++) {
         size_t c = cell_index(ix, iy, iz);
         for (int k = 0; k < G.nv; k++) { vv[k] = G.v[k * nc + c]; uu[k] = G.u[k * nc + c]; }
-        int e = sys_cons2prim(uu, vv);
+        /* inside an IMEX stage of the resistive system the inversion also
+         * solves the stiff relation for E (and changes u there) */
+        int e = (G.rmhd && G.imex_adt > 0.) ? rmhd_cons2prim_implicit(uu, vv, G.imex_adt) : sys_cons2prim(uu, vv);
         if (e != 0) {
           if (G.nrep[REP_U2V] < REPLOG) G.replog[G.nrep[REP_U2V]] = (long)c;
           G.nrep[REP_U2V]++;
@@ -549,7 +581,92 @@ int find_pressure_from_temperature(double temp_input, double *pres) {
 }
 
 /* evolve_main for explicit RK2/RK3, evolve.f90:114-203 + 276-295 */
+/* the fluxes and sources of one stage into du / du_stiff slot irk */
+static int ssp_rhs(int irk) {
+  const size_t n = ncell() * G.nu;
+  G.du = G.du_all + (size_t)(irk - 1) * n;
+  G.du_stiff = G.dus_all + (size_t)(irk - 1) * n;
+  memset(G.du, 0, n * sizeof(double));
+  memset(G.du_stiff, 0, n * sizeof(double));
+  G.glm_ch = G.glm_ch_old;
+  if (G.nx > 1 && sweep(1)) return G.errcode;
+  if (G.ny > 1 && sweep(2)) return G.errcode;
+  if (G.nz > 1 && sweep(3)) return G.errcode;
+  if (evolve_src()) { G.run_crashed = 1; return G.errcode; }
+  G.glm_ch_old = fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
+  return 0;
+}
+
+/* evolve_main for the IMEX-SSP3(4,3,3) scheme (nrk = SSP),
+ * evolve.f90:127-130, 204-264: explicit stages from u0 and the stored
+ * du of stages 2-4, stiff parts du_stiff of stages 1-4.  For ideal MHD the
+ * stiff part is zero and the stages are the reference's; for the resistive
+ * system every stage solves its implicit part inside evolve_u2v, the first
+ * one included (the reference evaluates the stage-1 stiff part at u^n).
+ * After the last stage the final combination and evolve_u2v. */
+static int evolve_main_ssp(int irk) {
+  const double alpha = 0.24169426078821, beta = 0.06042356519705, eta = 0.12915286960590;
+  const double delta = 0.5 - beta - eta - alpha;
+  const size_t n = ncell() * G.nu;
+  double *du1 = G.du_all, *du2 = G.du_all + n, *du3 = G.du_all + 2 * n, *du4 = G.du_all + 3 * n;
+  double *ds1 = G.dus_all, *ds2 = G.dus_all + n, *ds3 = G.dus_all + 2 * n, *ds4 = G.dus_all + 3 * n;
+  (void)du1;
+  const double dt = G.dt;
+  if (irk == 1) {
+    double dt_local = evolve_dt();
+    dt_local = fmin2(dt_local, G.maxdt);
+    dt_local = fmin2(dt_local, G.tout - G.t);
+    if (fabs(G.t + G.dt - G.tout) < 1.e-9) dt_local = G.tout - G.t;
+    else if ((G.t + 2. * dt_local - G.tout) > 1.e-9) dt_local = (G.tout - G.t) / 2.;
+    G.dt = dt_local;
+    memcpy(G.u0, G.u, n * sizeof(double));
+    if (G.rmhd) {
+      /* the implicit first stage K1 = u^n + alpha dt R(K1) of the scheme,
+       * which the reference replaces by K1 = u^n (no inversion at stage 1);
+       * with a stiff current that shortcut is not stable */
+      G.imex_adt = alpha * G.dt;
+      int e = evolve_u2v();
+      G.imex_adt = 0.;
+      if (e) return e;
+    }
+    if (ssp_rhs(1)) return G.errcode;
+    if (G.divclean) evolve_glm_source(G.dt / 2.);
+  } else if (irk == 2) {
+    for (size_t i = 0; i < n; i++) G.u[i] = G.u0[i] + dt * alpha * ds1[i];
+    G.imex_adt = alpha * dt;
+    int e = evolve_u2v();
+    G.imex_adt = 0.;
+    if (e) return e;
+    if (ssp_rhs(2)) return G.errcode;
+  } else if (irk == 3) {
+    for (size_t i = 0; i < n; i++) G.u[i] = G.u0[i] - dt * (du2[i] + (1. - alpha) * ds2[i]);
+    G.imex_adt = alpha * dt;
+    int e = evolve_u2v();
+    G.imex_adt = 0.;
+    if (e) return e;
+    if (ssp_rhs(3)) return G.errcode;
+  } else {
+    for (size_t i = 0; i < n; i++)
+      G.u[i] = G.u0[i] - dt * (du2[i] / 4. + du3[i] / 4. + beta * ds1[i] + eta * ds2[i] + delta * ds3[i]);
+    G.imex_adt = alpha * dt;
+    int e = evolve_u2v();
+    G.imex_adt = 0.;
+    if (e) return e;
+    if (ssp_rhs(4)) return G.errcode;
+    for (size_t i = 0; i < n; i++)
+      G.u[i] = G.u0[i] - dt * (du2[i] / 6. + du3[i] / 6. + du4[i] * 2. / 3. + ds2[i] / 6. + ds3[i] / 6. +
+                               ds4[i] * 2. / 3.);
+    if (G.divclean) evolve_glm_source(G.dt / 2.);
+    if (evolve_u2v()) return G.errcode;
+  }
+  if (G.viscosity && G.enable_smooth_viscosity) {
+    if (evolve_smooth_viscosity()) return G.errcode;
+  }
+  return 0;
+}
+
 static int evolve_main(int irk) {
+  if (G.nrk == SSP) return evolve_main_ssp(irk);
   static const double crk[2][3][3] = {
     /* [c][irk][nrk]: crk(c,irk,nrk) of evolve.f90:122-125 */
     {{0., 0., 0.}, {0., 1. / 2., 3. / 4.}, {0., 0., 1. / 3.}},
@@ -557,7 +674,10 @@ static int evolve_main(int irk) {
   const int nrk = G.nrk;
   const double c1 = crk[0][irk - 1][nrk - 1], c2 = crk[1][irk - 1][nrk - 1];
   const size_t n = ncell() * G.nu;
+  G.du = G.du_all;
+  G.du_stiff = G.dus_all;
   memset(G.du, 0, n * sizeof(double));
+  memset(G.du_stiff, 0, n * sizeof(double));
   G.glm_ch = G.glm_ch_old;
   if (G.nx > 1 && sweep(1)) return G.errcode;
   if (G.ny > 1 && sweep(2)) return G.errcode;
@@ -574,7 +694,12 @@ static int evolve_main(int irk) {
     if (G.divclean) evolve_glm_source(G.dt / 2.);
     memcpy(G.u0, G.u, n * sizeof(double));
   }
-  for (size_t i = 0; i < n; i++) G.u[i] = c1 * G.u0[i] + c2 * (G.u[i] - G.dt * G.du[i]);
+  if (G.rmhd) {
+    /* resistive system with an explicit RK: the stiff current explicitly */
+    for (size_t i = 0; i < n; i++) G.u[i] = c1 * G.u0[i] + c2 * (G.u[i] - G.dt * (G.du[i] + G.du_stiff[i]));
+  } else {
+    for (size_t i = 0; i < n; i++) G.u[i] = c1 * G.u0[i] + c2 * (G.u[i] - G.dt * G.du[i]);
+  }
   if (G.divclean && irk == nrk) evolve_glm_source(G.dt / 2.);
   if (evolve_u2v()) return G.errcode;
   if (G.viscosity && G.enable_smooth_viscosity) {

@@ -49,6 +49,8 @@ typedef struct {
   double temp_end, check_max; int freeze_type, temp_check;
   /* fields */
   double *v, *u, *u0, *du, *du_stiff, *deriv, *vnewstep, *vold, *unewstep, *uold;
+  double *du_all, *dus_all;   /* 4 slots each: du, du_stiff of the IMEX stages */
+  double imex_adt;            /* alpha dt of the implicit solve in evolve_u2v, 0: none */
   int kvold_end;   /* components krh..kvold_end of vold/uold: 5 (hydro), 8 (MHD) */
   double *ax, *ay, *az;
   /* how often each repair branch fired since orc_setup (REP_* below) */
@@ -96,6 +98,13 @@ int sys_prim2cons_0(double *v, double *u);
 int sys_cons2prim(double *u, double *v);
 int sys_flux(double *v, double *u, double *f, double *vf, int dir);
 int sys_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff);
+/* resistive MHD with or without viscosity (oracle_rmhd.c) */
+int rmhd_nfluid(void);
+int rmhd_prim2cons(double *v, double *u, int initial);
+int rmhd_cons2prim(double *u, double *v);
+int rmhd_cons2prim_implicit(double *u, double *v, double adt);
+int rmhd_flux(double *v, double *u, double *f, double *vf, int dir);
+int rmhd_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff);
 void get_derived_pi(const double *p, double *pitt, double *pitx, double *pity, double *pitz);
 void get_derived_pi_zz(const double *p, double *pitt, double *pitx, double *pity, double *pitz, double *pizz);
 void system_metric(void);

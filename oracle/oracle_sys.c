@@ -81,6 +81,7 @@ static void system_EXB(const double *E, const double *B, double *X) {
 
 /* system_prim2cons, system.f90:164-263 (HD, viscous and ideal-MHD branches) */
 int sys_prim2cons(double *v, double *u) {
+  if (G.rmhd) return rmhd_prim2cons(v, u, 0);
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
   int err = eos_energy(rh, pr, &en);
   if (err > 0) return err;
@@ -614,6 +615,7 @@ static int mhd_cons2prim(double *u, double *v) {
 /* system_cons2prim, system.f90:423-665: ideal HD Newton on p (511-562) and
  * viscous nested iteration (564-665) */
 int sys_cons2prim(double *u, double *v) {
+  if (G.rmhd) return rmhd_cons2prim(u, v);
   const double tol = 1.e-12, eps = 1.e-12;
   const int iter_max = 100, ivmax = 100, ipmax = 100;
   const double *g = G.g_cov;
@@ -721,6 +723,7 @@ int sys_cons2prim(double *u, double *v) {
 
 /* system_flux, system.f90:1316-1583 (HD and viscous branches) */
 int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
+  if (G.rmhd) return rmhd_flux(v, u, f, vf, dir);
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
   int err = eos_energy(rh, pr, &en);
   if (err > 0) return err;
@@ -927,6 +930,7 @@ static void get_sigma(int ix, int iy, int iz, double *sigma, double *dd) {
 
 /* system_source, system.f90:1587-1842 (HD and viscous branches) */
 int sys_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff) {
+  if (G.rmhd) return rmhd_source(ix, iy, iz, v, src, src_stiff);
   const double lambda0 = 1., lambda1 = 1., lambda2 = 0.;
   for (int k = 0; k < G.nv; k++) { src[k] = 0.; src_stiff[k] = 0.; }
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;

@@ -288,3 +288,78 @@ def pbpb_config(nx=384, ny=384, nz=256, zc=0.0, dx=0.08, deta=0.05, eta_over_s=0
                        xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
                        zlim=(zc - nz * deta / 2, zc + nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=0.1,
                        eta_over_s=eta_over_s, tau_pi_coeff=5.0, enezero=0.001)
+
+
+# ---------------------------------------------------------------------------
+# Resistive MHD (mhd = 2, with the Dedner scalar), with and without viscosity
+# ---------------------------------------------------------------------------
+def resistive_layout(viscosity):
+    """(index of B^x, index of E^x, index of psi, nv)"""
+    nf = 12 if viscosity else 5
+    return nf, nf + 3, nf + 6, nf + 7
+
+
+def rmhd_fireball(nx=16, ny=14, nz=10, viscosity=1, eqstate=1, nrk=4, B0=0.3, E0=0.05, coordinates=2,
+                  sigma_el=0.0058, sigma_chi=0.0015, rec="WENO5", maxdt=1.0, bc=2, seed=0):
+    """fireball3d with a transverse magnetic field and an electric field
+    (not the ideal -v x B, so that the current flows), for the resistive
+    system; viscous runs carry the fireball's shear tensor."""
+    ic, rc, vf = fireball3d(nx, ny, nz, viscosity=viscosity, eqstate=eqstate, rec=rec, nrk=nrk,
+                            coordinates=coordinates, bc=bc, seed=seed)
+    ic[6] = 2
+    ic[7] = 1
+    ic[12] = 7
+    rc[8] = maxdt
+    rc[14] = sigma_el
+    rc[15] = sigma_chi
+    kb, ke, kg, nv = resistive_layout(viscosity)
+    v = np.zeros((nv,) + vf.shape[1:])
+    v[:kb] = vf
+    x = cell_centres(nx, -3, 3)
+    y = cell_centres(ny, -3, 3)
+    z = cell_centres(nz, -1, 1)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    v[kb + 1] = B0 * np.exp(-(X ** 2 + Y ** 2) / 8)
+    v[kb] = 0.2 * B0 * np.sin(Y)
+    v[kb + 2] = 0.05 * B0
+    v[ke] = E0 * np.exp(-(X ** 2 + Y ** 2) / 6) * np.cos(0.5 * Y)
+    v[ke + 1] = 0.5 * E0 * np.sin(0.5 * X)
+    v[ke + 2] = 0.1 * E0 * np.exp(-(X ** 2 + Y ** 2) / 8)
+    return ic, rc, v
+
+
+def uniform_resistive(E0=(0.3, -0.2, 0.1), B0=(0.0, 0.0, 0.0), p=1.0, sigma_el=0.0058, nrk=4, n=4,
+                      maxdt=0.01, coordinates=1, viscosity=0):
+    """A uniform fluid at rest with uniform E and B on a small periodic grid:
+    no gradients, so only the local current acts (Minkowski: dE/dt =
+    -sigma (E + v x B ...) with v = 0, i.e. E(t) = E0 exp(-sigma t) for
+    B = 0, the energy moving to the fluid)."""
+    ic, rc = make_config(nx=n, ny=n, nz=n, coordinates=coordinates, viscosity=viscosity, mhd=2, divclean=1,
+                         eqstate=1, algo_solver=7, nrk=nrk, bc=0, init_type=0,
+                         xlim=(-1.0, 1.0), ylim=(-1.0, 1.0), zlim=(-1.0, 1.0), tmin=1.0,
+                         cfl=0.4, maxdt=maxdt, sigma_el=sigma_el, sigma_chi=0.0)
+    kb, ke, kg, nv = resistive_layout(viscosity)
+    v = np.zeros((nv, n, n, n))
+    v[0] = 1.0
+    v[4] = p
+    for k in range(3):
+        v[ke + k] = E0[k]
+        v[kb + k] = B0[k]
+    return ic, rc, v
+
+
+def light_wave(n=64, amp=1e-3, nrk=3, rec="WENO5", sigma_el=0.0):
+    """A small electromagnetic plane wave (E^y = B^eta = amp sin(2 pi x)) in a
+    dense uniform fluid at rest, Minkowski, periodic in x: with sigma = 0 it
+    travels in +x at the speed of light."""
+    ic, rc = make_config(nx=n, ny=1, nz=1, coordinates=1, mhd=2, divclean=1, eqstate=1, algo_solver=7,
+                         rec=rec, nrk=nrk, bc=0, init_type=0, xlim=(-0.5, 0.5), ylim=(-0.5, 0.5),
+                         zlim=(-0.5, 0.5), tmin=0.0, cfl=0.4, maxdt=1.0, sigma_el=sigma_el, sigma_chi=0.0)
+    kb, ke, kg, nv = resistive_layout(0)
+    x = cell_centres(n, -0.5, 0.5)
+    v = np.zeros((nv, 1, 1, n))
+    v[0] = 1.0
+    v[4] = 100.0
+    v[ke + 1, 0, 0] = amp * np.sin(2 * np.pi * x)
+    v[kb + 2, 0, 0] = amp * np.sin(2 * np.pi * x)
+    return ic, rc, v

@@ -144,3 +144,34 @@ def test_restart_with_freezeout(tmp_path):
     eb = h.fo_elements()
     assert na == nb > 0 and np.array_equal(ea, eb)
     h.close()
+
+
+@pytest.mark.parametrize("how", ["timed", "stages"])
+def test_dump_after_timed_and_single_stage_steps(oracle, tmp_path, how):
+    """vold / uold are tracked however the steps are taken: with kernel
+    timing on (plain launches instead of the captured graph) and through
+    echo_rk_stage / echo_end_step."""
+    ic, rc, v0 = fireball3d(24, 20, 16, viscosity=1, eqstate=3)
+    g = _grid(ic, rc)
+    g.init_from_prim(v0)
+    g.set_dump(True)
+    if how == "timed":
+        g.set_timing(True)
+        g.steps(3)
+    else:
+        for _ in range(3):
+            for irk in (1, 2, 3):
+                g.rk_stage(irk)
+            g.end_step()
+    gp = tmp_path / "gpu_dump.dat"
+    g.dump(gp, 6)
+    oracle.setup(ic, rc)
+    oracle.init_from_prim(v0)
+    oracle.steps(3)
+    op = tmp_path / "orc_dump.dat"
+    assert oracle_dump(oracle, op, 6) == 0
+    n = int(ic[0] * ic[1] * ic[2])
+    a, b = _parse(gp, g.nv, n, 5), _parse(op, g.nv, n, 5)
+    for k, m in (("v", g.nv), ("u", g.nv), ("vold", 5), ("uold", 5)):
+        assert rel_linf(a[k].reshape(m, -1), b[k].reshape(m, -1)) <= TOL, k
+    g.close()
