@@ -101,11 +101,68 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   // once when there are none); it writes the next step's vnewstep too,
   // before k_end_step flips the parity
   k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, Uout, g->fix, g->VS[0],
-                                                    g->VS[1], last ? 1 : 0);
+                                                    g->VS[1], last ? 1 : 0, 0, 1);
+}
+
+// one stage of IMEX-SSP3(4,3,3) for the resistive systems (kernels.hpp:
+// k_imex_k1, k_imex_k2, k_update_imex).  The launches are shifted against
+// the scheme's stages: stage 1 sets dt and solves the implicit parts of K1
+// and K2 (no sweep: the explicit rate of K1 has weight zero); stages 2 and 3
+// sweep K2 (K3), form their rates and solve K3 (K4); stage 4 sweeps K4 and
+// forms the step's result.  Each launch that inverts is followed by the
+// repair of the cells it could not invert.
+template <int SYS, bool ZZ>
+static void enqueue_imex_stage_t(echo_grid* g, int irk) {
+  if constexpr (SysT<SYS>::RES) {
+    const unsigned nbu = nblocks_xcd(g->N, UPDATE_BLOCK);
+    double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
+    double* Ust = g->U[1];
+    auto fixup = [&](double* U, int last, int imex) {
+      k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, U, g->fix, g->VS[0], g->VS[1], last, imex,
+                                                        last);
+    };
+    if (g->timing) hipEventRecord(g->ev[0], g->stream);
+    if (irk == 1) {
+      if (g->timing) for (int i = 1; i < 4; i++) hipEventRecord(g->ev[i], g->stream);
+      k_cfl<<<1, 1024, 0, g->stream>>>(g->ga, g->partial, g->nb[0], g->nb[1], g->nb[2]);
+      if (g->timing) hipEventRecord(g->ev[4], g->stream);
+      k_imex_k1<SYS, ZZ><<<nbu, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], Ust, g->fix);
+      fixup(Ust, 0, 1);
+      k_imex_k2<SYS, ZZ><<<nbu, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], Ust, g->imex, g->fix);
+      if (g->timing) hipEventRecord(g->ev[5], g->stream);
+      fixup(Ust, 0, 1);
+      return;
+    }
+    if (g->active[0]) launch_flux_dir<1, SYS, ZZ>(g, part[0]);
+    if (g->timing) hipEventRecord(g->ev[1], g->stream);
+    if (g->active[1]) launch_flux_dir<2, SYS, ZZ>(g, part[1]);
+    if (g->timing) hipEventRecord(g->ev[2], g->stream);
+    if (g->active[2]) launch_flux_dir<3, SYS, ZZ>(g, part[2]);
+    if (g->timing) {
+      hipEventRecord(g->ev[3], g->stream);
+      hipEventRecord(g->ev[4], g->stream);
+    }
+#define ECHO_IMEX_UPDATE(ST)                                                                        \
+  k_update_imex<SYS, ZZ, ST><<<nbu, UPDATE_BLOCK, 0, g->stream>>>(                                \
+      g->ga, g->V, g->U[0], Ust, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2], g->VS[0], \
+      g->VS[1], g->imex, g->fix)
+    if (irk == 2) ECHO_IMEX_UPDATE(2);
+    else if (irk == 3) ECHO_IMEX_UPDATE(3);
+    else ECHO_IMEX_UPDATE(4);
+#undef ECHO_IMEX_UPDATE
+    if (g->timing) hipEventRecord(g->ev[5], g->stream);
+    if (irk == 4) fixup(g->U[0], 1, 0);
+    else fixup(Ust, 0, 1);
+  }
 }
 
 static void enqueue_stage(echo_grid* g, int irk) {
-  dispatch_sys(g->cfg, [&](auto sys, auto zz) { enqueue_stage_t<decltype(sys)::value, decltype(zz)::value>(g, irk); });
+  dispatch_sys(g->cfg, [&](auto sys, auto zz) {
+    constexpr int S = decltype(sys)::value;
+    constexpr bool Z = decltype(zz)::value;
+    if (g->cfg.nrk == 4) enqueue_imex_stage_t<S, Z>(g, irk);
+    else enqueue_stage_t<S, Z>(g, irk);
+  });
 }
 
 // one flux sweep (direction d = 0, 1, 2) with the current tile shape
@@ -216,6 +273,7 @@ static void release(echo_grid* g) {
     hipFree(g->partial);
   }
   hipFree(g->VS[0]); hipFree(g->VS[1]); hipFree(g->tab); hipFree(g->tabix);
+  hipFree(g->imex.P4); hipFree(g->imex.ACC);
   if (g->owns_scratch) {
     hipFree(g->fix.Pold); hipFree(g->fix.Upre); hipFree(g->fix.failbits);
     hipFree(g->fix.list); hipFree(g->fix.level);
@@ -238,6 +296,10 @@ const char* echo_last_error(void) { return echo::g_err.c_str(); }
 // words of the failure bitmap of a grid of n cells
 static long long fix_words(long long n) { return (n + 31) / 32; }
 
+// components per face of a flux buffer: nv, and for resistive MHD the
+// difference of the normal E across the face (for div E)
+static int face_slots(const echo_config& c, int nv) { return c.mhd == 2 ? nv + 1 : nv; }
+
 // Set-up of one grid.  stream: launch stream to use (not owned) or null for
 // a stream of its own; share_scratch: leave F, L and partial unallocated
 // (an ensemble points them at its shared pool).
@@ -247,10 +309,15 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   *out = nullptr;
   const echo_config& c = *cfg;
   if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(ECHO_E_ARG, "grid sizes must be >= 1");
-  if (c.mhd == 2) return fail(ECHO_E_ARG, "resistive MHD (MHD=2) is not supported by this build");
-  if (c.mhd != 0 && c.mhd != 1) return fail(ECHO_E_ARG, "MHD must be 0 or 1");
-  if (c.mhd && c.viscosity) return fail(ECHO_E_ARG, "viscosity together with MHD is not supported");
-  if (c.mhd && !((c.algo_solver == 4 && c.eqstate == 2) || (c.algo_solver == 5 && c.eqstate == 1) ||
+  if (c.mhd != 0 && c.mhd != 1 && c.mhd != 2) return fail(ECHO_E_ARG, "MHD must be 0, 1 or 2");
+  if (c.mhd == 1 && c.viscosity)
+    return fail(ECHO_E_ARG, "viscosity with ideal MHD is not supported (the resistive system, MHD=2, "
+                            "takes viscosity)");
+  if (c.mhd == 2 && !c.divclean)
+    return fail(ECHO_E_ARG, "resistive MHD (MHD=2) needs divergence cleaning (DIVCLEAN=1)");
+  if (c.mhd == 2 && (c.sigma_el < 0. || c.sigma_chi < 0.))
+    return fail(ECHO_E_ARG, "SIGMA_EL and SIGMA_CHI must be >= 0");
+  if (c.mhd == 1 && !((c.algo_solver == 4 && c.eqstate == 2) || (c.algo_solver == 5 && c.eqstate == 1) ||
                  (c.algo_solver == 6 && c.eqstate == 1 && c.coordinates == 1) ||
                  ((c.algo_solver == 1 || c.algo_solver == 2) && (c.eqstate == 1 || c.eqstate == 3))))
     return fail(ECHO_E_ARG, "MHD needs ALG_SOLV 1 or 2 with EQ_STATE 1 or 3, 4 with EQ_STATE 2, "
@@ -259,7 +326,11 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   if (c.derivatives_out)
     return fail(ECHO_E_ARG, "derivativ = 1: the derivative output of out_derivatives (out.f90:860-1250) "
                             "is not built");
-  if (c.nrk != 2 && c.nrk != 3) return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2) or RK3 (3)");
+  if (c.nrk == 4 && c.mhd != 2)
+    return fail(ECHO_E_ARG, "INT_ALGO 4 (IMEX-SSP3(4,3,3)) is built for resistive MHD (MHD=2); "
+                            "use RK2 (2) or RK3 (3)");
+  if (c.nrk != 2 && c.nrk != 3 && c.nrk != 4)
+    return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2), RK3 (3) or IMEX-SSP3(4,3,3) (4)");
   if (c.boundary_cond != 0 && c.boundary_cond != 2)
     return fail(ECHO_E_ARG, "boundary_cond must be 0 (periodic) or 2 (outflow)");
   if (c.recal < 1 || c.recal > 7) return fail(ECHO_E_ARG, "unsupported reconstruction code %d", c.recal);
@@ -274,7 +345,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   g->dev = device;
   hipError_t he = hipSetDevice(device);
   if (he != hipSuccess) { delete g; return fail(ECHO_E_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
-  g->nv = c.mhd ? (c.divclean ? 9 : 8) : (c.viscosity ? 12 : 5);
+  g->nv = c.mhd == 2 ? (c.viscosity ? 19 : 12) : c.mhd ? (c.divclean ? 9 : 8) : (c.viscosity ? 12 : 5);
   g->N = (long long)c.nx * c.ny * c.nz;
   g->nf[0] = (long long)(c.nx + 1) * c.ny * c.nz;
   g->nf[1] = (long long)c.nx * (c.ny + 1) * c.nz;
@@ -302,7 +373,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   ALLOC(g->U[1], szs);
   for (int d = 0; d < 3 && !share_scratch; d++) {
     if (!g->active[d]) continue;
-    ALLOC(g->F[d], sizeof(double) * (size_t)g->nv * g->nf[d]);
+    ALLOC(g->F[d], sizeof(double) * (size_t)face_slots(c, g->nv) * g->nf[d]);
     if (c.viscosity) ALLOC(g->L[d], sizeof(double) * 4 * (size_t)g->nf[d]);
   }
   if (c.viscosity) {
@@ -311,6 +382,10 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   }
   if (!share_scratch) ALLOC(g->partial, sizeof(double) * (size_t)nb_cap);
   ALLOC(g->S, sizeof(DevScalars));
+  if (c.nrk == 4) {
+    ALLOC(g->imex.P4, szs);
+    ALLOC(g->imex.ACC, szs);
+  }
   g->fix_upre_cap = (c.viscosity && c.smooth_temp > 0.) ? (long long)g->nv * g->N : 0;
   if (!share_scratch) {
     ALLOC(g->fix.Pold, sizeof(double) * (size_t)g->N);
@@ -395,8 +470,9 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
     DevScalars S0{};
     S0.t = c.tmin; S0.timeold = c.tmin; S0.timeinterval = 0.; S0.g_cov_3_old = 0.;
     S0.dt = c.maxdt; S0.tout = c.tmin + c.dtout; S0.dtout = c.dtout; S0.tmax = c.tmax;
-    S0.glm_ch_old = 0.5;
-    S0.glm_ch = 0.5;
+    // resistive systems clean at the speed of light
+    S0.glm_ch_old = c.mhd == 2 ? 1. : 0.5;
+    S0.glm_ch = S0.glm_ch_old;
     S0.crash_cell = 0x7fffffff;
     S0.smooth_pressure = (c.viscosity && c.smooth_temp > 0.) ? pressure_from_temperature(Eh, c.smooth_temp) : 0.;
     if (hipMemcpy(g->S, &S0, sizeof S0, hipMemcpyHostToDevice) != hipSuccess) {
@@ -425,6 +501,10 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   a.algo_solver = c.algo_solver;
   a.divclean = c.mhd && c.divclean;
   a.glm_alpha = c.glm_alpha;
+  // conductivities in fm^-1 (hbar c = 0.197326 GeV fm, common.f90:51)
+  a.res = c.mhd == 2;
+  a.sig = c.sigma_el / 0.197326;
+  a.sch = c.sigma_chi / 0.197326;
   a.S = g->S;
   if (!share_scratch && hipMemset(g->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(g->N)) != hipSuccess) {
     release(g);
@@ -527,7 +607,9 @@ static int glauber_primitives(echo_grid* g, const echo_glauber_params* p, const 
   const int ns = n_a + n_b + n_coll;
   if (ns < 1 || ns > GLAUBER_MAX_SRC) return fail(ECHO_E_ARG, "1..%d sources supported, got %d", GLAUBER_MAX_SRC, ns);
   if (!(p->sig_mc > 0.) || !(p->ybeam > 0.)) return fail(ECHO_E_ARG, "need sig_mc > 0 and ybeam > 0");
-  if (g->cfg.mhd && g->cfg.viscosity) return fail(ECHO_E_ARG, "unsupported system");
+  if (g->cfg.mhd == 2)
+    return fail(ECHO_E_ARG, "the Glauber initialisation is built for hydro, viscous hydro and ideal MHD "
+                            "(resistive grids start from echo_init_from_prim)");
   HIPCHECK(hipSetDevice(g->dev));
   // centre of mass of the sources, in the reference's summation order
   // (glaubermc.f90:806-821)
@@ -589,6 +671,7 @@ int echo_init_glauber_mhd(echo_grid* g, const echo_glauber_params* p, const doub
                           const echo_bfield_params* bp, const echo_nucleons* lists) {
   if (!g || !p || !bp || !lists) return fail(ECHO_E_ARG, "null argument");
   if (!g->cfg.mhd) return fail(ECHO_E_ARG, "the grid has no magnetic field (mhd = 0)");
+  if (g->cfg.mhd == 2) return fail(ECHO_E_ARG, "the Glauber initialisation is built for ideal MHD (mhd = 1)");
   if (!(bp->gamma_col > 1.)) return fail(ECHO_E_ARG, "need gamma_col > 1");
   if (bp->magfield_type < 1 || bp->magfield_type > 3) return fail(ECHO_E_ARG, "magfield_type must be 1, 2 or 3");
   // each list up to its first neutron, as glaubermc.f90:1026-1058 reads it
@@ -822,6 +905,8 @@ int echo_set_stop(echo_grid* g, double temp_end, int32_t freeze_type) {
 int echo_set_dump(echo_grid* g, int32_t on) {
   if (!g) return fail(ECHO_E_ARG, "null argument");
   if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_dump inside a step");
+  if (on && g->cfg.mhd == 2)
+    return fail(ECHO_E_ARG, "the restart dump is built for hydro, viscous hydro and ideal MHD");
   HIPCHECK(hipSetDevice(g->dev));
   if (on && !g->dump_buf[0]) {
     const size_t sz = sizeof(double) * 2 * dump_kend(g) * (size_t)g->N;
@@ -1034,7 +1119,7 @@ int echo_ensemble_create(const echo_config* cfgs, int32_t n_events, const double
     e->g.push_back(g);
     for (int d = 0; d < 3; d++) {
       if (!g->active[d]) continue;
-      szF[d] = std::max(szF[d], (size_t)((long long)g->nv * g->nf[d]));
+      szF[d] = std::max(szF[d], (size_t)((long long)face_slots(g->cfg, g->nv) * g->nf[d]));
       if (g->cfg.viscosity) szL[d] = std::max(szL[d], (size_t)(4LL * g->nf[d]));
     }
     szP = std::max(szP, g->partial_cap);
@@ -1156,6 +1241,7 @@ int echo_write_frame(echo_grid* g, const int32_t* sel, int32_t precision, int32_
   if (precision != 4 && precision != 8) return fail(ECHO_E_ARG, "precision must be 4 or 8");
   if (iout < 0 || iout > 9999) return fail(ECHO_E_ARG, "iout must be 0..9999");
   if (sel[27]) return fail(ECHO_E_ARG, "the comoving density (rc) is not written");
+  if (g->cfg.mhd == 2) return fail(ECHO_E_ARG, "output frames are built for hydro, viscous hydro and ideal MHD");
   const echo_config& c = g->cfg;
   const bool visc = c.viscosity != 0, mhd = c.mhd != 0;
   const bool anypit = sel[9] || sel[10] || sel[11] || sel[12];

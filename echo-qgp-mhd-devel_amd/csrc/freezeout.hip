@@ -330,6 +330,8 @@ int echo_fo_width(const echo_grid* g) {
 
 int echo_fo_setup(echo_grid* g, int32_t freeze_type, double freeze_value, double* hdr) {
   if (!g) return fail(ECHO_E_ARG, "echo_fo_setup: null grid");
+  if (g->cfg.mhd == 2)
+    return fail(ECHO_E_ARG, "echo_fo_setup: the hypersurface is built for hydro, viscous hydro and ideal MHD");
   if (freeze_type != 0 && freeze_type != 1)
     return fail(ECHO_E_ARG, "freeze_type must be 0 (temperature) or 1 (energy density), got %d", freeze_type);
   if (!isfinite(freeze_value)) return fail(ECHO_E_ARG, "freeze_value must be finite");

@@ -51,6 +51,7 @@ struct echo_grid {
   double* F[3] = {nullptr, nullptr, nullptr};
   double* L[3] = {nullptr, nullptr, nullptr};
   double* VS[2] = {nullptr, nullptr};
+  echo::ImexBufs imex{};        // IMEX-SSP3(4,3,3) accumulators (INT_ALGO 4)
   double* partial = nullptr;
   double* tab = nullptr;
   int* tabix = nullptr;
@@ -115,10 +116,14 @@ inline unsigned nblocks_xcd(long long n, int bs) {
 }
 
 // the system of echo_config: hydro, viscous hydro ('no'/'zz'), ideal MHD
-// without / with GLM
+// without / with GLM, resistive MHD without / with viscosity
 template <typename Fn>
 inline void dispatch_sys(const echo_config& c, Fn&& fn) {
-  if (c.mhd) {
+  if (c.mhd == 2) {
+    if (!c.viscosity) fn(std::integral_constant<int, SYS_RMHD>(), std::false_type());
+    else if (c.obtained_zz) fn(std::integral_constant<int, SYS_VRMHD>(), std::true_type());
+    else fn(std::integral_constant<int, SYS_VRMHD>(), std::false_type());
+  } else if (c.mhd) {
     if (c.divclean) fn(std::integral_constant<int, SYS_MHDGLM>(), std::false_type());
     else fn(std::integral_constant<int, SYS_MHD>(), std::false_type());
   } else if (c.viscosity) {

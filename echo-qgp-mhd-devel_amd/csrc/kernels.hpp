@@ -61,7 +61,17 @@ struct GridArgs {
   int algo_solver;      // MHD con2prim algorithm (4, 5 or 6)
   int divclean;         // Dedner GLM on
   double glm_alpha;     // GLM damping parameter
+  int res;              // resistive MHD (E evolved): light-speed CFL, glm_ch = 1
+  double sig, sch;      // electric and chiral conductivity, fm^-1
   DevScalars* S;
+};
+
+// IMEX-SSP3(4,3,3) scratch (resistive systems): the explicit start of the
+// fourth stage as it accumulates (P4) and the final combination as it
+// accumulates (ACC), nv components each, in units of a rate (u = u0 - dt *)
+struct ImexBufs {
+  double* P4;
+  double* ACC;
 };
 
 // 128 threads (two waves) per flux block: with 24 KB of LDS per block, six
@@ -127,9 +137,14 @@ __host__ __device__ __forceinline__ long long flux_tiles(int nx, int ny, int nz,
 // repair.  Each lane touches only its own column of sw, so no barrier is
 // needed.  Every variant then fits without scratch spills; before this, the
 // viscous WENO5 kernel spilled ~48 VGPRs per lane to scratch, which lives in
-// HBM (about 1.7 GB of extra traffic per sweep on 192 x 192 x 128).
+// HBM (about 1.7 GB of extra traffic per sweep on 192 x 192 x 128).  The
+// viscous resistive system (NV = 19, 39 KB of LDS per block) fits four
+// blocks per CU, two waves per SIMD, and is bounded for that.
+template <int SYS>
+constexpr int flux_waves() { return SysT<SYS>::NV > 12 ? 2 : 3; }
+
 template <int DIR, int REC, int SYS, bool ZZ, int TY = 1>
-__global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double* __restrict__ V,
+__global__ __launch_bounds__(FLUX_BLOCK, flux_waves<SYS>()) void k_flux(GridArgs a, const double* __restrict__ V,
                                                      double* __restrict__ F, double* __restrict__ L,
                                                      double* __restrict__ partial) {
   constexpr bool VISC = SysT<SYS>::VISC;
@@ -208,6 +223,7 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
     __shared__ double sw[2][NV][FLUX_BLOCK];
     constexpr int KREG = VISC ? KPR + 1 : NV;
     double wl[NV], wr[NV], c0[5];
+    double dEface = 0.;   // resistive: E^d(right cell) - E^d(left cell), for div E
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       double s[6];
@@ -218,6 +234,9 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
       rec_face<REC>(s, l, r);
       if (k < KREG) { wl[k] = l; wr[k] = r; } else { sw[0][k][threadIdx.x] = l; sw[1][k][threadIdx.x] = r; }
       if (k <= KPR) c0[k] = s[2];
+      if constexpr (SysT<SYS>::RES) {
+        if (k == SysT<SYS>::KE + DIR - 1) dEface = s[3] - s[2];
+      }
     }
     // repair of reconstructed states with the cell left of the interface
     // (evolve.f90:334-369; the reference uses w0(ix) for both sides)
@@ -248,13 +267,25 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
     if constexpr (SysT<SYS>::GLM) {
       // work_glm{x,y,z}, work.f90:67-139: Dedner interface values of psi and
       // the normal B.  The reference sets wr(psi) = wl(psi) before taking
-      // their difference, so the B update reduces to the average.
-      constexpr int KB = DIR == 1 ? KBX : (DIR == 2 ? KBY : KBZ);
+      // their difference, so the B update reduces to the average.  In the
+      // viscous resistive system these components wait in LDS.
+      constexpr int KB = SysT<SYS>::KB + DIR - 1, KG = SysT<SYS>::KG;
       const double ch = S->glm_ch;
-      wl[KGLM] = 0.5 * (wl[KGLM] + wr[KGLM]) - (wr[KB] - wl[KB]) * ch / 2;
-      wr[KGLM] = wl[KGLM];
-      wl[KB] = 0.5 * (wl[KB] + wr[KB]) - (wr[KGLM] - wl[KGLM]) / (2 * ch);
-      wr[KB] = wl[KB];
+      double gl, gr, bl, br;
+      if constexpr (KG < KREG) { gl = wl[KG]; gr = wr[KG]; bl = wl[KB]; br = wr[KB]; }
+      else {
+        gl = sw[0][KG][threadIdx.x]; gr = sw[1][KG][threadIdx.x];
+        bl = sw[0][KB][threadIdx.x]; br = sw[1][KB][threadIdx.x];
+      }
+      gl = 0.5 * (gl + gr) - (br - bl) * ch / 2;
+      gr = gl;
+      bl = 0.5 * (bl + br) - (gr - gl) / (2 * ch);
+      br = bl;
+      if constexpr (KG < KREG) { wl[KG] = gl; wr[KG] = gr; wl[KB] = bl; wr[KB] = br; }
+      else {
+        sw[0][KG][threadIdx.x] = gl; sw[1][KG][threadIdx.x] = gr;
+        sw[0][KB][threadIdx.x] = bl; sw[1][KB][threadIdx.x] = br;
+      }
     }
     // Park wr in LDS while the left side state is formed, then wl while the
     // right one is.  side_state completes the pi^{tau mu} components of its
@@ -298,6 +329,9 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
       // aflux(dir) keeps the last interface of the sweep (work.f90:217,310,401)
       if (f == nf - 1) a.S->aflux[DIR - 1] = amax;
       double a1 = (ap == 0.0 && am == 0.0) ? 1. : rcp_nr(ap + am);
+      // resistive: B, E and psi with the speed of light (coordinate speed
+      // 1 / sqrt(g_dir))
+      const double clight = DIR == 3 ? M.gm : 1.;
 #pragma unroll
       for (int k = 0; k < NV; k++) {
         double ql[NV], qr[NV];
@@ -307,9 +341,12 @@ __global__ __launch_bounds__(FLUX_BLOCK, 3) void k_flux(GridArgs a, const double
         double urk = u_comp<SYS>(k, M, sr, qr) * M.gp;
         double flk = f_comp<DIR, SYS>(k, M, sl, ql) * M.gp;
         double frk = f_comp<DIR, SYS>(k, M, sr, qr) * M.gp;
-        __builtin_nontemporal_store(a1 * (ap * flk + am * frk - ap * am * (urk - ulk)),
-                                    &F[(long long)k * nf + f]);
+        double fk;
+        if (SysT<SYS>::RES && k >= SysT<SYS>::KB) fk = 0.5 * (flk + frk - clight * (urk - ulk));
+        else fk = a1 * (ap * flk + am * frk - ap * am * (urk - ulk));
+        __builtin_nontemporal_store(fk, &F[(long long)k * nf + f]);
       }
+      if constexpr (SysT<SYS>::RES) __builtin_nontemporal_store(dEface, &F[(long long)NV * nf + f]);
     }
   }
   // block max of the interface speeds -> partial[blockIdx.x]
@@ -364,7 +401,14 @@ static __global__ __launch_bounds__(1024) void k_cfl(GridArgs a, const double* _
     if (a.nx > 1) amax = dmax(amax, a.ddx * mm[0]);
     if (a.ny > 1) amax = dmax(amax, a.ddy * mm[1]);
     if (a.nz > 1) amax = dmax(amax, a.ddz * mm[2]);
-    if (a.divclean) {
+    if (a.res) {
+      // resistive systems: the speed of light limits the step
+      const double g3 = make_metric(a.coordinates, a.S->t).g3;
+      amax = 0.;
+      if (a.nx > 1) amax = dmax(amax, a.ddx);
+      if (a.ny > 1) amax = dmax(amax, a.ddy);
+      if (a.nz > 1) amax = dmax(amax, a.ddz / sqrt(g3));
+    } else if (a.divclean) {
       amax = 1.;   // evolve.f90:734-736
     } else if (amax == 0) {
       if (a.nx > 1) amax = dmax(amax, a.ddx);
@@ -428,6 +472,57 @@ __device__ __forceinline__ Deriv cell_deriv(const GridArgs& a, const Metric& M, 
   return D;
 }
 
+// div E of a resistive grid at a cell: each sweep stored, per face, the
+// difference of E^d between the cells on either side (slot nv of its flux
+// buffer); the central difference of system_compute_divE
+// (system.f90:1901-1918) is the mean of the cell's two face differences.
+__device__ __forceinline__ double res_div_e(const GridArgs& a, int x, int y, int z,
+                                            const double* __restrict__ FX, const double* __restrict__ FY,
+                                            const double* __restrict__ FZ, int nv) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  double r = 0.;
+  if (nx > 1) {
+    const long long nf = (long long)(nx + 1) * ny * nz, f0 = x + (long long)(nx + 1) * (y + (long long)ny * z);
+    r = r + (FX[nv * nf + f0] + FX[nv * nf + f0 + 1]) * (a.ddx / 2.);
+  }
+  if (ny > 1) {
+    const long long nf = (long long)nx * (ny + 1) * nz, f0 = x + (long long)nx * (y + (long long)(ny + 1) * z);
+    r = r + (FY[nv * nf + f0] + FY[nv * nf + f0 + nx]) * (a.ddy / 2.);
+  }
+  if (nz > 1) {
+    const long long nf = (long long)nx * ny * (nz + 1), f0 = x + (long long)nx * (y + (long long)ny * z);
+    r = r + (FZ[nv * nf + f0] + FZ[nv * nf + f0 + (long long)nx * ny]) * (a.ddz / 2.);
+  }
+  return r;
+}
+
+// flux divergence of a cell, sum over the active directions of
+// dd * (F(face+1) - F(face)) (evolve.f90:383 and the y/z analogues)
+template <int NV>
+__device__ __forceinline__ void flux_div(const GridArgs& a, int x, int y, int z, const double* __restrict__ FX,
+                                         const double* __restrict__ FY, const double* __restrict__ FZ,
+                                         double* du) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+#pragma unroll
+  for (int k = 0; k < NV; k++) du[k] = 0.;
+  if (nx > 1) {
+    long long nf = (long long)(nx + 1) * ny * nz, f0 = x + (long long)(nx + 1) * (y + (long long)ny * z);
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddx * (FX[k * nf + f0 + 1] - FX[k * nf + f0]);
+  }
+  if (ny > 1) {
+    long long nf = (long long)nx * (ny + 1) * nz, f0 = x + (long long)nx * (y + (long long)(ny + 1) * z);
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddy * (FY[k * nf + f0 + nx] - FY[k * nf + f0]);
+  }
+  if (nz > 1) {
+    long long nf = (long long)nx * ny * (nz + 1), f0 = x + (long long)nx * (y + (long long)ny * z);
+    const long long st = (long long)nx * ny;
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddz * (FZ[k * nf + f0 + st] - FZ[k * nf + f0]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // k_update: one thread per cell.  Flux divergence (evolve.f90:383 and the
 // y/z analogues) minus the sources (evolve_src, evolve.f90:689-718), the RK
@@ -458,24 +553,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
   const Metric M = make_metric(a.coordinates, S->t);
   const long long N = a.N;
   double du[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) du[k] = 0.;
-  if (nx > 1) {
-    long long nf = (long long)(nx + 1) * ny * nz, f0 = x + (long long)(nx + 1) * (y + (long long)ny * z);
-#pragma unroll
-    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddx * (FX[k * nf + f0 + 1] - FX[k * nf + f0]);
-  }
-  if (ny > 1) {
-    long long nf = (long long)nx * (ny + 1) * nz, f0 = x + (long long)nx * (y + (long long)(ny + 1) * z);
-#pragma unroll
-    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddy * (FY[k * nf + f0 + nx] - FY[k * nf + f0]);
-  }
-  if (nz > 1) {
-    long long nf = (long long)nx * ny * (nz + 1), f0 = x + (long long)nx * (y + (long long)ny * z);
-    const long long st = (long long)nx * ny;
-#pragma unroll
-    for (int k = 0; k < NV; k++) du[k] = du[k] + a.ddz * (FZ[k * nf + f0 + st] - FZ[k * nf + f0]);
-  }
+  flux_div<NV>(a, x, y, z, FX, FY, FZ, du);
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(&V[k * N + c]);   // read once
@@ -489,7 +567,18 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     double vv[NV], src[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) vv[k] = v[k];
-    int e = sys_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
+    int e;
+    if constexpr (SysT<SYS>::RES) {
+      // div E from the face differences the sweeps stored, and Ohm's
+      // current taken explicitly (an explicit RK has no implicit part)
+      const double rhoc = res_div_e(a, x, y, z, FX, FY, FZ, NV);
+      double sst[3];
+      e = res_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, rhoc, a.sig, a.sch, src, sst);
+#pragma unroll
+      for (int k = 0; k < 3; k++) src[SysT<SYS>::KE + k] = src[SysT<SYS>::KE + k] + sst[k];
+    } else {
+      e = sys_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
+    }
     if (e > 0) {
       atomicMax(&S->errcode, e);
       atomicExch(&S->run_crashed, 1);
@@ -507,23 +596,25 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     uc[k] = __builtin_nontemporal_load(&Ucur[k * N + c]);
   }
   if constexpr (SysT<SYS>::GLM) {
+    constexpr int KG = SysT<SYS>::KG;
     if (first_stage) {
       // evolve_glm_source(dt/2) before u0 = u (evolve.f90:196): the damped
       // psi is also the RK base of the later stages
       const double ch = S->glm_ch;
       double ef = exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
-      u0[KGLM] = u0[KGLM] * ef;
-      uc[KGLM] = u0[KGLM];
-      U0[KGLM * N + c] = u0[KGLM];
+      u0[KG] = u0[KG] * ef;
+      uc[KG] = u0[KG];
+      U0[KG * N + c] = u0[KG];
     }
   }
   double un[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) un[k] = c1 * u0[k] + c2 * (uc[k] - dt * du[k]);
   if constexpr (SysT<SYS>::GLM) {
+    constexpr int KG = SysT<SYS>::KG;
     if (last_stage) {   // evolve.f90:201
       const double ch = S->glm_ch;
-      un[KGLM] = un[KGLM] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
+      un[KG] = un[KG] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
     }
   }
   int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);
@@ -559,6 +650,221 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 }
 
 // ---------------------------------------------------------------------------
+// IMEX-SSP3(4,3,3) (Pareschi & Russo; coefficients of evolve.f90:127-130,
+// stages of evolve.f90:204-264) for the resistive systems.  With du the
+// explicit rate (flux divergence minus the non-stiff sources) and ds the
+// stiff one (minus Ohm's current, only E), u = u0 - dt * (...):
+//   K1 = u0                                  + a dt R(K1)
+//   K2 = u0 + a dt ds1                       + a dt R(K2)
+//   K3 = u0 - dt (du2 + (1 - a) ds2)         + a dt R(K3)
+//   K4 = u0 - dt (du2/4 + du3/4 + b ds1 + h ds2 + d ds3) + a dt R(K4)
+//   u1 = u0 - dt (du2/6 + du3/6 + 2/3 du4 + ds2/6 + ds3/6 + 2/3 ds4)
+// (the explicit rate of K1 has weight zero, so the first stage needs no
+// sweep).  Each implicit solve (res_cons2prim_implicit) inverts the stage
+// start U* for the primitives and E; U keeps U*, V gets the stage's
+// primitives.  The kernels below are shifted against the stages: each
+// finishes the rates of one stage and solves the next stage's implicit part.
+// ---------------------------------------------------------------------------
+constexpr double IMEX_A = 0.24169426078821, IMEX_B = 0.06042356519705, IMEX_H = 0.12915286960590;
+constexpr double IMEX_D = 0.5 - IMEX_B - IMEX_H - IMEX_A;
+
+// implicit solve of one cell from its stage start us; marks a failure for
+// k_fixup (which repairs with U = U*, V for the cells before it)
+template <int SYS, bool ZZ>
+__device__ __forceinline__ void imex_solve(const GridArgs& a, const Metric& M, long long c, double* __restrict__ V,
+                                           double* __restrict__ Ust, const FixBufs& fb, const double* us,
+                                           double* v, double adt) {
+  constexpr int NV = SysT<SYS>::NV;
+  const long long N = a.N;
+  __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
+  double uu[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    uu[k] = us[k];
+    __builtin_nontemporal_store(us[k], &Ust[(long long)k * N + c]);
+  }
+  const int e = res_cons2prim_implicit<SYS, ZZ>(a.eos, M, uu, v, adt, a.sig, a.sch);
+  if (e != 0) {
+    atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+    atomicAdd(&a.S->nfail, 1);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) __builtin_nontemporal_store(v[k], &V[(long long)k * N + c]);
+}
+
+// stiff rate ds = -sqrt(-g) R of a cell's primitives (E components), the
+// du_stiff of evolve_src (evolve.f90:689-718)
+template <int SYS>
+__device__ __forceinline__ void imex_stiff_rate(const GridArgs& a, const Metric& M, const double* v, double* ds) {
+  using T = SysT<SYS>;
+  const double vel[3] = {v[KVX], v[KVY], v[KVZ]};
+  double R[3];
+  ohm_stiff(M, vel, &v[T::KE], &v[T::KB], a.sig, a.sch, R);
+#pragma unroll
+  for (int k = 0; k < 3; k++) ds[k] = -(R[k] * M.gp);
+}
+
+// stage 1: K1 from U* = u0
+template <int SYS, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_k1(GridArgs a, double* __restrict__ V,
+                                                          const double* __restrict__ U0,
+                                                          double* __restrict__ Ust, FixBufs fb) {
+  constexpr int NV = SysT<SYS>::NV;
+  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N || a.S->temp_check || a.S->run_crashed) return;
+  const long long N = a.N;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  double us[NV], v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    us[k] = U0[(long long)k * N + c];
+    v[k] = V[(long long)k * N + c];
+  }
+  imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, us, v, IMEX_A * a.S->dt);
+}
+
+// the stiff rate of K1 (cell-local), P4 = b ds1, and K2 from
+// U* = u0 + a dt ds1
+template <int SYS, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_k2(GridArgs a, double* __restrict__ V,
+                                                          const double* __restrict__ U0,
+                                                          double* __restrict__ Ust, ImexBufs ib, FixBufs fb) {
+  using T = SysT<SYS>;
+  constexpr int NV = T::NV;
+  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N || a.S->temp_check || a.S->run_crashed) return;
+  const long long N = a.N;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  const double dt = a.S->dt;
+  double us[NV], v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    us[k] = U0[(long long)k * N + c];
+    v[k] = V[(long long)k * N + c];
+  }
+  double ds[3];
+  imex_stiff_rate<SYS>(a, M, v, ds);
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const int ke = k - T::KE;
+    const bool isE = ke >= 0 && ke < 3;
+    ib.P4[(long long)k * N + c] = isE ? IMEX_B * ds[isE ? ke : 0] : 0.;
+    if (isE) us[k] = us[k] + dt * IMEX_A * ds[ke];
+  }
+  imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, us, v, IMEX_A * dt);
+}
+
+// stages 2-4 after their sweeps: the rates of stage STAGE at K_STAGE
+// (flux divergence, non-stiff sources with the charge density div E, stiff
+// rate), their accumulation, and K_{STAGE+1} (STAGE 2, 3) or the step's
+// result with the GLM damping of psi (evolve.f90:262), its plain inversion
+// and the viscous smoothing (STAGE 4)
+template <int SYS, bool ZZ, int STAGE>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
+    GridArgs a, double* __restrict__ V, double* __restrict__ U0, double* __restrict__ Ust,
+    const double* __restrict__ FX, const double* __restrict__ FY, const double* __restrict__ FZ,
+    const double* __restrict__ LX, const double* __restrict__ LY, const double* __restrict__ LZ,
+    double* __restrict__ VS0, double* __restrict__ VS1, ImexBufs ib, FixBufs fb) {
+  using T = SysT<SYS>;
+  constexpr bool VISC = T::VISC;
+  constexpr int NV = T::NV;
+  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N || a.S->temp_check) return;
+  DevScalars* S = a.S;
+  if (S->run_crashed) return;
+  const int nx = a.nx, ny = a.ny;
+  const int x = (int)(c % nx);
+  const long long r = c / nx;
+  const int y = (int)(r % ny), z = (int)(r / ny);
+  const Metric M = make_metric(a.coordinates, S->t);
+  const long long N = a.N;
+  double du[NV];
+  flux_div<NV>(a, x, y, z, FX, FY, FZ, du);
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(&V[k * N + c]);
+  Deriv D{};
+  const int par = S->vs_parity;
+  double* VSn = par ? VS1 : VS0;
+  double* VSo = par ? VS0 : VS1;
+  if constexpr (VISC) D = cell_deriv(a, M, c, x, y, z, LX, LY, LZ, VSn, VSo);
+  double ds[3];
+  {
+    double vv[NV], src[NV], sst[3];
+#pragma unroll
+    for (int k = 0; k < NV; k++) vv[k] = v[k];
+    const double rhoc = res_div_e(a, x, y, z, FX, FY, FZ, NV);
+    const int e = res_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, rhoc, a.sig, a.sch, src, sst);
+    if (e > 0) {
+      atomicMax(&S->errcode, e);
+      atomicExch(&S->run_crashed, 1);
+      atomicCAS(&S->crash_kind, CRASH_NONE, CRASH_SOURCE);
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) du[k] = du[k] - src[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) ds[k] = -sst[k];
+  }
+  const double dt = S->dt;
+  double u0[NV], un[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) u0[k] = __builtin_nontemporal_load(&U0[k * N + c]);
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const int ke = k - T::KE;
+    const bool isE = ke >= 0 && ke < 3;
+    const double s = isE ? ds[isE ? ke : 0] : 0.;
+    const long long o = (long long)k * N + c;
+    if constexpr (STAGE == 2) {
+      ib.P4[o] = ib.P4[o] + (du[k] / 4. + IMEX_H * s);
+      ib.ACC[o] = du[k] / 6. + s / 6.;
+      un[k] = u0[k] - dt * (du[k] + (1. - IMEX_A) * s);
+    } else if constexpr (STAGE == 3) {
+      un[k] = u0[k] - dt * (ib.P4[o] + du[k] / 4. + IMEX_D * s);
+      ib.ACC[o] = ib.ACC[o] + (du[k] / 6. + s / 6.);
+    } else {
+      un[k] = u0[k] - dt * (ib.ACC[o] + du[k] * 2. / 3. + s * 2. / 3.);
+    }
+  }
+  if constexpr (STAGE < 4) {
+    imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, un, v, IMEX_A * dt);
+  } else {
+    {
+      const double ch = S->glm_ch;
+      un[T::KG] = un[T::KG] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
+    }
+    __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
+    const int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);
+    if (e != 0) {
+      atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+      atomicAdd(&S->nfail, 1);
+    } else if (VISC && a.enable_smooth) {
+      const double ps = S->smooth_pressure;
+      if (v[KPR] < ps) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) fb.Upre[k * N + c] = un[k];
+        double q = v[KPR] / ps;
+#pragma unroll
+        for (int k = KPIBU; k <= KPIZZ; k++) v[k] = v[k] * (q * q);
+        if constexpr (ZZ) derived_pi<true>(v, M);
+        prim2cons<SYS, ZZ>(a.eos, M, v, un);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      __builtin_nontemporal_store(un[k], &U0[k * N + c]);
+      __builtin_nontemporal_store(v[k], &V[k * N + c]);
+    }
+    if (VISC) {
+      __builtin_nontemporal_store(v[KVX], &VSo[c]);
+      __builtin_nontemporal_store(v[KVY], &VSo[N + c]);
+      __builtin_nontemporal_store(v[KVZ], &VSo[2 * N + c]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_fixup: the con2prim repair of evolve_u2v (evolve.f90:795-842) for the
 // cells that k_update could not invert, with the reference's sequential
 // semantics.  The reference repairs cell by cell in loop order (x fastest)
@@ -584,7 +890,7 @@ __device__ __forceinline__ bool fix_failed(const unsigned* __restrict__ bits, lo
 template <int SYS, bool ZZ>
 __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, double* __restrict__ V,
                             double* __restrict__ U, const FixBufs& fb, double* __restrict__ VSnext,
-                            int last_stage) {
+                            int last_stage, int imex, int smooth_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   DevScalars* S = a.S;
@@ -599,9 +905,13 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
                            c - (z > 0 ? sxy : 0), c + (z < nz - 1 ? sxy : 0)};
   // u of a neighbour as the reference's loop sees it: a smoothed cell's u
   // before its smoothing (fb.Upre), else U (the RK result of later cells,
-  // failed ones included, and the repaired u of earlier failed cells)
+  // failed ones included, and the repaired u of earlier failed cells).
+  // Inside an implicit IMEX stage U holds the explicit start U* of every
+  // cell, and V the E its implicit solve found: earlier cells are read with
+  // that E (u_E = sqrt(-g) E), later ones (not yet solved in the
+  // reference's loop) with E*.
   const double* us[6];
-  const bool smooth = VISC && a.enable_smooth;
+  const bool smooth = VISC && a.enable_smooth && smooth_stage;
   const double ps = S->smooth_pressure;
 #pragma unroll
   for (int j = 0; j < 6; j++) {
@@ -613,8 +923,18 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     const long long o = (long long)k * N;
-    uu[k] = (us[0][o + nb[0]] + us[1][o + nb[1]] + us[2][o + nb[2]] + us[3][o + nb[3]] + us[4][o + nb[4]] +
-             us[5][o + nb[5]]) / 6.;
+    double w[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) w[j] = us[j][o + nb[j]];
+    if constexpr (SysT<SYS>::RES) {
+      constexpr int KE = SysT<SYS>::KE;
+      if (imex && k >= KE && k < KE + 3) {
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+          if (nb[j] < c) w[j] = V[o + nb[j]] * M.gp;
+      }
+    }
+    uu[k] = (w[0] + w[1] + w[2] + w[3] + w[4] + w[5]) / 6.;
     vv[k] = V[o + c];   // the primitives k_update's failed inversion left
   }
   cons2prim<SYS, ZZ>(a.eos, M, uu, vv, a.algo_solver);
@@ -653,7 +973,8 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
 template <int SYS, bool ZZ>
 __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
                                                      FixBufs fb, double* __restrict__ VS0,
-                                                     double* __restrict__ VS1, int last_stage) {
+                                                     double* __restrict__ VS1, int last_stage, int imex,
+                                                     int smooth_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   DevScalars* S = a.S;
@@ -665,7 +986,8 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
     if (!S->temp_check) {
       // glm_ch_old = maxval(aflux) of this stage; the next stage starts with
       // glm_ch = glm_ch_old (evolve.f90:157-163)
-      S->glm_ch_old = dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
+      // (resistive systems clean at the speed of light)
+      S->glm_ch_old = a.res ? 1. : dmax(dmax(S->aflux[0], S->aflux[1]), S->aflux[2]);
       S->glm_ch = S->glm_ch_old;
       sh_go = S->nfail > 0;
     }
@@ -733,7 +1055,7 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
   for (int lev = 0; lev <= sh_maxlev; lev++) {
     for (int i = t; i < n; i += FIX_BLOCK) {
       const long long c = fb.list[i];
-      if (fb.level[c] == lev) repair_cell<SYS, ZZ>(a, M, c, V, U, fb, VSnext, last_stage);
+      if (fb.level[c] == lev) repair_cell<SYS, ZZ>(a, M, c, V, U, fb, VSnext, last_stage, imex, smooth_stage);
     }
     __syncthreads();
   }
@@ -741,7 +1063,7 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
   __syncthreads();
   // 4. evolve_smooth_viscosity (evolve.f90:1159-1207) of the repaired cells
   //    (not after a crash: evolve_main returns before it, evolve.f90:276-281)
-  if (VISC && a.enable_smooth && !sh_stop) {
+  if (VISC && a.enable_smooth && smooth_stage && !sh_stop) {
     const double ps = S->smooth_pressure;
     for (int i = t; i < n; i += FIX_BLOCK) {
       const long long c = fb.list[i];

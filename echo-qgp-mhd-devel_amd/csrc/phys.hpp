@@ -18,17 +18,27 @@ enum { MINKOWSKI = 1, BJORKEN = 2 };
 enum { SHEAR_VISCOUS_1D = 2, GUBSER_VISCOUS = 5 };
 
 // Evolved systems (system_setup, system.f90:67-128): hydro, viscous hydro,
-// ideal MHD, ideal MHD with Dedner GLM divergence cleaning.
-enum { SYS_HD = 0, SYS_VISC = 1, SYS_MHD = 2, SYS_MHDGLM = 3 };
+// ideal MHD, ideal MHD with Dedner GLM divergence cleaning, and resistive
+// MHD (E evolved, with the Dedner scalar) without and with viscosity.
+enum { SYS_HD = 0, SYS_VISC = 1, SYS_MHD = 2, SYS_MHDGLM = 3, SYS_RMHD = 4, SYS_VRMHD = 5 };
+// magnetic field and GLM slots of the ideal MHD systems (kbx..kbz, kglm)
+enum { KBX = 5, KBY = 6, KBZ = 7, KGLM = 8 };
 template <int SYS>
 struct SysT {
-  static constexpr bool VISC = SYS == SYS_VISC;
-  static constexpr bool MHD = SYS == SYS_MHD || SYS == SYS_MHDGLM;
-  static constexpr bool GLM = SYS == SYS_MHDGLM;
-  static constexpr int NV = SYS == SYS_HD ? 5 : (SYS == SYS_VISC ? 12 : (SYS == SYS_MHD ? 8 : 9));
+  static constexpr bool RES = SYS == SYS_RMHD || SYS == SYS_VRMHD;
+  static constexpr bool VISC = SYS == SYS_VISC || SYS == SYS_VRMHD;
+  static constexpr bool MHD = SYS == SYS_MHD || SYS == SYS_MHDGLM;   // ideal MHD
+  static constexpr bool GLM = SYS == SYS_MHDGLM || RES;
+  // a resistive system is its fluid system (hydro or viscous, NF variables)
+  // followed by B^x, B^y, B^eta, E^x, E^y, E^eta and psi (the layout of
+  // system_setup, system.f90:80-124)
+  static constexpr int FLUID = VISC ? SYS_VISC : SYS_HD;
+  static constexpr int NF = VISC ? 12 : 5;
+  static constexpr int NV = RES ? NF + 7 : (SYS == SYS_HD ? 5 : (SYS == SYS_VISC ? 12 : (SYS == SYS_MHD ? 8 : 9)));
+  static constexpr int KB = RES ? NF : KBX;        // B^x
+  static constexpr int KE = NF + 3;                // E^x (resistive systems)
+  static constexpr int KG = RES ? NF + 6 : KGLM;   // psi
 };
-// magnetic field and GLM slots of the MHD systems (kbx..kbz, kglm)
-enum { KBX = 5, KBY = 6, KBZ = 7, KGLM = 8 };
 
 // Reconstruction codes (same numbering as the C-ABI echo_config.recal).
 enum { REC_TVD2 = 1, REC_CENO3 = 2, REC_WENO3 = 3, REC_WENO5 = 4, REC_PPM4 = 5,
@@ -432,7 +442,11 @@ __device__ __forceinline__ void e_cross_b(const Metric& M, const double* E, cons
 }
 
 template <int SYS, bool ZZ>
+__device__ int res_prim2cons_fwd(const Eos& E, const Metric& M, double* v, double* u);
+
+template <int SYS, bool ZZ>
 __device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* v, double* u) {
+  if constexpr (SysT<SYS>::RES) return res_prim2cons_fwd<SYS, ZZ>(E, M, v, u);
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en;
@@ -491,18 +505,33 @@ __device__ __forceinline__ int prim2cons(const Eos& E, const Metric& M, double* 
 // nv-vectors) is what keeps the flux kernel's register use low.
 // ---------------------------------------------------------------------------
 struct SideState {
-  double rh, glf, ww, pr, v2, cs2, ca2, vdir, wwdir;
+  double rh, glf, ww, pr, v2, cs2, ca2, vdir, wwdir, enpr;
   PiDer d;
   double E[3], X[3], EEM;   // MHD: covariant E, E x B, field energy
+  double Ev[3];             // resistive MHD: contravariant E
 };
+
+// resistive systems (defined at the end of this file)
+template <int SYS, bool ZZ>
+__device__ int res_prim2cons(const Eos& E, const Metric& M, double* v, double* u);
+template <int DIR, int SYS, bool ZZ>
+__device__ int res_side_state(const Eos& E, const Metric& M, double* v, SideState& s);
+template <int SYS>
+__device__ double res_u_comp(int k, const Metric& M, const SideState& s, const double* v);
+template <int DIR, int SYS>
+__device__ double res_f_comp(int k, const Metric& M, const SideState& s, const double* v);
+template <int SYS, bool ZZ>
+__device__ int res_cons2prim(const Eos& E, const Metric& M, const double* u, double* v, int algo);
 
 template <int DIR, int SYS, bool ZZ>
 __device__ __forceinline__ int side_state(const Eos& E, const Metric& M, double* v, SideState& s) {
+  if constexpr (SysT<SYS>::RES) return res_side_state<DIR, SYS, ZZ>(E, M, v, s);
   constexpr bool VISC = SysT<SYS>::VISC;
   double rh = v[KRH], vx = v[KVX], vy = v[KVY], vz = v[KVZ], pr = v[KPR], en, cs2;
   int err_sound;
   int err = eos_energy_sound(E, rh, pr, en, cs2, err_sound);
   if (err > 0) return err;
+  s.enpr = en + pr;
   double v2 = M.g1 * vx * vx + M.g2 * vy * vy + M.g3 * vz * vz;
   if (v2 > 1) return 333;
   double glf = rsqrt_nr(1. - v2);
@@ -542,6 +571,7 @@ __device__ __forceinline__ void side_speeds(const Metric& M, const SideState& s,
 
 template <int SYS>
 __device__ __forceinline__ double u_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  if constexpr (SysT<SYS>::RES) return res_u_comp<SYS>(k, M, s, v);
   constexpr bool VISC = SysT<SYS>::VISC;
   if constexpr (SysT<SYS>::MHD) {
     switch (k) {
@@ -566,6 +596,7 @@ __device__ __forceinline__ double u_comp(int k, const Metric& M, const SideState
 
 template <int DIR, int SYS>
 __device__ __forceinline__ double f_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  if constexpr (SysT<SYS>::RES) return res_f_comp<DIR, SYS>(k, M, s, v);
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr bool GLM = SysT<SYS>::GLM;
   const double wd = s.wwdir;
@@ -1041,6 +1072,7 @@ __device__ __forceinline__ int mhd_cons2prim(const Eos& E, int algo, const Metri
 template <int SYS, bool ZZ>
 __device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const double* u, double* v,
                                          int algo = 5) {
+  if constexpr (SysT<SYS>::RES) return res_cons2prim<SYS, ZZ>(E, M, u, v, algo);
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   if constexpr (SysT<SYS>::MHD) return mhd_cons2prim<SYS>(E, algo, M, u, v);
@@ -1129,6 +1161,269 @@ __device__ __forceinline__ int cons2prim(const Eos& E, const Metric& M, const do
     if constexpr (ZZ) derived_pi<true>(v, M);
     return errcode;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Resistive MHD (E evolved, Dedner scalar), without and with viscosity.  The
+// reference has no working path for it (system_cons2prim has no algorithm 7,
+// system.f90:1160-1162; viscosity with MHD is refused, common.f90:1904-1908);
+// this is the build's design, assembled from the reference's pieces: the
+// fluid part of every operator is the hydro or viscous system's own, and the
+// electromagnetic part follows the MHD branch of system_prim2cons /
+// system_flux (system.f90:200-263, 1425-1549) with E evolved: E x B and
+// (E^2 + B^2)/2 in the momentum and energy, the Maxwell stresses in the
+// momentum fluxes, the induction (B) and Ampere (E) fluxes, and the GLM
+// fluxes of psi and the normal B.  Covariant E_eta = g33 E^eta throughout
+// (the reference's resistive flux drops g33, system.f90:1369).
+// ---------------------------------------------------------------------------
+struct EmTerms {
+  double Ec[3];   // covariant E
+  double X[3];    // E x B, covariant, times sqrt(-g) (system_EXB)
+  double E2, B2, eem;
+};
+
+__device__ __forceinline__ EmTerms em_terms(const Metric& M, const double* Ev, const double* B) {
+  EmTerms t;
+  t.Ec[0] = Ev[0];
+  t.Ec[1] = Ev[1];
+  t.Ec[2] = Ev[2] * M.g3;
+  t.E2 = t.Ec[0] * Ev[0] + t.Ec[1] * Ev[1] + t.Ec[2] * Ev[2];
+  t.B2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2] * M.g3;
+  t.X[0] = M.gp * (Ev[1] * B[2] - Ev[2] * B[1]);
+  t.X[1] = M.gp * (Ev[2] * B[0] - Ev[0] * B[2]);
+  t.X[2] = M.gp * (Ev[0] * B[1] - Ev[1] * B[0]);
+  t.eem = 0.5 * (t.E2 + t.B2);
+  return t;
+}
+
+// contravariant a x b of contravariant a, b (minus the E of
+// system_compute_E_in_ideal_MHD, system.f90:1882-1897, made contravariant)
+__device__ __forceinline__ void cross_contra(const Metric& M, const double* a, const double* b, double* c) {
+  c[0] = M.gp * (a[1] * b[2] - a[2] * b[1]);
+  c[1] = M.gp * (a[2] * b[0] - a[0] * b[2]);
+  c[2] = M.gp * (a[0] * b[1] - a[1] * b[0]) / M.g3;
+}
+
+template <int SYS, bool ZZ>
+__device__ int res_prim2cons(const Eos& E, const Metric& M, double* v, double* u) {
+  using T = SysT<SYS>;
+  const int e = prim2cons<T::FLUID, ZZ>(E, M, v, u);
+  if (e > 0) return e;
+  const EmTerms t = em_terms(M, &v[T::KE], &v[T::KB]);
+  u[KVX] = u[KVX] + M.gp * t.X[0];
+  u[KVY] = u[KVY] + M.gp * t.X[1];
+  u[KVZ] = u[KVZ] + M.gp * t.X[2];
+  u[KPR] = u[KPR] + M.gp * t.eem;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    u[T::KB + k] = M.gp * v[T::KB + k];
+    u[T::KE + k] = M.gp * v[T::KE + k];
+  }
+  u[T::KG] = v[T::KG];   // (psi / gp) * gp, system.f90:253
+  return 0;
+}
+
+template <int SYS, bool ZZ>
+__device__ int res_prim2cons_fwd(const Eos& E, const Metric& M, double* v, double* u) {
+  return res_prim2cons<SYS, ZZ>(E, M, v, u);
+}
+
+// the fluid state's side values, then the field's; the fluid variables take
+// the fast magnetosonic speeds of system.f90:1557-1580 with B^2 - E^2 for B^2
+template <int DIR, int SYS, bool ZZ>
+__device__ int res_side_state(const Eos& E, const Metric& M, double* v, SideState& s) {
+  using T = SysT<SYS>;
+  const int e = side_state<DIR, T::FLUID, ZZ>(E, M, v, s);
+  if (e) return e;
+  const EmTerms t = em_terms(M, &v[T::KE], &v[T::KB]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    s.Ev[k] = v[T::KE + k];
+    s.E[k] = t.Ec[k];
+    s.X[k] = t.X[k];
+  }
+  s.EEM = t.eem;
+  const double bb2 = dmax(t.B2 - t.E2, 0.);
+  s.ca2 = bb2 / (bb2 + s.enpr);
+  return 0;
+}
+
+template <int SYS>
+__device__ double res_u_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  using T = SysT<SYS>;
+  if (k < T::NF) {
+    double r = u_comp<T::FLUID>(k, M, s, v);
+    if (k == KVX) r = r + s.X[0];
+    else if (k == KVY) r = r + s.X[1];
+    else if (k == KVZ) r = r + s.X[2];
+    else if (k == KPR) r = r + s.EEM;
+    return r;
+  }
+  if (k == T::KG) return v[T::KG] / M.gp;
+  return v[k];   // B, E
+}
+
+template <int DIR, int SYS>
+__device__ double res_f_comp(int k, const Metric& M, const SideState& s, const double* v) {
+  using T = SysT<SYS>;
+  const double* B = &v[T::KB];
+  const double* Ev = s.Ev;
+  const double* Ec = s.E;
+  constexpr int D = DIR - 1;
+  if (k < T::NF) {
+    const double r = f_comp<DIR, T::FLUID>(k, M, s, v);
+    // Maxwell stress -E^d E_i - B^d B_i + delta_di EEM and the Poynting flux
+    double em = 0.;
+    if (k == KVX) em = -Ev[D] * Ec[0] - B[D] * B[0] + (D == 0 ? s.EEM : 0.);
+    else if (k == KVY) em = -Ev[D] * Ec[1] - B[D] * B[1] + (D == 1 ? s.EEM : 0.);
+    else if (k == KVZ) em = -Ev[D] * Ec[2] - B[D] * B[2] * M.g3 + (D == 2 ? s.EEM : 0.);
+    else if (k == KPR) em = D == 2 ? s.X[2] / M.g3 : s.X[D];
+    else return r;
+    return r + em;
+  }
+  const double psi = v[T::KG];
+  if constexpr (DIR == 1) {
+    if (k == T::KB) return psi;
+    if (k == T::KB + 1) return -Ec[2] * M.gm;
+    if (k == T::KB + 2) return Ec[1] * M.gm;
+    if (k == T::KE) return 0.;
+    if (k == T::KE + 1) return B[2] * M.g3 * M.gm;
+    if (k == T::KE + 2) return -B[1] * M.g2 * M.gm;
+    return B[0] / M.gp;
+  } else if constexpr (DIR == 2) {
+    if (k == T::KB) return Ec[2] * M.gm;
+    if (k == T::KB + 1) return psi;
+    if (k == T::KB + 2) return -Ec[0] * M.gm;
+    if (k == T::KE) return -B[2] * M.g3 * M.gm;
+    if (k == T::KE + 1) return 0.;
+    if (k == T::KE + 2) return B[0] * M.g1 * M.gm;
+    return B[1] / M.gp;
+  } else {
+    if (k == T::KB) return -Ec[1] * M.gm;
+    if (k == T::KB + 1) return Ec[0] * M.gm;
+    if (k == T::KB + 2) return psi / M.g3;
+    if (k == T::KE) return B[1] * M.g2 * M.gm;
+    if (k == T::KE + 1) return -B[0] * M.g1 * M.gm;
+    if (k == T::KE + 2) return 0.;
+    return B[2] / M.gp;
+  }
+}
+
+// E and B are read off the conserved variables, their energy and momentum
+// subtracted, and the fluid system's inversion runs on the rest
+template <int SYS, bool ZZ>
+__device__ int res_cons2prim(const Eos& E, const Metric& M, const double* u, double* v, int algo) {
+  using T = SysT<SYS>;
+  double B[3], Ev[3], uf[T::NF];
+#pragma unroll
+  for (int k = 0; k < 3; k++) { B[k] = u[T::KB + k] * M.gm; Ev[k] = u[T::KE + k] * M.gm; }
+  const EmTerms t = em_terms(M, Ev, B);
+#pragma unroll
+  for (int k = 0; k < T::NF; k++) uf[k] = u[k];
+  uf[KVX] = u[KVX] - M.gp * t.X[0];
+  uf[KVY] = u[KVY] - M.gp * t.X[1];
+  uf[KVZ] = u[KVZ] - M.gp * t.X[2];
+  uf[KPR] = u[KPR] - M.gp * t.eem;
+  const int e = cons2prim<T::FLUID, ZZ>(E, M, uf, v, algo);
+#pragma unroll
+  for (int k = 0; k < 3; k++) { v[T::KB + k] = B[k]; v[T::KE + k] = Ev[k]; }
+  v[T::KG] = u[T::KG];
+  return e;
+}
+
+// The stiff current of Ohm's law per unit sqrt(-g) (system.f90:1653-1662,
+// with v x B in the metric-consistent form and v.B computed):
+// -gamma [sig (E - (v.E) v + v x B) + sch (B - (v.B) v - v x E)]
+__device__ __forceinline__ void ohm_stiff(const Metric& M, const double* vel, const double* Ev, const double* B,
+                                          double sig, double sch, double* S) {
+  const double v2 = M.g1 * vel[0] * vel[0] + M.g2 * vel[1] * vel[1] + M.g3 * vel[2] * vel[2];
+  const double glf = 1. / sqrt(1. - v2);
+  double vxB[3], vxE[3];
+  cross_contra(M, vel, B, vxB);
+  cross_contra(M, vel, Ev, vxE);
+  const double vE = vel[0] * Ev[0] + vel[1] * Ev[1] + M.g3 * vel[2] * Ev[2];
+  const double vB = vel[0] * B[0] + vel[1] * B[1] + M.g3 * vel[2] * B[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    S[k] = -glf * (sig * (Ev[k] - vE * vel[k] + vxB[k]) + sch * (B[k] - vB * vel[k] - vxE[k]));
+}
+
+// The implicit part of an IMEX stage: u holds U*; finds E with
+// E = E* + adt R(E, v), R the stiff current above and v the fluid velocity
+// of the inversion with that E (B and the fluid's D, S, energy do not
+// change), by fixed-point iteration on v around a 3x3 linear solve for E
+// (Cramer's rule).  Writes the new E into u.  The role of the reference's
+// missing algorithm 7 (its locals, system.f90:494-497, declare a 3x3 system).
+template <int SYS, bool ZZ>
+__device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, double* v, double adt,
+                                      double sig, double sch) {
+  using T = SysT<SYS>;
+  double B[3], Es[3], Ev[3], uf[T::NF];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    B[k] = u[T::KB + k] * M.gm;
+    Es[k] = u[T::KE + k] * M.gm;
+    Ev[k] = v[T::KE + k];
+  }
+  int e = 0;
+  const double gp = M.gp, g3 = M.g3;
+  for (int it = 0; it < 50; it++) {
+    const double vel[3] = {v[KVX], v[KVY], v[KVZ]};
+    const double v2 = M.g1 * vel[0] * vel[0] + M.g2 * vel[1] * vel[1] + M.g3 * vel[2] * vel[2];
+    const double glf = 1. / sqrt(1. - v2);
+    const double a = adt * glf * sig, b = adt * glf * sch;
+    double vxB[3];
+    cross_contra(M, vel, B, vxB);
+    const double vB = vel[0] * B[0] + vel[1] * B[1] + g3 * vel[2] * B[2];
+    const double vl[3] = {vel[0], vel[1], g3 * vel[2]};
+    double r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) r[i] = Es[i] - a * vxB[i] - b * (B[i] - vB * vel[i]);
+    // (1 + a) E - a (v.E) v - b (v x E) = r;  (v x E)^i = C_ij E^j
+    const double C[3][3] = {{0., -gp * vel[2], gp * vel[1]},
+                            {gp * vel[2], 0., -gp * vel[0]},
+                            {-gp * vel[1] / g3, gp * vel[0] / g3, 0.}};
+    double m[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) m[i][j] = (i == j ? 1. + a : 0.) - a * vel[i] * vl[j] - b * C[i][j];
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    double En[3];
+    En[0] = (r[0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (r[1] * m[2][2] - m[1][2] * r[2]) +
+             m[0][2] * (r[1] * m[2][1] - m[1][1] * r[2])) / det;
+    En[1] = (m[0][0] * (r[1] * m[2][2] - m[1][2] * r[2]) - r[0] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+             m[0][2] * (m[1][0] * r[2] - r[1] * m[2][0])) / det;
+    En[2] = (m[0][0] * (m[1][1] * r[2] - r[1] * m[2][1]) - m[0][1] * (m[1][0] * r[2] - r[1] * m[2][0]) +
+             r[0] * (m[1][0] * m[2][1] - m[1][1] * m[2][0])) / det;
+    double dE = 0., mE = 0.;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      dE = dmax(dE, fabs(En[k] - Ev[k]));
+      mE = dmax(mE, fabs(En[k]));
+      Ev[k] = En[k];
+    }
+    const EmTerms t = em_terms(M, Ev, B);
+#pragma unroll
+    for (int k = 0; k < T::NF; k++) uf[k] = u[k];
+    uf[KVX] = u[KVX] - gp * t.X[0];
+    uf[KVY] = u[KVY] - gp * t.X[1];
+    uf[KVZ] = u[KVZ] - gp * t.X[2];
+    uf[KPR] = u[KPR] - gp * t.eem;
+    e = cons2prim<T::FLUID, ZZ>(E, M, uf, v, 7);
+    if (e) break;
+    if (it > 0 && dE <= 1e-14 * mE) break;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    v[T::KB + k] = B[k];
+    v[T::KE + k] = Ev[k];
+    u[T::KE + k] = Ev[k] * gp;
+  }
+  v[T::KG] = u[T::KG];
+  return e;
 }
 
 }  // namespace echo

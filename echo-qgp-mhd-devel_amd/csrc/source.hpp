@@ -228,4 +228,37 @@ __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coo
   return 0;
 }
 
+// Sources of the resistive systems: the fluid system's (above), the
+// electromagnetic stress in the Bjorken energy source (the T^{eta eta} of
+// the field: EEM / g33 - (E^eta)^2 - (B^eta)^2, system.f90:1647-1652 with the
+// evolved E), and in the E equations the convective current -rho_c v
+// (explicit) and Ohm's conduction current (ohm_stiff, returned in sstiff,
+// times sqrt(-g)); rhoc is div E at the cell.  sig, sch: the electric and
+// chiral conductivities in fm^-1.
+template <int SYS, bool ZZ>
+__device__ __forceinline__ int res_source(const Eos& E, const Metric& M, int coordinates, const ViscParams& P,
+                                          double* v, const Deriv& D, double rhoc, double sig, double sch,
+                                          double* src, double* sstiff) {
+  using T = SysT<SYS>;
+  const int e = sys_source<T::FLUID, ZZ>(E, M, coordinates, P, v, D, src);
+  if (e > 0) return e;
+#pragma unroll
+  for (int k = T::NF; k < T::NV; k++) src[k] = 0.;
+  const double* B = &v[T::KB];
+  const double* Ev = &v[T::KE];
+  const double vel[3] = {v[KVX], v[KVY], v[KVZ]};
+  if (coordinates == BJORKEN) {
+    const EmTerms t = em_terms(M, Ev, B);
+    src[KPR] = src[KPR] + (-M.t * (t.eem / M.g3 - Ev[2] * Ev[2] - B[2] * B[2])) * M.gp;
+  }
+  double S[3];
+  ohm_stiff(M, vel, Ev, B, sig, sch, S);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    src[T::KE + k] = -rhoc * vel[k] * M.gp;
+    sstiff[k] = S[k] * M.gp;
+  }
+  return 0;
+}
+
 }  // namespace echo

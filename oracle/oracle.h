@@ -53,6 +53,9 @@ void orc_get_repair_counts(long *out);
 /* the cells (x fastest) of the first min(n, 256) u2v repairs; returns how
  * many were written */
 int  orc_get_repair_cells(long *out, int n);
+/* the cell (x fastest) whose con2prim repair failed and crashed the run,
+ * -1 if none */
+long orc_crash_cell(void);
 
 /* pointwise operators at the current metric */
 void orc_set_time(double t);

@@ -84,6 +84,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.enable_smooth_viscosity = G.smooth_temp > 0.;
   G.glm_ch_old = 0.5;
   G.aflux[0] = G.aflux[1] = G.aflux[2] = 0.;
+  if (G.rmhd) G.glm_ch_old = 1.;   /* resistive systems clean at the speed of light */
   /* the resistive system (oracle_rmhd.c) with or without viscosity, with
    * divergence cleaning; ideal MHD with viscosity does not exist (the
    * reference refuses viscosity with MHD, common.f90:1904-1908) */
@@ -128,6 +129,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.smooth_first = 1;
   G.temp_end = 0.; G.check_max = 0.; G.freeze_type = 0; G.temp_check = 0;
   memset(G.nrep, 0, sizeof G.nrep);
+  G.crash_cell = -1;
   return G.nv;
 }
 
@@ -166,6 +168,8 @@ void orc_get_state(double *v, double *u) {
 void orc_get_deriv(double *d) { memcpy(d, G.deriv, ncell() * NDERIV * sizeof(double)); }
 
 void orc_get_repair_counts(long *out) { memcpy(out, G.nrep, sizeof G.nrep); }
+
+long orc_crash_cell(void) { return G.crash_cell; }
 
 int orc_get_repair_cells(long *out, int n) {
   int m = G.nrep[REP_U2V] < REPLOG ? (int)G.nrep[REP_U2V] : REPLOG;
@@ -513,7 +517,7 @@ static int evolve_u2v(void) {
           sys_cons2prim(uu, vv);
           v2 = vv[KVX] * vv[KVX] + vv[KVY] * vv[KVY] + vv[KVZ] * vv[KVZ] * g[2];
           e = ((vv[KPR] > 0) && (v2 < 1)) ? 0 : 1;
-          if (e) G.nrep[REP_U2V_FAIL]++;
+          if (e) { G.nrep[REP_U2V_FAIL]++; G.crash_cell = (long)c; }
         }
         if (G.viscosity && G.obtained_zz) G.v[KPIZZ * nc + c] = vv[KPIZZ];
         /* u is written before the error check (evolve.f90:848-849): a cell
@@ -593,8 +597,25 @@ static int ssp_rhs(int irk) {
   if (G.ny > 1 && sweep(2)) return G.errcode;
   if (G.nz > 1 && sweep(3)) return G.errcode;
   if (evolve_src()) { G.run_crashed = 1; return G.errcode; }
-  G.glm_ch_old = fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
+  G.glm_ch_old = G.rmhd ? 1. : fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
   return 0;
+}
+
+/* the stiff rate of the first IMEX stage of the resistive system: the
+ * explicit rate of K1 has weight zero in every later stage, so only
+ * du_stiff = -src_stiff is formed (cell by cell, no sweep) */
+static void ssp_stiff_only(void) {
+  const size_t nc = ncell(), n = nc * G.nu;
+  G.du = G.du_all;
+  G.du_stiff = G.dus_all;
+  memset(G.du, 0, n * sizeof(double));
+  memset(G.du_stiff, 0, n * sizeof(double));
+  double vv[MAXNV], s3[3];
+  for (size_t c = 0; c < nc; c++) {
+    for (int k = 0; k < G.nv; k++) vv[k] = G.v[k * nc + c];
+    rmhd_stiff(vv, s3);
+    for (int k = 0; k < 3; k++) G.du_stiff[(size_t)(G.kex + k) * nc + c] = -s3[k];
+  }
 }
 
 /* evolve_main for the IMEX-SSP3(4,3,3) scheme (nrk = SSP),
@@ -628,8 +649,10 @@ static int evolve_main_ssp(int irk) {
       int e = evolve_u2v();
       G.imex_adt = 0.;
       if (e) return e;
+      ssp_stiff_only();
+    } else {
+      if (ssp_rhs(1)) return G.errcode;
     }
-    if (ssp_rhs(1)) return G.errcode;
     if (G.divclean) evolve_glm_source(G.dt / 2.);
   } else if (irk == 2) {
     for (size_t i = 0; i < n; i++) G.u[i] = G.u0[i] + dt * alpha * ds1[i];
@@ -659,6 +682,9 @@ static int evolve_main_ssp(int irk) {
     if (G.divclean) evolve_glm_source(G.dt / 2.);
     if (evolve_u2v()) return G.errcode;
   }
+  /* the resistive system smooths the step's result only, not the states
+   * of its implicit stages */
+  if (G.rmhd && irk < 4) return 0;
   if (G.viscosity && G.enable_smooth_viscosity) {
     if (evolve_smooth_viscosity()) return G.errcode;
   }
@@ -683,7 +709,7 @@ static int evolve_main(int irk) {
   if (G.ny > 1 && sweep(2)) return G.errcode;
   if (G.nz > 1 && sweep(3)) return G.errcode;
   if (evolve_src()) { G.run_crashed = 1; return G.errcode; }
-  G.glm_ch_old = fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
+  G.glm_ch_old = G.rmhd ? 1. : fmax2(fmax2(G.aflux[0], G.aflux[1]), G.aflux[2]);
   if (irk == 1) {
     double dt_local = evolve_dt();
     dt_local = fmin2(dt_local, G.maxdt);

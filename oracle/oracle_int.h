@@ -58,6 +58,7 @@ typedef struct {
   /* cells (cell_index) of the first REPLOG u2v repairs since orc_setup */
 #define REPLOG 256
   long replog[REPLOG];
+  long crash_cell;   /* the cell whose con2prim repair failed, -1: none */
 } orc_state;
 
 /* repair branches of evolve_df{x,y,z} (evolve.f90:334-369) and
@@ -105,6 +106,7 @@ int rmhd_cons2prim(double *u, double *v);
 int rmhd_cons2prim_implicit(double *u, double *v, double adt);
 int rmhd_flux(double *v, double *u, double *f, double *vf, int dir);
 int rmhd_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff);
+void rmhd_stiff(const double *v, double *s3);
 void get_derived_pi(const double *p, double *pitt, double *pitx, double *pity, double *pitz);
 void get_derived_pi_zz(const double *p, double *pitt, double *pitx, double *pity, double *pitz, double *pizz);
 void system_metric(void);

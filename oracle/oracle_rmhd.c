@@ -307,17 +307,28 @@ int rmhd_source(int ix, int iy, int iz, double *v, double *src, double *src_stif
     const double eem = 0.5 * (E2 + B2);
     src[KPR] = src[KPR] + (-G.t * (eem / g[2] - E[2] * E[2] - B[2] * B[2])) * gp;
   }
+  const double rhoc = div_e(ix, iy, iz);
+  double s3[3];
+  rmhd_stiff(v, s3);
+  for (int k = 0; k < 3; k++) {
+    src[G.kex + k] = -rhoc * vel[k] * gp;
+    src_stiff[G.kex + k] = s3[k];
+  }
+  return 0;
+}
+
+/* the stiff (Ohm) current of the E equations per cell, times sqrt(-g):
+ * -gamma [sig (E - (v.E) v + v x B) + sch (B - (v.B) v - v x E)] */
+void rmhd_stiff(const double *v, double *s3) {
+  const double *g = G.g_cov;
+  const double *vel = &v[KVX], *B = &v[G.kbx], *E = &v[G.kex];
   const double v2 = g[0] * vel[0] * vel[0] + g[1] * vel[1] * vel[1] + g[2] * vel[2] * vel[2];
   const double glf = 1. / sqrt(1. - v2);
   const double sig = G.sigma_el / HBARC, sch = G.sigma_chi / HBARC;
-  const double rhoc = div_e(ix, iy, iz);
   double vxB[3], vxE[3];
   cross_contra(vel, B, vxB);
   cross_contra(vel, E, vxE);
   const double vE = dot3(vel, E), vB = dot3(vel, B);
-  for (int k = 0; k < 3; k++) {
-    src[G.kex + k] = -rhoc * vel[k] * gp;
-    src_stiff[G.kex + k] = -glf * (sig * (E[k] - vE * vel[k] + vxB[k]) + sch * (B[k] - vB * vel[k] - vxE[k])) * gp;
-  }
-  return 0;
+  for (int k = 0; k < 3; k++)
+    s3[k] = -glf * (sig * (E[k] - vE * vel[k] + vxB[k]) + sch * (B[k] - vB * vel[k] - vxE[k])) * G.gp;
 }

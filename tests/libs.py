@@ -197,6 +197,12 @@ class Oracle(_Lib):
         self.lib.orc_get_repair_counts(a)
         return dict(zip(self.REPAIRS, list(a)))
 
+    def crash_cell(self):
+        """the cell whose con2prim repair crashed the run (-1: none)"""
+        f = self.lib.orc_crash_cell
+        f.restype = C.c_long
+        return f()
+
     def repair_cells(self):
         """cells (x fastest) of the first 256 con2prim repairs since setup"""
         a = (C.c_long * 256)()

@@ -300,12 +300,14 @@ def resistive_layout(viscosity):
 
 
 def rmhd_fireball(nx=16, ny=14, nz=10, viscosity=1, eqstate=1, nrk=4, B0=0.3, E0=0.05, coordinates=2,
-                  sigma_el=0.0058, sigma_chi=0.0015, rec="WENO5", maxdt=1.0, bc=2, seed=0):
+                  sigma_el=0.0058, sigma_chi=0.0015, rec="WENO5", maxdt=1.0, bc=2, seed=0, smooth_temp=0.0,
+                  obtained_zz=1):
     """fireball3d with a transverse magnetic field and an electric field
     (not the ideal -v x B, so that the current flows), for the resistive
     system; viscous runs carry the fireball's shear tensor."""
     ic, rc, vf = fireball3d(nx, ny, nz, viscosity=viscosity, eqstate=eqstate, rec=rec, nrk=nrk,
-                            coordinates=coordinates, bc=bc, seed=seed)
+                            coordinates=coordinates, bc=bc, seed=seed, smooth_temp=smooth_temp,
+                            obtained_zz=obtained_zz)
     ic[6] = 2
     ic[7] = 1
     ic[12] = 7
@@ -362,4 +364,26 @@ def light_wave(n=64, amp=1e-3, nrk=3, rec="WENO5", sigma_el=0.0):
     v[4] = 100.0
     v[ke + 1, 0, 0] = amp * np.sin(2 * np.pi * x)
     v[kb + 2, 0, 0] = amp * np.sin(2 * np.pi * x)
+    return ic, rc, v
+
+
+def forced_resistive(viscosity=0, rec="TVD2", pbg=1e-2, nrk=4, sigma_el=0.0058, B0=0.05, E0=0.02):
+    """forced_background as a resistive grid (small B and E fields); the
+    caller forces superluminal conserved states with force_superluminal, so
+    that the implicit solves of the IMEX stages fail and are repaired."""
+    ic, rc, vf = forced_background(eqstate=1, viscosity=viscosity, rec=rec, pbg=pbg)
+    ic = ic.copy()
+    rc = rc.copy()
+    ic[6], ic[7], ic[10], ic[12] = 2, 1, nrk, 7
+    rc[14] = sigma_el
+    kb, ke, kg, nv = resistive_layout(viscosity)
+    v = np.zeros((nv,) + vf.shape[1:])
+    v[:kb] = vf
+    x = cell_centres(vf.shape[3], -1, 1)
+    y = cell_centres(vf.shape[2], -1, 1)
+    z = cell_centres(vf.shape[1], -1, 1)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    v[kb] = B0 * np.cos(Y)
+    v[kb + 1] = B0 * np.sin(X)
+    v[ke + 2] = E0 * np.cos(X + Y)
     return ic, rc, v
