@@ -43,13 +43,18 @@ typedef struct echo_config {
   int32_t coordinates;         /* [COORD] 1 Minkowski, 2 Bjorken (Milne) */
   int32_t viscosity;           /* [VISCOUS] 0/1 */
   int32_t bulk;                /* [BULK] 0/1 */
-  int32_t mhd;                 /* [MHD] 0 hydro, 1 ideal MHD (2 resistive:
-                                  ECHO_E_ARG, non-functional in the reference) */
+  int32_t mhd;                 /* [MHD] 0 hydro, 1 ideal MHD, 2 resistive
+                                  MHD (needs DIVCLEAN; with VISCOUS the
+                                  viscous resistive system, nv 19).  The
+                                  reference's resistive path does not run
+                                  (system.f90:1160-1162); this one is a
+                                  design of its own */
   int32_t divclean;            /* [DIVCLEAN] */
   int32_t eqstate;             /* [EQ_STATE] 1 p=e/3, 2 ideal gas, 3 table */
   int32_t recal;               /* [REC_ALGO] 1 TVD2 2 CENO3 3 WENO3 4 WENO5
                                   5 PPM4 6 MPE3 7 MPE5 */
-  int32_t nrk;                 /* [INT_ALGO] 2 RK2, 3 RK3 */
+  int32_t nrk;                 /* [INT_ALGO] 2 RK2, 3 RK3, 4 IMEX-SSP3(4,3,3)
+                                  (MHD = 2 only) */
   int32_t boundary_cond;       /* [BOUND_CON] 0 periodic, 2 outflow */
   int32_t algo_solver;         /* [ALG_SOLV] MHD con2prim: 4 ideal gas
                                   (EQ_STATE 2), 5 p=e/3 (EQ_STATE 1), 6 p=e/3
@@ -332,7 +337,10 @@ int echo_c2p_surrogate(echo_grid *g, echo_mlp *m, const double *d_u, double *d_v
 
 /* Kernel timing in milliseconds, measured with HIP events on the grid's
  * stream when enabled with echo_set_timing(g, 1).  After echo_steps (per
- * launch): t[0] flux x, t[1] flux y, t[2] flux z, t[3] cell update.  After
+ * launch): t[0] flux x, t[1] flux y, t[2] flux z (means over the stages
+ * that sweep: all RK stages, IMEX stages 2-4), t[3] cell update with its
+ * repair launch (mean over all stages; the first IMEX stage's segment holds
+ * both of its implicit solves).  After
  * echo_cons2prim_batch_dev: t[0] the solver kernel.  After
  * echo_c2p_surrogate: t[0] network + refinement kernel, t[1] fallback. */
 int echo_set_timing(echo_grid *g, int32_t on);

@@ -43,6 +43,8 @@ void orc_get_state(double *v, double *u);
 void orc_get_deriv(double *d);
 void orc_get_scalars(double *scal);
 int  orc_steps(int nsteps);
+/* MAXDT from now on (a test that follows a larger grid's time step) */
+void orc_set_maxdt(double maxdt);
 /* one RK stage irk (evolve_main) without the step bookkeeping */
 int  orc_stage(int irk);
 /* evolve_u2v alone on the current v, u */

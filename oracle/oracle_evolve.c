@@ -795,6 +795,8 @@ int orc_steps(int nsteps) {
   return 0;
 }
 
+void orc_set_maxdt(double maxdt) { G.maxdt = maxdt; }
+
 /* evolve_u2v alone on the current v, u (host-forced repair states) */
 int orc_u2v(void) {
   G.run_crashed = 0; G.errcode = 0;

@@ -197,6 +197,13 @@ class Oracle(_Lib):
         self.lib.orc_get_repair_counts(a)
         return dict(zip(self.REPAIRS, list(a)))
 
+    def set_maxdt(self, maxdt):
+        """MAXDT from now on"""
+        f = self.lib.orc_set_maxdt
+        f.restype = None
+        f.argtypes = [C.c_double]
+        f(float(maxdt))
+
     def crash_cell(self):
         """the cell whose con2prim repair crashed the run (-1: none)"""
         f = self.lib.orc_crash_cell

@@ -69,11 +69,12 @@ def test_reflection_symmetry_384x384x256():
 # ---------------------------------------------------------------------------
 # The unfloored Glauber-MC state of the large grid (echo_init_glauber,
 # ENEZERO 0.001).  Near the beam-rapidity cut of the deposition (|eta| ->
-# ybeam = 5.36) the energy density falls to the vacuum value and the
-# Navier-Stokes stresses exceed the pressure; there the reference's own
-# scheme (the oracle, which matches it bit for bit on the repair paths)
-# repairs hundreds of cells per step and in the second step a con2prim
-# repair fails, which stops the run (evolve.f90:836-840).
+# ybeam = 5.36) the energy density falls to the vacuum value, the
+# Navier-Stokes stresses exceed the pressure and v^eta approaches 1.  The
+# first step needs no con2prim repair; in the second, thousands of cells are
+# repaired and one repair fails, which stops the run (evolve.f90:836-840).
+# The oracle (which matches the reference bit for bit on the repair paths)
+# shows the same on the cells around the failure.
 # ---------------------------------------------------------------------------
 def _glauber_state(ic, rc, pa, pb, pc):
     from libs import Oracle, oracle_glauber_prim
@@ -85,19 +86,20 @@ def _glauber_state(ic, rc, pa, pb, pc):
 
 
 def test_glauber_rapidity_tail_slab_vs_oracle():
-    """An eta slab of the large grid around eta = 5.2 (384 x 384 x 8 cells):
-    step 1 with its repairs against the oracle, then the failed repair of
-    step 2 with the same error code, the oracle's crashing cell in the
-    message, and the cells before it (in the reference's loop order) equal."""
+    """An eta slab of the large grid around eta = 5.2 (384 x 384 x 16 cells,
+    outflow boundaries of its own): step 1 with its repairs against the
+    oracle, then the failed repair of step 2 with the same error code, the
+    oracle's crashing cell in the message, and the cells before it (in the
+    reference's loop order) equal."""
     from echoamd import EchoError, EchoGrid, read_eos_rows
     from scenarios import rel_linf
-    ic, rc = pbpb_config(nz=8, zc=5.2)
+    ic, rc = pbpb_config(nz=16, zc=5.2)
     pa, pb, pc = glauber_event(seed=5)
     o, v0 = _glauber_state(ic, rc, pa, pb, pc)
     assert o.init_from_prim(v0) == 0
     g = EchoGrid(ic, rc, eos_rows=read_eos_rows())
     assert g.init_glauber(GLAUBER_PAR, pa, pb, pc) == 0
-    print("\nslab 384x384x8 at eta 5.2: step 1", flush=True)
+    print("\nslab 384x384x16 at eta 5.2: step 1", flush=True)
     assert o.steps(1) == 0
     g.steps(1)
     n1 = o.repair_counts()["u2v"]
@@ -109,12 +111,12 @@ def test_glauber_rapidity_tail_slab_vs_oracle():
     assert o.steps(1) == 1
     cnt = o.repair_counts()
     assert cnt["u2v_fail"] == 1
-    crash = o.repair_cells()[cnt["u2v"] - 1] if cnt["u2v"] <= 256 else None
+    crash = o.crash_cell()
     with pytest.raises(EchoError) as ei:
         g.steps(1)
     assert ei.value.code == 1 and "repair failed" in str(ei.value)
     print(str(ei.value), flush=True)
-    if crash is not None:
+    if crash >= 0:
         nx, ny = int(ic[0]), int(ic[1])
         assert f"ix={crash % nx + 1} iy={(crash // nx) % ny + 1} iz={crash // (nx * ny) + 1}" in str(ei.value)
         vg, _ = g.state()
@@ -126,24 +128,68 @@ def test_glauber_rapidity_tail_slab_vs_oracle():
     g.close()
 
 
-def test_glauber_unfloored_384x384x256():
-    """The whole grid from echo_init_glauber's unfloored state: the first
-    step runs (with repairs), and within three steps the run stops on a
-    failed con2prim repair, reported as such, as the oracle's slab run of
-    the rapidity tail does."""
+def test_glauber_unfloored_384x384x256_vs_oracle_box():
+    """The whole grid from echo_init_glauber's unfloored state on the GPU:
+    step 1 runs without repairs, step 2 stops on a failed con2prim repair,
+    reported with its cell.  The oracle runs the 52^3-cell box around that
+    cell (outflow boundaries; a face on the grid's own boundary is the same
+    boundary) from its own Glauber state, with the GPU's time steps (its own
+    CFL step is not smaller: the box holds a subset of the grid's speeds).
+    After step 1 the box's interior (9 cells = 3 stages x 3 stencil cells
+    away from the box's inner faces) equals the GPU's grid there; in step 2
+    the oracle's run fails at the same cell."""
+    import re
     from echoamd import EchoError, EchoGrid, read_eos_rows
+    from libs import make_config
+    from scenarios import rel_linf
     ic, rc = pbpb_config()
+    nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
     pa, pb, pc = glauber_event(seed=5)
     g = EchoGrid(ic, rc, eos_rows=read_eos_rows())
     assert g.init_glauber(GLAUBER_PAR, pa, pb, pc) == 0
     print("\nfull size 384x384x256, unfloored Glauber state: step 1", flush=True)
     g.steps(1)
-    assert g.scalars()["crashed"] == 0
-    n1 = g.repairs()
-    print(f"repaired cells in step 1: {n1}", flush=True)
-    assert n1 > 0
+    assert g.scalars()["crashed"] == 0 and g.repairs() == 0
+    dt1 = g.scalars()["dt"]
+    v1, _ = g.state()
+    print("step 2", flush=True)
     with pytest.raises(EchoError) as ei:
-        g.steps(2)
-    print(str(ei.value), f"repaired cells: {g.repairs()}", flush=True)
-    assert ei.value.code == 1 and "repair failed" in str(ei.value)
+        g.steps(1)
+    msg = str(ei.value)
+    print(msg, f"repaired cells: {g.repairs()}", flush=True)
+    assert ei.value.code == 1 and "repair failed" in msg
+    m = re.search(r"ix=(\d+) iy=(\d+) iz=(\d+)", msg)
+    cell = (int(m.group(1)) - 1, int(m.group(2)) - 1, int(m.group(3)) - 1)
+    dt2 = g.scalars()["dt"]
     g.close()
+    # the oracle's box around the failing cell
+    H = 26
+    lo = [min(max(c - H, 0), n - 2 * H) for c, n in zip(cell, (nx, ny, nz))]
+    x1, y1, z1 = rc[0], rc[2], rc[4]
+    dx, dy, dz = (rc[1] - rc[0]) / nx, (rc[3] - rc[2]) / ny, (rc[5] - rc[4]) / nz
+    icb, rcb = make_config(nx=2 * H, ny=2 * H, nz=2 * H, coordinates=2, viscosity=1, bulk=0, eqstate=3,
+                           rec="WENO5", nrk=3, bc=2, obtained_zz=1, init_type=3,
+                           xlim=(x1 + lo[0] * dx, x1 + (lo[0] + 2 * H) * dx),
+                           ylim=(y1 + lo[1] * dy, y1 + (lo[1] + 2 * H) * dy),
+                           zlim=(z1 + lo[2] * dz, z1 + (lo[2] + 2 * H) * dz),
+                           tmin=rc[6], cfl=rc[7], maxdt=dt1, eta_over_s=rc[10], tau_pi_coeff=rc[11],
+                           enezero=rc[12])
+    o, v0 = _glauber_state(icb, rcb, pa, pb, pc)
+    assert o.init_from_prim(v0) == 0
+    assert o.steps(1) == 0 and o.scalars()["dt"] == dt1
+    vo, _ = o.state()
+    inner = []
+    for d, n in zip(range(3), (nx, ny, nz)):
+        a = 0 if lo[d] == 0 else 9
+        b = 2 * H if lo[d] + 2 * H == n else 2 * H - 9
+        inner.append(slice(a, b))
+    box = v1[:, lo[2]:lo[2] + 2 * H, lo[1]:lo[1] + 2 * H, lo[0]:lo[0] + 2 * H]
+    sel = (slice(None), inner[2], inner[1], inner[0])
+    err = rel_linf(box[sel], vo[sel])
+    print(f"box at {lo}: step 1 rel L-inf {err:.2e}", flush=True)
+    assert err <= 1e-10
+    o.set_maxdt(dt2)
+    assert o.steps(1) == 1 and o.repair_counts()["u2v_fail"] == 1
+    c = o.crash_cell()
+    got = (lo[0] + c % (2 * H), lo[1] + (c // (2 * H)) % (2 * H), lo[2] + c // (4 * H * H))
+    assert got == cell, (got, cell)
