@@ -5,6 +5,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "grid.hpp"
 #include "mlp.hpp"
 
@@ -270,20 +272,13 @@ int echo_cons2prim_batch_dev(echo_grid* g, const double* d_u, double* d_v, int32
   return 0;
 }
 
-int echo_c2p_surrogate(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v, int32_t* d_status,
-                       int64_t n, int32_t iters, double tol, int64_t* n_fallback) {
-  if (!g || !m || !d_u || !d_v || !d_status || n < 0) return fail(ECHO_E_ARG, "invalid argument");
-  if (g->dev != m->dev) return fail(ECHO_E_ARG, "network and grid on different devices");
-  if (!surrogate_config_ok(g->cfg))
-    return fail(ECHO_E_ARG, "the learned inversion covers ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1)");
-  if (iters < 0 || iters > 8 || !(tol > 0.0)) return fail(ECHO_E_ARG, "need 0 <= iters <= 8 and tol > 0");
-  if (n > 0x7fffffffLL) return fail(ECHO_E_ARG, "at most 2^31-1 states per call");
-  if (n_fallback) *n_fallback = 0;
-  if (n == 0) return 0;
-  HIPCHECK(hipSetDevice(g->dev));
-  if (int e = ensure_fail_cap(m, n)) return e;
-  mlp::k_zero_int<<<1, 64, 0, g->stream>>>(m->fail_count, 1);
-  mlp::C2pArgs ca{g->cfg.coordinates, iters, tol, g->S, m->fail_count, m->fail_list};
+// the learned inversion with the fallback counter and list at fail_count,
+// fail_list (room for n entries); shared by both entry points below
+static int surrogate_run(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v, int32_t* d_status,
+                         int64_t n, int32_t iters, double tol, int64_t* n_fallback, int* fail_count,
+                         int* fail_list) {
+  mlp::k_zero_int<<<1, 64, 0, g->stream>>>(fail_count, 1);
+  mlp::C2pArgs ca{g->cfg.coordinates, iters, tol, g->S, fail_count, fail_list};
   const int nb = mlp_grid_blocks(m->dev, n);
   if (g->timing) HIPCHECK(hipEventRecord(g->ev[0], g->stream));
   if (g->cfg.divclean)
@@ -297,15 +292,15 @@ int echo_c2p_surrogate(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v
   int ncu = 256;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev);
   if (g->cfg.divclean)
-    mlp::k_c2p_fallback<SYS_MHDGLM><<<ncu, 256, 0, g->stream>>>(g->ga, d_u, d_v, d_status, n, m->fail_list,
-                                                                m->fail_count);
+    mlp::k_c2p_fallback<SYS_MHDGLM><<<ncu, 256, 0, g->stream>>>(g->ga, d_u, d_v, d_status, n, fail_list,
+                                                                fail_count);
   else
-    mlp::k_c2p_fallback<SYS_MHD><<<ncu, 256, 0, g->stream>>>(g->ga, d_u, d_v, d_status, n, m->fail_list,
-                                                             m->fail_count);
+    mlp::k_c2p_fallback<SYS_MHD><<<ncu, 256, 0, g->stream>>>(g->ga, d_u, d_v, d_status, n, fail_list,
+                                                             fail_count);
   HIPCHECK(hipGetLastError());
   if (g->timing) HIPCHECK(hipEventRecord(g->ev[2], g->stream));
   int nf = 0;
-  HIPCHECK(hipMemcpyAsync(&nf, m->fail_count, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipMemcpyAsync(&nf, fail_count, sizeof(int), hipMemcpyDeviceToHost, g->stream));
   HIPCHECK(hipStreamSynchronize(g->stream));
   if (n_fallback) *n_fallback = nf;
   if (g->timing) {
@@ -317,6 +312,74 @@ int echo_c2p_surrogate(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v
     g->tms[2] = g->tms[3] = 0.0;
   }
   return 0;
+}
+
+static int surrogate_check(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v, int32_t* d_status,
+                           int64_t n, int32_t iters, double tol) {
+  if (!g || !m || !d_u || !d_v || !d_status || n < 0) return fail(ECHO_E_ARG, "invalid argument");
+  if (g->dev != m->dev) return fail(ECHO_E_ARG, "network and grid on different devices");
+  if (!surrogate_config_ok(g->cfg))
+    return fail(ECHO_E_ARG, "the learned inversion covers ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1)");
+  if (iters < 0 || iters > 8 || !(tol > 0.0)) return fail(ECHO_E_ARG, "need 0 <= iters <= 8 and tol > 0");
+  if (n > 0x7fffffffLL) return fail(ECHO_E_ARG, "at most 2^31-1 states per call");
+  return 0;
+}
+
+int echo_c2p_surrogate(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v, int32_t* d_status,
+                       int64_t n, int32_t iters, double tol, int64_t* n_fallback) {
+  if (int e = surrogate_check(g, m, d_u, d_v, d_status, n, iters, tol)) return e;
+  if (n_fallback) *n_fallback = 0;
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(g->dev));
+  if (int e = ensure_fail_cap(m, n)) return e;
+  return surrogate_run(g, m, d_u, d_v, d_status, n, iters, tol, n_fallback, m->fail_count, m->fail_list);
+}
+
+}  // extern "C"
+
+// ---- the allocation-free variant of echo_c2p_surrogate --------------------
+// Its scratch (the fallback counter in word 0, the fallback list after it)
+// is this file-scope device array, one copy per device, held until the
+// process ends.  On the host the name is a stand-in, not a device address:
+// scratch_on() takes the address of each device's copy once, with
+// hipGetSymbolAddress while that device is current, and every use goes
+// through that address.
+static __device__ __align__(256) int scratch[1 + ECHO_NOALLOC_MAX_STATES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+int* scratch_dev[MAX_DEVICES] = {};
+
+int scratch_on(int dev, int** out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(ECHO_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ECHO_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    scratch_dev[dev] = static_cast<int*>(p);
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int echo_c2p_surrogate_noalloc(echo_grid* g, echo_mlp* m, const double* d_u, double* d_v, int32_t* d_status,
+                               int64_t n, int32_t iters, double tol, int64_t* n_fallback) {
+  if (int e = surrogate_check(g, m, d_u, d_v, d_status, n, iters, tol)) return e;
+  if (n > ECHO_NOALLOC_MAX_STATES)
+    return fail(ECHO_E_ARG, "echo_c2p_surrogate_noalloc: at most %d states per call (ECHO_NOALLOC_MAX_STATES), "
+                            "got %lld", ECHO_NOALLOC_MAX_STATES, (long long)n);
+  if (n_fallback) *n_fallback = 0;
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(g->dev));
+  int* sc = nullptr;
+  if (int e = scratch_on(g->dev, &sc)) return e;
+  return surrogate_run(g, m, d_u, d_v, d_status, n, iters, tol, n_fallback, sc, sc + 1);
 }
 
 // ---- device memory for callers without a runtime of their own ------------

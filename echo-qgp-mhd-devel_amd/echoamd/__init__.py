@@ -29,7 +29,7 @@ EXPORTS = (
     "echo_set_timing", "echo_get_timing", "echo_last_error",
     "echo_cons2prim_batch_dev", "echo_mlp_create", "echo_mlp_destroy", "echo_mlp_set_params",
     "echo_mlp_get_params", "echo_mlp_forward", "echo_mlp_grad", "echo_mlp_train",
-    "echo_c2p_surrogate", "echo_set_tile", "echo_get_tile", "echo_tune",
+    "echo_c2p_surrogate", "echo_c2p_surrogate_noalloc", "echo_set_tile", "echo_get_tile", "echo_tune",
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
@@ -138,6 +138,8 @@ def load_library(path=LIB_PATH):
         "echo_mlp_train": (C.c_int, [P, P, P, P, C.c_int64, C.POINTER(TrainConfig), P]),
         "echo_c2p_surrogate": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
                                          C.POINTER(C.c_int64)]),
+        "echo_c2p_surrogate_noalloc": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
+                                                 C.POINTER(C.c_int64)]),
         "echo_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
         "echo_get_tile": (C.c_int, [P, _ip]),
         "echo_tune": (C.c_int, [P, C.c_int32, _dp]),
@@ -534,6 +536,14 @@ class C2pNet:
         nf = C.c_int64()
         self._check(self.lib.echo_c2p_surrogate(grid.h, self.h, d_u, d_v, d_status, int(n),
                                                 int(iters), float(tol), C.byref(nf)))
+        return nf.value
+
+    def invert_noalloc(self, grid, d_u, d_v, d_status, n, iters=2, tol=1e-11):
+        """invert() through the allocation-free entry point (its scratch is a
+        static device array; n <= ECHO_NOALLOC_MAX_STATES = 2^22)."""
+        nf = C.c_int64()
+        self._check(self.lib.echo_c2p_surrogate_noalloc(grid.h, self.h, d_u, d_v, d_status, int(n),
+                                                        int(iters), float(tol), C.byref(nf)))
         return nf.value
 
 

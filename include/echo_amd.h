@@ -335,6 +335,19 @@ int echo_c2p_surrogate(echo_grid *g, echo_mlp *m, const double *d_u, double *d_v
                        int32_t *d_status, int64_t n, int32_t iters, double tol,
                        int64_t *n_fallback);
 
+/* echo_c2p_surrogate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The fallback
+ * counter and list live in a static device array of the library, one per
+ * device (4 + 4 * ECHO_NOALLOC_MAX_STATES bytes, about 16 MB, held until the
+ * process ends); n above ECHO_NOALLOC_MAX_STATES is rejected with ECHO_E_ARG.
+ * Results equal echo_c2p_surrogate's.  Every call on a device shares that
+ * device's array: calls that overlap on one device race. */
+#define ECHO_NOALLOC_MAX_STATES (1 << 22)
+int echo_c2p_surrogate_noalloc(echo_grid *g, echo_mlp *m, const double *d_u, double *d_v,
+                               int32_t *d_status, int64_t n, int32_t iters, double tol,
+                               int64_t *n_fallback);
+
 /* Kernel timing in milliseconds, measured with HIP events on the grid's
  * stream when enabled with echo_set_timing(g, 1).  After echo_steps (per
  * launch): t[0] flux x, t[1] flux y, t[2] flux z (means over the stages
