@@ -1,17 +1,21 @@
-"""Benchmark of the MI355X grid update: zone-updates/s of full RK3 steps.
+"""Benchmark of the MI355X grid update: zone-updates/s of full time steps.
 
 Workload (one GPU): the 192 x 192 x 128 (x, y, eta) Milne grid of the
 single-event Au-Au run (dx = dy = 0.15 fm, deta = 0.1, tau0 = 0.4 fm,
-tabulated lattice-QCD EoS from qcdIEOS0.dat, eta/s = 0.1, zeta/s = 0,
-WENO5 + HLL + RK3, CFL 0.4), evolved as 3+1D second-order viscous hydro.
-The initial energy density is a seeded Monte-Carlo Glauber event (Woods-Saxon
-R = 6.38 fm, a = 0.535 fm, sigma_NN = 42 mb, b = 7 fm, Gaussian smearing
-0.4 fm) with a boost-invariant plateau and Gaussian tails in eta, and a
-Navier-Stokes initial shear tensor; all synthetic (no files read).
+tabulated lattice-QCD EoS from qcdIEOS0.dat, eta/s = 0.1, zeta/s = 0),
+evolved as 3+1D viscous resistive MHD with divergence cleaning (nv = 19:
+rho, v, p, Pi, pi^{mu nu}, B, E, psi), WENO5 + HLL, IMEX-SSP3(4,3,3) with
+the Ohmic current implicit (sigma_el = 5.8 MeV), CFL 0.4.  The initial
+energy density is a seeded Monte-Carlo Glauber event (Woods-Saxon R = 6.38
+fm, a = 0.535 fm, sigma_NN = 42 mb, b = 7 fm, Gaussian smearing 0.4 fm) with
+a boost-invariant plateau and Gaussian tails in eta and a Navier-Stokes
+initial shear tensor; the magnetic field is a divergence-free dipole-like
+field with eB_y = 0.1 m_pi^2 at the centre and a Gaussian transverse width of
+3 fm; E = 0 (the fluid starts at rest).  All synthetic (no files read).
 
-One zone-update = one cell advanced through one full time step (3 RK
-stages).  value = cells * steps / wall time of the timed steps, with the
-state resident in HBM; each step is one replay of a captured HIP graph.
+One zone-update = one cell advanced through one full time step (all four
+IMEX stages).  value = cells * steps / wall time of the timed steps, with
+the state resident in HBM; each step is one replay of a captured HIP graph.
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -32,15 +36,35 @@ HBAR = 0.197326
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_config(nx=192, ny=192, nz=128):
+# eB = 0.1 m_pi^2 in code units: B / (Q (hbar c)^{3/2}), Q = 0.30282212
+# (common.f90:51-55)
+M_PI = 0.13957
+B_PEAK = 0.1 * M_PI ** 2 / (0.30282212 * HBAR ** 1.5)
+B_WIDTH = 3.0
+
+
+def workload_config(nx=192, ny=192, nz=128, nrk=4, resistive=True):
     from libs import make_config
     dx, deta = 0.15, 0.1
     ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=0, eqstate=3,
-                         rec="WENO5", nrk=3, bc=2, obtained_zz=1, init_type=3,
+                         rec="WENO5", nrk=nrk, bc=2, obtained_zz=1, init_type=3,
+                         mhd=2 if resistive else 0, divclean=1 if resistive else 0, algo_solver=7,
                          xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
                          zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=0.1,
-                         eta_over_s=0.1, tau_pi_coeff=5.0, enezero=0.001)
+                         eta_over_s=0.1, tau_pi_coeff=5.0, enezero=0.001, sigma_el=0.0058,
+                         sigma_chi=0.0)
     return ic, rc
+
+
+def dipole_field(x, y, b0=B_PEAK, w=B_WIDTH):
+    """B = curl (A_z e_z) with A_z = -b0 w sqrt(pi/2) erf(x / (sqrt(2) w))
+    exp(-y^2 / 2w^2): B_y = -dA_z/dx = b0 exp(-(x^2 + y^2) / 2w^2) and the
+    return field B_x = dA_z/dy (divergence-free; uniform in eta)."""
+    from scipy.special import erf
+    g = np.exp(-y ** 2 / (2 * w ** 2))
+    by = b0 * np.exp(-x ** 2 / (2 * w ** 2)) * g
+    bx = b0 * w * np.sqrt(np.pi / 2) * erf(x / (np.sqrt(2) * w)) * (y / w ** 2) * g
+    return bx, by
 
 
 def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_nn=4.2,
@@ -91,38 +115,65 @@ def workload_state(ic, rc, eos_rows, seed=12345, b=7.0):
     temp = np.interp(e, te, T)
     s = (e + p) / temp
     eta_v = HBAR * rc[10] * s
-    v = np.zeros((12, nz, ny, nx))
+    nv = 19 if int(ic[6]) == 2 else 12
+    v = np.zeros((nv, nz, ny, nx))
     v[0] = 1.0
     v[4] = p
     # Navier-Stokes initial shear of viscous_initio (viscous.f90:101-136)
     v[9] = 2. * eta_v / (3. * tau0)
     v[10] = 2. * eta_v / (3. * tau0)
     v[11] = -tau0 ** 2 * 4. * eta_v / (3. * tau0)
+    if nv == 19:
+        x = (np.arange(nx) + 0.5) * dx - nx * dx / 2
+        y = (np.arange(ny) + 0.5) * dx - ny * dx / 2
+        X, Y = np.meshgrid(x, y, indexing="xy")
+        bx, by = dipole_field(X, Y)
+        v[12] = bx[None]
+        v[13] = by[None]
     return v, npart
 
 
-def algorithmic_bytes(nx, ny, nz, nv, visc):
+# Compulsory FP64 state traffic of a step, 8 nv P bytes per cell (halos
+# counted once, EoS table cache-resident): P = 34 array passes for the
+# IMEX-SSP3(4,3,3) stages with stored stage rates, P = 14 for RK3 with the
+# inversion fused into each stage's update.
+P_STEP = {4: 34, 3: 14}
+
+
+def algorithmic_bytes(nx, ny, nz, nv, nrk):
+    """Compulsory bytes per launch: a sweep reads the primitives once (its
+    face fluxes are intermediate data of this design, not counted); a cell
+    update (a stage after the first) reads V, u^n and one accumulator (RK:
+    u of the stage) and writes the accumulator and V, plus the two lagged
+    velocity snapshots of the viscous time derivatives; the step: 8 nv P N."""
+    N = nx * ny * nz
+    sweep = 8 * nv * N
+    upd = 8 * (5 * nv * N + 6 * N)
+    return sweep, upd, 8 * nv * P_STEP[nrk] * N
+
+
+def design_bytes(nx, ny, nz, nv, resistive):
+    """Bytes per launch as the kernels are built: a sweep reads V and writes
+    its face fluxes (nv per face, one more for div E with resistive MHD) and
+    the four face sums of the Lorentz velocity; the update reads all of them."""
     N = nx * ny * nz
     nf = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
-    lw = 4 if visc else 0
-    flux = [8 * (nv * N + (nv + lw) * f) for f in nf]
-    # stage >= 2: read F and L of 3 dirs, V, u0, u_cur (+ 2 x 3 velocity
-    # snapshots for dts); write u_out and V
-    upd = 8 * (sum((nv + lw) * f for f in nf) + nv * N * 3 + (6 * N if visc else 0) + 2 * nv * N)
+    nfv = nv + (1 if resistive else 0) + 4
+    flux = [8 * (nv * N + nfv * f) for f in nf]
+    upd = 8 * (sum(nfv * f for f in nf) + 5 * nv * N + 6 * N)
     return flux, upd
 
 
-def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=32, steps=1):
-    """Reference Fortran (oracle/_ref) or, if absent, the C oracle, on a
-    192x192xnz_sample slab of the same workload: one thread."""
-    from libs import Oracle, Ref, ref_available
+def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=16, steps=1):
+    """The C oracle (the FP64 restatement, oracle/liboracle.so), single
+    thread, on a 192 x 192 x nz_sample slab of the same workload."""
+    from libs import Oracle
     ic = ic_full.copy()
     rc = rc_full.copy()
     ic[2] = nz_sample
     deta = (rc_full[5] - rc_full[4]) / ic_full[2]
     rc[4], rc[5] = -nz_sample * deta / 2, nz_sample * deta / 2
-    kind = "reference" if ref_available() else "port"
-    lib = Ref() if kind == "reference" else Oracle()
+    lib = Oracle()
     lib.setup(ic, rc)
     v, _ = workload_state(ic, rc, eos_rows)
     lib.init_from_prim(v)
@@ -130,17 +181,17 @@ def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=32, steps=1):
     e = lib.steps(steps)
     dt = time.perf_counter() - t0
     cells = int(ic[0]) * int(ic[1]) * int(ic[2])
-    return {"value": cells * steps / dt, "unit": "zone-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{int(ic[0])}x{int(ic[1])}x{nz_sample} slab of the same workload, "
-                      f"{steps} RK3 step(s), {dt:.1f} s, single thread"
+    return {"value": cells * steps / dt, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{int(ic[0])}x{int(ic[1])}x{nz_sample} slab of the same workload (nv {lib.nv}, "
+                      f"INT_ALGO {int(ic[10])}), {steps} step(s), {dt:.1f} s, single thread, C oracle"
                       + ("" if e == 0 else f", crashed with code {e}")}
 
 
-def read_traffic(kernel, nx, ny, nz):
+def read_traffic(kernel, nx, ny, nz, tag="vrmhd_imex"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (profiles/hbm_traffic_<nx>x<ny>x<nz>.txt: lines '<kernel> <bytes>'), or
-    None when no summary covers this workload."""
-    path = os.path.join(REPO, "profiles", f"hbm_traffic_{nx}x{ny}x{nz}.txt")
+    (profiles/hbm_traffic_<tag>_<nx>x<ny>x<nz>.txt: lines '<kernel> <bytes>'),
+    or None when no summary covers this workload."""
+    path = os.path.join(REPO, "profiles", f"hbm_traffic_{tag}_{nx}x{ny}x{nz}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -222,9 +273,10 @@ def c2p_leg(n_states, seed=99):
 def ensemble_leg(n_events, steps, eos_rows, seed=2026):
     """Event-by-event ensemble: n_events Au-Au events with impact parameter
     b ~ U[0, 12] fm, each on a grid sized to its fireball (transverse
-    96^2..192^2 cells of 0.15 fm, 64..128 eta cells of 0.1), viscous hydro as
-    the single-event run, all advanced together (one captured graph per step,
-    autotuned tiles).  Value: total cells x steps / wall time."""
+    96^2..192^2 cells of 0.15 fm, 64..128 eta cells of 0.1), viscous
+    resistive MHD as the single-event run with RK3, all advanced together
+    (one captured graph per step, autotuned tiles).  Value: total cells x
+    steps / wall time."""
     from echoamd import Ensemble
     rng = np.random.default_rng(seed)
     bs = rng.uniform(0.0, 12.0, n_events)
@@ -233,7 +285,7 @@ def ensemble_leg(n_events, steps, eos_rows, seed=2026):
         f = 1.0 - b / 12.0
         nt = int(round((96 + 96 * f) / 16)) * 16
         ne = int(round((64 + 64 * f) / 16)) * 16
-        ic, rc = workload_config(nt, nt, ne)
+        ic, rc = workload_config(nt, nt, ne, nrk=3)
         v, _ = workload_state(ic, rc, eos_rows, seed=seed + k, b=b)
         cfgs.append((ic, rc))
         states.append(v)
@@ -271,16 +323,43 @@ def ensemble_leg(n_events, steps, eos_rows, seed=2026):
             "tiles_chosen": {str(t): tiles.count(t) for t in map(list, {tuple(t) for t in tiles})}}
 
 
+def time_steps(g, steps, warmup):
+    g.steps(warmup)
+    t0 = time.perf_counter()
+    g.steps(steps)          # returns after the device has finished
+    return time.perf_counter() - t0
+
+
+def side_leg(nx, ny, nz, eos_rows, steps, **kw):
+    """ms per step of a variant of the workload (graph replays)."""
+    from echoamd import EchoGrid
+    ic, rc = workload_config(nx, ny, nz, **kw)
+    v0, _ = workload_state(ic, rc, eos_rows)
+    g = EchoGrid(ic, rc, device=0, eos_rows=eos_rows)
+    g.init_from_prim(v0)
+    del v0
+    g.steps(1)
+    g.tune(reps=3)
+    wall = time_steps(g, steps, 2)
+    N = nx * ny * nz
+    crashed = g.scalars()["crashed"]
+    g.close()
+    return {"nv": 19 if kw.get("resistive", True) else 12, "int_algo": kw.get("nrk", 4),
+            "ms_per_step": 1e3 * wall / steps, "zone_updates_per_s": N * steps / wall,
+            "crashed": crashed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=192)
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--no-legs", action="store_true", help="only the headline (no side legs)")
     ap.add_argument("--c2p-states", type=int, default=1 << 24,
                     help="states of the learned-con2prim leg (0: skip)")
     ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
@@ -299,49 +378,62 @@ def main():
     v0, npart = workload_state(ic, rc, eos_rows)
     g = EchoGrid(ic, rc, device=0, eos_rows=eos_rows)
     g.init_from_prim(v0)
-    nx, ny, nz, nv = args.nx, args.ny, args.nz, g.nv
+    del v0
+    nx, ny, nz, nv, nrk = args.nx, args.ny, args.nz, g.nv, int(ic[10])
     N = nx * ny * nz
+    print(f"bench: {nx}x{ny}x{nz} nv {nv} INT_ALGO {nrk}: {args.steps} timed steps", file=sys.stderr,
+          flush=True)
     g.steps(1)
     tune_ms = None if args.no_tune else g.tune(reps=3)
     if args.plain:
         g.set_timing(True)
-    g.steps(args.warmup)
-    t0 = time.perf_counter()
-    g.steps(args.steps)          # returns after the device has finished
-    wall = time.perf_counter() - t0
+    wall = time_steps(g, args.steps, args.warmup)
     sc = g.scalars()
     # per-kernel durations, HIP events on the grid's stream (plain launches)
     g.set_timing(True)
     g.steps(3)
     tk = g.timing()
     g.set_timing(False)
-    flux_b, upd_b = algorithmic_bytes(nx, ny, nz, nv, True)
-    kern = {"flux_x": (tk["flux_x"], flux_b[0]), "flux_y": (tk["flux_y"], flux_b[1]),
-            "flux_z": (tk["flux_z"], flux_b[2]), "update": (tk["update"], upd_b)}
+    sweep_b, upd_b, step_b = algorithmic_bytes(nx, ny, nz, nv, nrk)
+    dflux, dupd = design_bytes(nx, ny, nz, nv, True)
+    kern = {"flux_x": (tk["flux_x"], sweep_b, dflux[0]), "flux_y": (tk["flux_y"], sweep_b, dflux[1]),
+            "flux_z": (tk["flux_z"], sweep_b, dflux[2]), "update": (tk["update"], upd_b, dupd)}
     dom = max(kern, key=lambda k: kern[k][0])
-    ms, byt = kern[dom]
+    ms, byt, dbyt = kern[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     value = N * args.steps / wall
+    ms_step = 1e3 * wall / args.steps
+    step_gbs = step_b / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded MC-Glauber initial states; seeded con2prim states)",
-        "config": {"workload": f"single-event Au-Au grid {nx}x{ny}x{nz} (x,y,eta), 3+1D "
-                               "second-order viscous hydro (eta/s=0.1), tabulated EoS, "
-                               "WENO5+HLL+RK3; magnetic/resistive terms not in this run",
-                   "cells": N, "nv": nv, "stages_per_step": 3, "participants": npart,
-                   "t_end": sc["t"], "tiles": g.tile().tolist()},
-        "roofline": {"kernel": f"k_flux/k_update ({dom})", "bound": "hbm",
+        "data": "synthetic (seeded MC-Glauber initial state and dipole B field; seeded con2prim states)",
+        "config": {"workload": f"single-event Au-Au grid {nx}x{ny}x{nz} (x,y,eta), 3+1D viscous resistive "
+                               "MHD (eta/s=0.1, sigma_el=5.8 MeV, eB_y=0.1 m_pi^2 dipole), tabulated EoS, "
+                               "WENO5+HLL, IMEX-SSP3(4,3,3)",
+                   "cells": N, "nv": nv, "stages_per_step": 4, "sweeping_stages_per_step": 3,
+                   "participants": npart, "t_end": sc["t"], "crashed": sc["crashed"],
+                   "tiles": g.tile().tolist()},
+        "roofline": {"kernel": f"k_flux/k_update_imex ({dom})", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(dom, nx, ny, nz),
+                     "bytes_basis": "compulsory state traffic (a sweep reads V once; an update reads V, "
+                                    "u^n, an accumulator, writes the accumulator and V)",
                      "ms_per_launch": {k: kern[k][0] for k in kern},
-                     "algorithmic_bytes_per_launch": {k: kern[k][1] for k in kern}},
+                     "algorithmic_bytes_per_launch": {k: kern[k][1] for k in kern},
+                     "design_bytes_per_launch": {k: kern[k][2] for k in kern},
+                     "design_frac": dbyt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS if ms > 0 else 0.0,
+                     "step": {"P": P_STEP[nrk], "bytes": step_b, "achieved": step_gbs,
+                              "frac": step_gbs / HBM_PEAK_GBS}},
     }
     if tune_ms is not None:
         out["config"]["tune_ms_per_launch"] = {"y": tune_ms[1].tolist(), "z": tune_ms[2].tolist()}
     g.close()
+    if not args.no_legs:
+        out["rk3_resistive"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3)
+        out["viscous_hydro_rk3"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3, resistive=False)
     if args.c2p_states > 0:
         out["c2p"] = c2p_leg(args.c2p_states)
     if args.events > 0:
