@@ -98,11 +98,16 @@ def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_
     return e_per_participant * e / (2 * np.pi * smear ** 2), len(part)
 
 
-def workload_state(ic, rc, eos_rows, seed=12345, b=7.0):
+# nuclei of the workloads (Woods-Saxon R, a in fm; A; sigma_NN in fm^2)
+AU197 = dict(R=6.38, a=0.535, A=197, sigma_nn=4.2)
+PB208 = dict(R=6.62, a=0.546, A=208, sigma_nn=6.76)
+
+
+def workload_state(ic, rc, eos_rows, seed=12345, b=7.0, nucleus=AU197):
     nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
     dx = (rc[1] - rc[0]) / nx
     tau0 = rc[6]
-    eT, npart = glauber_energy(nx, ny, dx, seed=seed, b=b)
+    eT, npart = glauber_energy(nx, ny, dx, seed=seed, b=b, **nucleus)
     eta = (np.arange(nz) + 0.5) * (rc[5] - rc[4]) / nz + rc[4]
     prof = np.exp(-np.maximum(np.abs(eta) - 2.0, 0.0) ** 2 / (2 * 1.0 ** 2))
     e = eT[None, :, :] * prof[:, None, None] + 0.05

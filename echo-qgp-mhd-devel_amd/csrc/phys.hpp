@@ -1352,7 +1352,10 @@ __device__ __forceinline__ void ohm_stiff(const Metric& M, const double* vel, co
 // E = E* + adt R(E, v), R the stiff current above and v the fluid velocity
 // of the inversion with that E (B and the fluid's D, S, energy do not
 // change), by fixed-point iteration on v around a 3x3 linear solve for E
-// (Cramer's rule).  Writes the new E into u.  The role of the reference's
+// (Cramer's rule), until E changes by at most 1e-12 of the field scale
+// max(|E|, |B|); the fluid inversion of the last E before that is kept (one
+// inversion per cell when the first update of E already converges).  Writes
+// the new E into u.  The role of the reference's
 // missing algorithm 7 (its locals, system.f90:494-497, declare a 3x3 system).
 template <int SYS, bool ZZ>
 __device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, double* v, double adt,
@@ -1398,13 +1401,18 @@ __device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, 
              m[0][2] * (m[1][0] * r[2] - r[1] * m[2][0])) / det;
     En[2] = (m[0][0] * (m[1][1] * r[2] - r[1] * m[2][1]) - m[0][1] * (m[1][0] * r[2] - r[1] * m[2][0]) +
              r[0] * (m[1][0] * m[2][1] - m[1][1] * m[2][0])) / det;
+    // the change of E against the field scale max(|E|, |B|), not against |E|
+    // itself, which can be orders of magnitude smaller than B
     double dE = 0., mE = 0.;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       dE = dmax(dE, fabs(En[k] - Ev[k]));
-      mE = dmax(mE, fabs(En[k]));
+      mE = dmax(mE, dmax(fabs(En[k]), fabs(B[k])));
       Ev[k] = En[k];
     }
+    // converged: v is the inversion with the previous E, equal to E to 1e-12
+    // of the field scale
+    if (it > 0 && dE <= 1e-12 * mE) break;
     const EmTerms t = em_terms(M, Ev, B);
 #pragma unroll
     for (int k = 0; k < T::NF; k++) uf[k] = u[k];
@@ -1414,7 +1422,6 @@ __device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, 
     uf[KPR] = u[KPR] - gp * t.eem;
     e = cons2prim<T::FLUID, ZZ>(E, M, uf, v, 7);
     if (e) break;
-    if (it > 0 && dE <= 1e-14 * mE) break;
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {

@@ -350,10 +350,9 @@ int echo_c2p_surrogate_noalloc(echo_grid *g, echo_mlp *m, const double *d_u, dou
 
 /* Kernel timing in milliseconds, measured with HIP events on the grid's
  * stream when enabled with echo_set_timing(g, 1).  After echo_steps (per
- * launch): t[0] flux x, t[1] flux y, t[2] flux z (means over the stages
- * that sweep: all RK stages, IMEX stages 2-4), t[3] cell update with its
- * repair launch (mean over all stages; the first IMEX stage's segment holds
- * both of its implicit solves).  After
+ * launch): t[0] flux x, t[1] flux y, t[2] flux z, t[3] cell update, means
+ * over the stages that sweep (all RK stages; IMEX stages 2-4, the first
+ * IMEX stage's implicit solves are in no mean).  After
  * echo_cons2prim_batch_dev: t[0] the solver kernel.  After
  * echo_c2p_surrogate: t[0] network + refinement kernel, t[1] fallback. */
 int echo_set_timing(echo_grid *g, int32_t on);

@@ -145,7 +145,11 @@ int rmhd_cons2prim(double *u, double *v) {
 /* The implicit part of an IMEX stage: u holds U*; finds E with
  * E = E* + adt R(E, v) where R is the stiff current (per unit sqrt(-g)) and
  * v the fluid velocity of the inversion with that E; B and the fluid's
- * conserved D, S, energy do not change.  Writes the new E into u. */
+ * conserved D, S, energy do not change.  Fixed point: solve the linear
+ * system for E at the current v, invert the fluid with that E, repeat until
+ * E changes by at most 1e-12 of the field scale max(|E|, |B|) (the last
+ * inversion is then the one with the E before the final solve).  Writes the
+ * new E into u. */
 int rmhd_cons2prim_implicit(double *u, double *v, double adt) {
   const double sig = G.sigma_el / HBARC, sch = G.sigma_chi / HBARC;
   double B[3], Es[3], E[3], uf[MAXNV];
@@ -177,16 +181,20 @@ int rmhd_cons2prim_implicit(double *u, double *v, double adt) {
              M[0][2] * (M[1][0] * r[2] - r[1] * M[2][0])) / det;
     En[2] = (M[0][0] * (M[1][1] * r[2] - r[1] * M[2][1]) - M[0][1] * (M[1][0] * r[2] - r[1] * M[2][0]) +
              r[0] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+    /* the change of E against the field scale max(|E|, |B|), not against
+     * |E| itself, which can be orders of magnitude smaller than B */
     double dE = 0., mE = 0.;
     for (int k = 0; k < 3; k++) {
       dE = fmax2(dE, fabs(En[k] - E[k]));
-      mE = fmax2(mE, fabs(En[k]));
+      mE = fmax2(mE, fmax2(fabs(En[k]), fabs(B[k])));
       E[k] = En[k];
     }
+    /* converged: v is the inversion with the previous E, which equals E to
+     * 1e-12 of the field scale */
+    if (it > 0 && dE <= 1e-12 * mE) break;
     fluid_part(u, E, B, uf);
     e = fluid_cons2prim(uf, v);
     if (e) break;
-    if (it > 0 && dE <= 1e-14 * mE) break;
   }
   for (int k = 0; k < 3; k++) { v[G.kbx + k] = B[k]; v[G.kex + k] = E[k]; u[G.kex + k] = E[k] * G.gp; }
   v[G.kglm] = u[G.kglm];
