@@ -860,6 +860,21 @@ int echo_get_repairs(echo_grid* g, int64_t* n) {
   return 0;
 }
 
+int echo_set_bulk_peak(echo_grid* g, double zs_max, double t_c, double width) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  if (!g->cfg.viscosity || !g->cfg.bulk) return fail(ECHO_E_ARG, "echo_set_bulk_peak needs VISCOUS and BULK");
+  if (!(zs_max >= 0.) || !(t_c > 0.) || !(width > 0.))
+    return fail(ECHO_E_ARG, "echo_set_bulk_peak: need zs_max >= 0, t_c > 0, width > 0");
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_bulk_peak inside a step");
+  g->ga.vp.zeta_peak = 1;
+  g->ga.vp.zs_max = zs_max;
+  g->ga.vp.zs_tc = t_c;
+  g->ga.vp.zs_w = width;
+  drop_graph(g);   // the captured launches hold the old parameters
+  if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
 int echo_set_tout(echo_grid* g, double tout) {
   if (!g) return fail(ECHO_E_ARG, "null argument");
   HIPCHECK(hipSetDevice(g->dev));

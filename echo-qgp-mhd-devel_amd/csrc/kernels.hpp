@@ -1331,7 +1331,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_glauber_prim(GridArgs a, Glaub
     if (err == 0) err = eos_sound(a.eos, 1., en, pr, cs2);
     const double hbar = 0.197326;   // common.f90:51
     const double eta = hbar * a.vp.eta_over_s * s;
-    const double zeta = a.vp.bulkvis ? 2. * eta * (1. / 3. - cs2) : 0.;
+    const double zeta = bulk_zeta(a.vp, eta, cs2, temp, s);
     const double g3 = a.coordinates == BJORKEN ? tau0 * tau0 : 1.;
     V[KPIZZ * a.N + c] = -g3 * 4. * eta / (3. * tau0);
     V[KPIXX * a.N + c] = 2. * eta / (3. * tau0);

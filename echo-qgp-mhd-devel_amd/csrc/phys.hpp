@@ -1368,6 +1368,12 @@ __device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, 
     Es[k] = u[T::KE + k] * M.gm;
     Ev[k] = v[T::KE + k];
   }
+  // every fluid inversion starts from the pressure and velocity held on
+  // entry, as the one inversion of an explicit stage does: the viscous
+  // inversion stops its velocity iteration at an absolute tolerance, so
+  // chaining the inversions would carry that iteration on and make the
+  // result depend on where the loop stops
+  const double g0x = v[KVX], g0y = v[KVY], g0z = v[KVZ], g0p = v[KPR];
   int e = 0;
   const double gp = M.gp, g3 = M.g3;
   for (int it = 0; it < 50; it++) {
@@ -1420,6 +1426,7 @@ __device__ int res_cons2prim_implicit(const Eos& E, const Metric& M, double* u, 
     uf[KVY] = u[KVY] - gp * t.X[1];
     uf[KVZ] = u[KVZ] - gp * t.X[2];
     uf[KPR] = u[KPR] - gp * t.eem;
+    v[KVX] = g0x; v[KVY] = g0y; v[KVZ] = g0z; v[KPR] = g0p;
     e = cons2prim<T::FLUID, ZZ>(E, M, uf, v, 7);
     if (e) break;
   }

@@ -18,7 +18,22 @@ struct Deriv {
 struct ViscParams {
   double eta_over_s, tau_pi_coeff;
   int bulkvis, init_type;
+  // bulk viscosity zeta/s peaked at T_c (echo_set_bulk_peak), else the
+  // reference's zeta = 2 eta (1/3 - c_s^2)
+  int zeta_peak;
+  double zs_max, zs_tc, zs_w;
 };
+
+// bulk viscosity from eta, c_s^2, T (GeV) and s: viscous.f90:186-190, or
+// zeta = hbar zs_max exp(-(T - T_c)^2 / (2 w^2)) s with echo_set_bulk_peak
+__device__ __forceinline__ double bulk_zeta(const ViscParams& P, double eta, double cs2, double temp, double s) {
+  if (!P.bulkvis) return 0.;
+  if (P.zeta_peak) {
+    const double x = (temp - P.zs_tc) / P.zs_w;
+    return 0.197326 * P.zs_max * exp(-0.5 * x * x) * s;   // hbar c, common.f90:47
+  }
+  return 2. * eta * (1. / 3. - cs2);
+}
 
 enum { SXX = 0, SYY = 1, SZZ = 2, SXY = 3, SXZ = 4, SYZ = 5 };
 
@@ -142,7 +157,7 @@ __device__ __forceinline__ int sys_source(const Eos& E, const Metric& M, int coo
       const double HBAR = 0.197326;   // common.f90:47
       eta = HBAR * P.eta_over_s * vs;
       if (P.init_type == SHEAR_VISCOUS_1D) eta = P.eta_over_s;
-      zeta = P.bulkvis ? 2. * eta * (1. / 3. - vcs2) : 0.;
+      zeta = bulk_zeta(P, eta, vcs2, vtemp, vs);
       tau_pi = (vtemp == 0 || vs == 0) ? 0. : P.tau_pi_coeff * eta / (vtemp * vs);
       if (P.init_type == GUBSER_VISCOUS) tau_pi = 5. * eta / (vtemp * vs);
       err = err_sound;

@@ -29,7 +29,7 @@ EXPORTS = (
     "echo_set_timing", "echo_get_timing", "echo_last_error",
     "echo_cons2prim_batch_dev", "echo_mlp_create", "echo_mlp_destroy", "echo_mlp_set_params",
     "echo_mlp_get_params", "echo_mlp_forward", "echo_mlp_grad", "echo_mlp_train",
-    "echo_c2p_surrogate", "echo_c2p_surrogate_noalloc", "echo_set_tile", "echo_get_tile", "echo_tune",
+    "echo_c2p_surrogate", "echo_c2p_surrogate_noalloc", "echo_set_bulk_peak", "echo_set_tile", "echo_get_tile", "echo_tune",
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
@@ -140,6 +140,7 @@ def load_library(path=LIB_PATH):
                                          C.POINTER(C.c_int64)]),
         "echo_c2p_surrogate_noalloc": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
                                                  C.POINTER(C.c_int64)]),
+        "echo_set_bulk_peak": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
         "echo_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
         "echo_get_tile": (C.c_int, [P, _ip]),
         "echo_tune": (C.c_int, [P, C.c_int32, _dp]),
@@ -426,6 +427,11 @@ class EchoGrid:
 
     def steps(self, n):
         return self._check(self.lib.echo_steps(self.h, int(n)))
+
+    def set_bulk_peak(self, zs_max, t_c=0.155, width=0.02):
+        """zeta/s = zs_max exp(-(T - t_c)^2 / (2 width^2)) (GeV) instead of the
+        reference's 2 eta (1/3 - c_s^2) / s."""
+        self._check(self.lib.echo_set_bulk_peak(self.h, float(zs_max), float(t_c), float(width)))
 
     def rk_stage(self, irk):
         return self._check(self.lib.echo_rk_stage(self.h, int(irk)))

@@ -170,6 +170,14 @@ int echo_get_scalars(echo_grid *g, double *s);
  * were rebuilt from the average of their face neighbours. */
 int echo_get_repairs(echo_grid *g, int64_t *n);
 
+/* Bulk viscosity with zeta/s peaked at the transition temperature:
+ * zeta = hbar zs_max exp(-(T - t_c)^2 / (2 width^2)) s, T and t_c, width in
+ * GeV, in place of the reference's zeta = 2 eta (1/3 - c_s^2)
+ * (get_viscous_parameters, viscous.f90:186-190), in the sources and in
+ * echo_init_glauber's Navier-Stokes Pi.  No counterpart in the reference.
+ * Needs VISCOUS and BULK; call between steps. */
+int echo_set_bulk_peak(echo_grid *g, double zs_max, double t_c, double width);
+
 /* Next output time tout (echo.f90:172, 243-247); the dt clamps of
  * evolve.f90:179-188 use it. */
 int echo_set_tout(echo_grid *g, double tout);

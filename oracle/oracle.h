@@ -43,6 +43,10 @@ void orc_get_state(double *v, double *u);
 void orc_get_deriv(double *d);
 void orc_get_scalars(double *scal);
 int  orc_steps(int nsteps);
+/* bulk viscosity zeta/s = zs_max exp(-(T - t_c)^2 / (2 width^2)) (T in
+ * GeV) instead of the reference's 2 eta (1/3 - c_s^2) / s, until the next
+ * orc_setup; needs viscosity with bulk */
+void orc_set_bulk_peak(double zs_max, double t_c, double width);
 /* MAXDT from now on (a test that follows a larger grid's time step) */
 void orc_set_maxdt(double maxdt);
 /* one RK stage irk (evolve_main) without the step bookkeeping */

@@ -79,6 +79,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.x1 = rc[0]; G.x2 = rc[1]; G.y1 = rc[2]; G.y2 = rc[3]; G.z1 = rc[4]; G.z2 = rc[5];
   G.tmin = rc[6]; G.cfl = rc[7]; G.maxdt = rc[8]; G.maxspeed = rc[9];
   G.eta_over_s = rc[10]; G.tau_pi_coeff = rc[11]; G.enezero = rc[12];
+  G.zeta_peak = 0; G.zs_max = G.zs_tc = G.zs_w = 0.;
   G.glm_alpha = rc[13]; G.sigma_el = rc[14]; G.sigma_chi = rc[15];
   G.dtout = rc[16]; G.smooth_temp = rc[17]; G.tmax = rc[18];
   G.enable_smooth_viscosity = G.smooth_temp > 0.;

@@ -86,7 +86,7 @@ int orc_glauber_prim(const double *par, const double *pa, int na, const double *
           err = eos_sound(1., en, pr, &cs2);
           if (err > 0) return err;
           const double eta = HBAR * G.eta_over_s * s;
-          const double zeta = G.bulkvis ? 2. * eta * (1. / 3. - cs2) : 0.;
+          const double zeta = bulk_zeta(eta, cs2, temp, s);
           const double tau0 = G.tmin;
           const double g3 = G.coordinates == 2 ? tau0 * tau0 : 1.;
           vout[KPIZZ * n + c] = -g3 * 4. * eta / (3. * tau0);

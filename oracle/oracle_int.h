@@ -29,6 +29,10 @@ typedef struct {
   int init_type, derivatives_out;
   double x1, x2, y1, y2, z1, z2, tmin, cfl, maxdt, maxspeed, eta_over_s;
   double tau_pi_coeff, enezero, glm_alpha, sigma_el, sigma_chi, dtout;
+  /* bulk viscosity zeta/s peaked at T_c (orc_set_bulk_peak); 0: the
+   * reference's zeta = 2 eta (1/3 - c_s^2) */
+  int zeta_peak;
+  double zs_max, zs_tc, zs_w;
   double smooth_temp, tmax;
   int enable_smooth_viscosity;
   /* system_setup, system.f90:67-128 */
@@ -107,6 +111,7 @@ int rmhd_cons2prim_implicit(double *u, double *v, double adt);
 int rmhd_flux(double *v, double *u, double *f, double *vf, int dir);
 int rmhd_source(int ix, int iy, int iz, double *v, double *src, double *src_stiff);
 void rmhd_stiff(const double *v, double *s3);
+double bulk_zeta(double eta, double cs2, double temp, double s);
 void get_derived_pi(const double *p, double *pitt, double *pitx, double *pity, double *pitz);
 void get_derived_pi_zz(const double *p, double *pitt, double *pitx, double *pity, double *pitz, double *pizz);
 void system_metric(void);

@@ -148,12 +148,18 @@ int rmhd_cons2prim(double *u, double *v) {
  * conserved D, S, energy do not change.  Fixed point: solve the linear
  * system for E at the current v, invert the fluid with that E, repeat until
  * E changes by at most 1e-12 of the field scale max(|E|, |B|) (the last
- * inversion is then the one with the E before the final solve).  Writes the
- * new E into u. */
+ * inversion is then the one with the E before the final solve).  Every
+ * fluid inversion starts from the primitives v held on entry, as the one
+ * inversion of an explicit stage does: the viscous inversion stops its
+ * velocity iteration at an absolute tolerance (system.f90:511-665), so
+ * chaining the inversions would carry that iteration on and make the
+ * result depend on where the loop stops.  Writes the new E into u. */
 int rmhd_cons2prim_implicit(double *u, double *v, double adt) {
   const double sig = G.sigma_el / HBARC, sch = G.sigma_chi / HBARC;
   double B[3], Es[3], E[3], uf[MAXNV];
   for (int k = 0; k < 3; k++) { B[k] = u[G.kbx + k] * G.gm; Es[k] = u[G.kex + k] * G.gm; E[k] = v[G.kex + k]; }
+  /* the starting guesses the fluid inversion reads: pressure and velocity */
+  const double g0[4] = {v[KVX], v[KVY], v[KVZ], v[KPR]};
   int e = 0;
   for (int it = 0; it < 50; it++) {
     const double *vel = &v[KVX];
@@ -193,6 +199,7 @@ int rmhd_cons2prim_implicit(double *u, double *v, double adt) {
      * 1e-12 of the field scale */
     if (it > 0 && dE <= 1e-12 * mE) break;
     fluid_part(u, E, B, uf);
+    v[KVX] = g0[0]; v[KVY] = g0[1]; v[KVZ] = g0[2]; v[KPR] = g0[3];
     e = fluid_cons2prim(uf, v);
     if (e) break;
   }

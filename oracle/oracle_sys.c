@@ -860,6 +860,26 @@ int sys_flux(double *v, double *u, double *f, double *vf, int dir) {
   return 0;
 }
 
+/* The bulk viscosity: the reference's zeta = 2 eta (1/3 - c_s^2)
+ * (viscous.f90:186-190), or, when set with orc_set_bulk_peak, a zeta/s
+ * peaked at T_c, zeta = hbar zs_max exp(-(T - T_c)^2 / (2 w^2)) s (no
+ * counterpart in the reference) */
+double bulk_zeta(double eta, double cs2, double temp, double s) {
+  if (!G.bulkvis) return 0.;
+  if (G.zeta_peak) {
+    const double x = (temp - G.zs_tc) / G.zs_w;
+    return HBAR * G.zs_max * exp(-0.5 * x * x) * s;
+  }
+  return 2. * eta * (1. / 3. - cs2);
+}
+
+void orc_set_bulk_peak(double zs_max, double t_c, double width) {
+  G.zeta_peak = 1;
+  G.zs_max = zs_max;
+  G.zs_tc = t_c;
+  G.zs_w = width;
+}
+
 /* get_viscous_parameters, viscous.f90:172-199 */
 static int get_viscous_parameters(double rh, double pr, double *eta, double *zeta, double *tau_pi, double *tau_pi_big) {
   double en, temp, s, cs2;
@@ -868,8 +888,7 @@ static int get_viscous_parameters(double rh, double pr, double *eta, double *zet
   if (err2) err = err2;
   *eta = HBAR * G.eta_over_s * s;
   if (G.init_type == SHEAR_VISCOUS_1D) *eta = G.eta_over_s;
-  if (G.bulkvis) *zeta = 2. * *eta * (1. / 3. - cs2);
-  else *zeta = 0.;
+  *zeta = bulk_zeta(*eta, cs2, temp, s);
   if (temp == 0 || s == 0) *tau_pi = 0.;
   else *tau_pi = G.tau_pi_coeff * *eta / (temp * s);
   if (G.init_type == GUBSER_VISCOUS) *tau_pi = 5. * *eta / (temp * s);

@@ -197,6 +197,13 @@ class Oracle(_Lib):
         self.lib.orc_get_repair_counts(a)
         return dict(zip(self.REPAIRS, list(a)))
 
+    def set_bulk_peak(self, zs_max, t_c=0.155, width=0.02):
+        """zeta/s peaked at t_c (orc_set_bulk_peak)"""
+        f = self.lib.orc_set_bulk_peak
+        f.restype = None
+        f.argtypes = [C.c_double] * 3
+        f(float(zs_max), float(t_c), float(width))
+
     def set_maxdt(self, maxdt):
         """MAXDT from now on"""
         f = self.lib.orc_set_maxdt

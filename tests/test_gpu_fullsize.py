@@ -214,11 +214,12 @@ def _bench():
     return bench
 
 
-def _symmetric_resistive_run(nx, ny, nz, dx, deta, nucleus, b, eta_over_s, steps):
+def _symmetric_resistive_run(nx, ny, nz, dx, deta, nucleus, b, eta_over_s, steps, bulk_peak=None):
     from libs import make_config
     from echoamd import EchoGrid, read_eos_rows
     bench = _bench()
-    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=0, eqstate=3, mhd=2,
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=1 if bulk_peak else 0,
+                         eqstate=3, mhd=2,
                          divclean=1, algo_solver=7, rec="WENO5", nrk=4, bc=2, obtained_zz=1, init_type=3,
                          xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
                          zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=0.1,
@@ -231,6 +232,8 @@ def _symmetric_resistive_run(nx, ny, nz, dx, deta, nucleus, b, eta_over_s, steps
     print(f"\nresistive {nx}x{ny}x{nz}: symmetric initial state and {steps} IMEX steps", flush=True)
     g = EchoGrid(ic, rc, eos_rows=eos)
     assert g.nv == 19
+    if bulk_peak:
+        g.set_bulk_peak(*bulk_peak)
     assert g.init_from_prim(v0) == 0
     del v0
     g.steps(steps)
@@ -271,6 +274,10 @@ def test_resistive_reflection_symmetry_192x192x128():
 
 
 def test_resistive_reflection_symmetry_384x384x256():
-    """The large Pb-Pb grid (dx 0.08, deta 0.05, b = 5 fm, eta/s 0.12),
+    """The large Pb-Pb grid (dx 0.08, deta 0.05, b = 5 fm, eta/s 0.12,
+    zeta/s peaked at T_c = 155 MeV with width 20 MeV and height 0.08),
     3 IMEX steps."""
-    _check_symmetry(_symmetric_resistive_run(384, 384, 256, 0.08, 0.05, _bench().PB208, 5.0, 0.12, 3))
+    worst = _symmetric_resistive_run(384, 384, 256, 0.08, 0.05, _bench().PB208, 5.0, 0.12, 3,
+                                     bulk_peak=(0.08, 0.155, 0.02))
+    assert 5 in worst   # the bulk pressure developed
+    _check_symmetry(worst)
