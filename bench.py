@@ -6,12 +6,13 @@ tabulated lattice-QCD EoS from qcdIEOS0.dat, eta/s = 0.1, zeta/s = 0),
 evolved as 3+1D viscous resistive MHD with divergence cleaning (nv = 19:
 rho, v, p, Pi, pi^{mu nu}, B, E, psi), WENO5 + HLL, IMEX-SSP3(4,3,3) with
 the Ohmic current implicit (sigma_el = 5.8 MeV), CFL 0.4.  The initial
-energy density is a seeded Monte-Carlo Glauber event (Woods-Saxon R = 6.38
-fm, a = 0.535 fm, sigma_NN = 42 mb, b = 7 fm, Gaussian smearing 0.4 fm) with
-a boost-invariant plateau and Gaussian tails in eta and a Navier-Stokes
-initial shear tensor; the magnetic field is a divergence-free dipole-like
-field with eB_y = 0.1 m_pi^2 at the centre and a Gaussian transverse width of
-3 fm; E = 0 (the fluid starts at rest).  All synthetic (no files read).
+energy density is a seeded Monte-Carlo Glauber event of the library's
+generator (Woods-Saxon R = 6.38 fm, a = 0.535 fm, sigma_NN = 42 mb, b = 7 fm;
+participants smeared by Gaussians of 0.4 fm) with a boost-invariant plateau
+and Gaussian tails in eta and a Navier-Stokes initial shear tensor; the
+magnetic field is a divergence-free dipole-like field with eB_y = 0.1 m_pi^2
+at the centre and a Gaussian transverse width of 3 fm; E = 0 (the fluid
+starts at rest).  All synthetic (no files read).
 
 One zone-update = one cell advanced through one full time step (all four
 IMEX stages).  value = cells * steps / wall time of the timed steps, with
@@ -67,28 +68,20 @@ def dipole_field(x, y, b0=B_PEAK, w=B_WIDTH):
     return bx, by
 
 
-def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_nn=4.2,
-                   smear=0.4, e_per_participant=25.0):
-    """Seeded MC-Glauber transverse energy density (GeV/fm^3 at tau0)."""
-    rng = np.random.default_rng(seed)
-
-    def nucleus():
-        pts = []
-        while len(pts) < A:
-            r = rng.uniform(0, R + 6 * a, 4 * A)
-            keep = rng.uniform(0, 1, r.size) < (r / (R + 6 * a)) ** 2 / (1 + np.exp((r - R) / a))
-            pts.extend(r[keep].tolist())
-        r = np.array(pts[:A])
-        cth = rng.uniform(-1, 1, A)
-        phi = rng.uniform(0, 2 * np.pi, A)
-        sth = np.sqrt(1 - cth ** 2)
-        return np.stack([r * sth * np.cos(phi), r * sth * np.sin(phi)], axis=1)
-
-    na = nucleus() + [b / 2, 0]
-    nb = nucleus() - [b / 2, 0]
-    d2 = ((na[:, None, :] - nb[None, :, :]) ** 2).sum(-1)
-    hit = d2 < sigma_nn / np.pi
-    part = np.concatenate([na[hit.any(1)], nb[hit.any(0)]])
+def glauber_energy(nx, ny, dx, seed=12345, b=7.0, nucleus="Au", sigma_nn=42.0, smear=0.4,
+                   e_per_participant=25.0):
+    """Seeded MC-Glauber transverse energy density (GeV/fm^3 at tau0): the
+    participants of one event of the library's Glauber-MC generator
+    (echo_glauber_generate, impact parameter b), each a Gaussian of width
+    smear."""
+    import tempfile
+    from echoamd import glauber_generate, glauber_read_event
+    with tempfile.TemporaryDirectory() as d:
+        n = glauber_generate(d, seed, nucleus=nucleus, sigma_nn=sigma_nn, nconf=1, nbcoll=1, b=b,
+                             min_participants=1)
+        assert n == 1, "no collision at this impact parameter"
+        ev = glauber_read_event(d, 1)
+    part = np.concatenate([ev["part_a"], ev["part_b"]])
     x = (np.arange(nx) + 0.5) * dx - nx * dx / 2
     y = (np.arange(ny) + 0.5) * dx - ny * dx / 2
     X, Y = np.meshgrid(x, y, indexing="xy")
@@ -98,9 +91,9 @@ def glauber_energy(nx, ny, dx, seed=12345, b=7.0, R=6.38, a=0.535, A=197, sigma_
     return e_per_participant * e / (2 * np.pi * smear ** 2), len(part)
 
 
-# nuclei of the workloads (Woods-Saxon R, a in fm; A; sigma_NN in fm^2)
-AU197 = dict(R=6.38, a=0.535, A=197, sigma_nn=4.2)
-PB208 = dict(R=6.62, a=0.546, A=208, sigma_nn=6.76)
+# nuclei of the workloads (nuclear_data.dat; sigma_NN in mb)
+AU197 = dict(nucleus="Au", sigma_nn=42.0)
+PB208 = dict(nucleus="Pb", sigma_nn=67.6)
 
 
 def workload_state(ic, rc, eos_rows, seed=12345, b=7.0, nucleus=AU197):

@@ -291,6 +291,9 @@ extern "C" {
 
 const char* echo_last_error(void) { return echo::g_err.c_str(); }
 
+// error text of the host-only translation units (glauber_mc.cpp)
+__attribute__((visibility("hidden"))) void echo_set_error_text(const char* msg) { echo::g_err = msg; }
+
 }  // extern "C"
 
 // words of the failure bitmap of a grid of n cells

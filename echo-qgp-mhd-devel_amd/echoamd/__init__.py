@@ -29,7 +29,8 @@ EXPORTS = (
     "echo_set_timing", "echo_get_timing", "echo_last_error",
     "echo_cons2prim_batch_dev", "echo_mlp_create", "echo_mlp_destroy", "echo_mlp_set_params",
     "echo_mlp_get_params", "echo_mlp_forward", "echo_mlp_grad", "echo_mlp_train",
-    "echo_c2p_surrogate", "echo_c2p_surrogate_noalloc", "echo_set_bulk_peak", "echo_set_tile", "echo_get_tile", "echo_tune",
+    "echo_c2p_surrogate", "echo_c2p_surrogate_noalloc", "echo_set_bulk_peak", "echo_glauber_generate",
+    "echo_glauber_read_event", "echo_set_tile", "echo_get_tile", "echo_tune",
     "echo_ensemble_create", "echo_ensemble_destroy", "echo_ensemble_size", "echo_ensemble_grid",
     "echo_ensemble_tune", "echo_ensemble_steps", "echo_ensemble_last_ms",
     "echo_dev_alloc", "echo_dev_free", "echo_dev_upload", "echo_dev_download",
@@ -77,6 +78,62 @@ class GlauberParams(C.Structure):
     """Mirror of struct echo_glauber_params."""
     _fields_ = [(n, C.c_double) for n in ("kappa", "ah", "sig_mc", "ybeam", "deta", "sigeta",
                                           "rhocenter")]
+
+
+class GlauberSampler(C.Structure):
+    _fields_ = [("A", C.c_int32), ("Z", C.c_int32), ("nconf", C.c_int32), ("nbcoll", C.c_int32),
+                ("min_participants", C.c_int32), ("fixed_b", C.c_int32), ("radius", C.c_double),
+                ("diffuse", C.c_double), ("rho0", C.c_double), ("sigma_nn", C.c_double),
+                ("bimpact", C.c_double), ("bmax", C.c_double)]
+
+
+class GlauberEvent(C.Structure):
+    _fields_ = [("b", C.c_double), ("n_part_a", C.c_int32), ("n_part_b", C.c_int32),
+                ("n_spec_a", C.c_int32), ("n_spec_b", C.c_int32), ("n_coll", C.c_int32),
+                ("part_xy", C.c_void_p), ("part_type", C.c_void_p), ("spec_xy", C.c_void_p),
+                ("spec_type", C.c_void_p), ("coll_xy", C.c_void_p)]
+
+
+# nuclei of nuclear_data.dat (Woods-Saxon R, a in fm, roze in fm^-3)
+NUCLEI = {
+    "Au": dict(A=197, Z=79, radius=6.38, diffuse=0.535, rho0=0.1693),
+    "Pb": dict(A=208, Z=82, radius=6.62, diffuse=0.546, rho0=0.1604),
+}
+
+
+def glauber_generate(directory, seed, nucleus="Au", sigma_nn=42.0, nconf=100, nbcoll=1, b=None, bmax=20.0,
+                     min_participants=18):
+    """Glauber-MC events (echo_glauber_generate) into directory; b: fixed
+    impact parameter in fm, or None for p(b) ~ b on [0, bmax].  Returns the
+    number of events written."""
+    lib = load_library()
+    nuc = NUCLEI[nucleus] if isinstance(nucleus, str) else nucleus
+    sp = GlauberSampler(nconf=nconf, nbcoll=nbcoll, min_participants=min_participants,
+                        fixed_b=0 if b is None else 1, sigma_nn=sigma_nn, bimpact=0.0 if b is None else b,
+                        bmax=bmax, **nuc)
+    n = C.c_int32()
+    _check(lib, lib.echo_glauber_generate(C.byref(sp), C.c_uint64(seed), str(directory).encode(), C.byref(n)))
+    return n.value
+
+
+def glauber_read_event(directory, iev, cap=600):
+    """Event iev (1-based) of directory (echo_glauber_read_event): dict of b,
+    part_a, part_b, spec_a, spec_b ((n, 2) arrays, protons first), their
+    types (0 proton, 1 neutron) and coll."""
+    lib = load_library()
+    pxy, sxy, cxy = np.zeros(2 * cap), np.zeros(2 * cap), np.zeros(2 * cap * cap // 4)
+    pt, st = np.zeros(cap, dtype=np.int32), np.zeros(cap, dtype=np.int32)
+    ev = GlauberEvent(part_xy=pxy.ctypes.data, part_type=pt.ctypes.data, spec_xy=sxy.ctypes.data,
+                      spec_type=st.ctypes.data, coll_xy=cxy.ctypes.data)
+    _check(lib, lib.echo_glauber_read_event(str(directory).encode(), int(iev), C.byref(ev), cap, cap,
+                                            cap * cap // 8))
+    na, nb, sa, sb, nc = ev.n_part_a, ev.n_part_b, ev.n_spec_a, ev.n_spec_b, ev.n_coll
+    P = pxy[:2 * (na + nb)].reshape(-1, 2)
+    S = sxy[:2 * (sa + sb)].reshape(-1, 2)
+    return dict(b=ev.b, part_a=P[:na].copy(), part_b=P[na:].copy(), type_part_a=pt[:na].copy(),
+                type_part_b=pt[na:na + nb].copy(), spec_a=S[:sa].copy(), spec_b=S[sa:].copy(),
+                type_spec_a=st[:sa].copy(), type_spec_b=st[sa:sa + sb].copy(),
+                coll=cxy[:2 * nc].reshape(-1, 2).copy())
 
 
 class BFieldParams(C.Structure):
@@ -141,6 +198,8 @@ def load_library(path=LIB_PATH):
         "echo_c2p_surrogate_noalloc": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
                                                  C.POINTER(C.c_int64)]),
         "echo_set_bulk_peak": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
+        "echo_glauber_generate": (C.c_int, [P, C.c_uint64, C.c_char_p, C.POINTER(C.c_int32)]),
+        "echo_glauber_read_event": (C.c_int, [C.c_char_p, C.c_int32, P, C.c_int32, C.c_int32, C.c_int32]),
         "echo_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
         "echo_get_tile": (C.c_int, [P, _ip]),
         "echo_tune": (C.c_int, [P, C.c_int32, _dp]),

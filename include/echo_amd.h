@@ -98,6 +98,50 @@ int echo_nv(const echo_grid *g);
  * v: nv*nx*ny*nz doubles (Fortran layout), host memory. */
 int echo_init_from_prim(echo_grid *g, const double *v);
 
+/* Glauber Monte-Carlo events, nucleus-nucleus collisions: generate_events
+ * (glaubermc.f90:80-635) with the project's random numbers (xoshiro256**
+ * seeded by splitmix64; the reference's gfortran rand() stream is not
+ * reproduced).  Woods-Saxon nucleons by rejection on r < 12.6 fm, each
+ * nucleus moved to its transverse centre of mass, impact parameter with
+ * p(b) ~ b on [0, bmax] or fixed, collisions at d^2 <= sigma_NN / pi, events
+ * with a collision and at least min_participants participants.  Writes
+ * summary.dat, partcoll.bin (participants, (x, y, type) records of 8 + 8 + 4
+ * bytes, 0 proton 1 neutron, A then B), spectators.bin (same records) and
+ * collisions.bin ((x, y) records) into dir, events one after the other;
+ * *n_events: the events written. */
+typedef struct echo_glauber_sampler {
+  int32_t A;                 /* [NUCLEUS] mass number of both nuclei (<= 300) */
+  int32_t Z;                 /* protons: nucleons 1..Z of a nucleus */
+  int32_t nconf;             /* [NCONF] nuclear configurations */
+  int32_t nbcoll;            /* [NBCOLL] impact parameters tried per configuration */
+  int32_t min_participants;  /* [MIN_PART] (reference default 18) */
+  int32_t fixed_b;           /* 1: b = bimpact, one per configuration */
+  double radius, diffuse;    /* Woods-Saxon R, a (fm), nuclear_data.dat */
+  double rho0;               /* Woods-Saxon normalisation roze (cancels out) */
+  double sigma_nn;           /* [SIGMA_IN] inelastic NN cross section, mb */
+  double bimpact;            /* [B] fm, with fixed_b */
+  double bmax;               /* fm (the reference uses 20 for AA) */
+} echo_glauber_sampler;
+int echo_glauber_generate(const echo_glauber_sampler *s, uint64_t seed, const char *dir,
+                          int32_t *n_events);
+
+/* One event of dir (1-based iev), as generate_energy_density_profile and
+ * generate_B_field_profile read it (glaubermc.f90:760-790, 1009-1058):
+ * participants of A then of B with their types, spectators likewise,
+ * collisions.  The caller's arrays hold cap_part, cap_spec, cap_coll (x, y)
+ * pairs (and types); ECHO_E_ARG when an event does not fit. */
+typedef struct echo_glauber_event {
+  double b;
+  int32_t n_part_a, n_part_b, n_spec_a, n_spec_b, n_coll;
+  double *part_xy;     /* 2 * cap_part */
+  int32_t *part_type;  /* cap_part */
+  double *spec_xy;     /* 2 * cap_spec */
+  int32_t *spec_type;  /* cap_spec */
+  double *coll_xy;     /* 2 * cap_coll */
+} echo_glauber_event;
+int echo_glauber_read_event(const char *dir, int32_t iev, echo_glauber_event *ev, int32_t cap_part,
+                            int32_t cap_spec, int32_t cap_coll);
+
 /* Glauber Monte-Carlo initial state computed on the device: the energy
  * deposition of generate_energy_density_profile (glaubermc.f90:686-842,
  * nucleus-nucleus branch) from given participant and collision positions,
@@ -105,8 +149,8 @@ int echo_init_from_prim(echo_grid *g, const double *v);
  * rho = 1, v = 0, p = eos_pressure(rhocenter, e + enezero)), for viscous
  * runs the Navier-Stokes stresses of viscous_initio (viscous.f90:101-136),
  * then prim2cons (+ con2prim) as echo_init_from_prim.  part_a, part_b, coll:
- * (x, y) pairs in fm (host memory), at most 4096 sources in total; the
- * random sampling of generate_events stays with the caller. */
+ * (x, y) pairs in fm (host memory), at most 4096 sources in total, as
+ * echo_glauber_read_event gives them. */
 typedef struct echo_glauber_params {
   double kappa;       /* [KAPPA] */
   double ah;          /* [AH] weight of the binary collisions */

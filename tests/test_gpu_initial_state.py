@@ -129,3 +129,32 @@ def test_glauber_rejects_bad_input():
         big = [(np.zeros((600, 2)), np.zeros(600, np.int32))] * 4
         g.init_glauber_mhd(PAR, [[0, 0]], [[0, 0]], [[0, 0]], bp, big)
     g.close()
+
+
+def test_glauber_from_generated_event(oracle, tmp_path):
+    """The whole initial-state path of the library: an Au-Au event of
+    echo_glauber_generate (b = 6 fm), read back with echo_glauber_read_event,
+    deposited on the device by echo_init_glauber, against the oracle's
+    deposition of the same lists, and three steps."""
+    from echoamd import EchoGrid, glauber_generate, glauber_read_event
+    assert glauber_generate(tmp_path, 17, nucleus="Au", nconf=1, b=6.0) == 1
+    ev = glauber_read_event(tmp_path, 1)
+    assert len(ev["part_a"]) + len(ev["part_b"]) > 100 and len(ev["coll"]) > 100
+    ic, rc = make_config(nx=48, ny=48, nz=12, coordinates=2, viscosity=1, eqstate=3,
+                         xlim=(-9.6, 9.6), ylim=(-9.6, 9.6), zlim=(-3.0, 3.0), tmin=0.6,
+                         init_type=3, eta_over_s=0.1, maxdt=0.05)
+    oracle.setup(ic, rc)
+    e, v0 = oracle_glauber_prim(oracle, PAR, ev["part_a"], ev["part_b"], ev["coll"])
+    assert e == 0
+    assert oracle.init_from_prim(v0) == 0
+    vo, uo = oracle.state()
+    g = EchoGrid(ic, rc)
+    assert g.init_glauber(PAR, ev["part_a"], ev["part_b"], ev["coll"]) == 0
+    vg, ug = g.state()
+    assert rel_linf(vg, vo) <= 1e-12 and rel_linf(ug, uo) <= 1e-12
+    assert oracle.steps(3) == 0
+    g.steps(3)
+    vg, ug = g.state()
+    vo, uo = oracle.state()
+    assert rel_linf(vg, vo) <= 1e-10 and rel_linf(ug, uo) <= 1e-10
+    g.close()
