@@ -3,14 +3,23 @@
 ! running the time loop of echo.f90:185-262 through the C-ABI.
 !
 ! Usage: echo_amd_driver [param_file]
-! Supported reference flags: INIT_TYPE (4, ideal Gubser flow), COORD,
-! VISCOUS, BULK, MHD, DIVCLEAN, GLM_PARAM, CUT_TEMP, NX, NY, NZ, XMIN..ZMAX,
-! TSTART, TSTOP, DTOUT, MAXDT, CFL, REC_ALGO, INT_ALGO, MAXSPEED, ENEZERO,
-! ETA_S, TAU_PI_C, TRACE_IMP, EOS, EOS_FILE, ALG_SOLV, EL_COND, CHIR_COND,
-! bound_con; other reference flags are ignored.  Driver flags: GUB_T0...
-! (T0 of the Gubser initialiser in fm^-1; the reference fixes 2.0 at
-! common.f90:1468), NSTEPS...= (step limit), CHUNK....= (steps per device
-! call), DUMP_FILE= (final v then u, FP64 stream), DEVICE...=.
+! Supported reference flags: INIT_TYPE (3 Glauber Monte Carlo, 4 ideal
+! Gubser flow), COORD, VISCOUS, BULK, MHD, DIVCLEAN, GLM_PARAM, CUT_TEMP, NX,
+! NY, NZ, XMIN..ZMAX, TSTART, TSTOP, TEMP_END, DTOUT, OUTP_PREC, MAXDT, CFL,
+! REC_ALGO (default MPE5), INT_ALGO (RK2 default, RK3, SSP for MHD=2),
+! MAXSPEED, ENEZERO, ETA_S, TAU_PI_C, TRACE_IMP, EOS, EOS_FILE, ALG_SOLV,
+! EL_COND, CHIR_COND, bound_con, the output selection of out_vars (density..=
+! .. glm......=, common.f90:1110-1190) with outdir...=, and for INIT_TYPE 3
+! NUCLEUS (Au, Pb), RADS, SIGMA_IN, B, FIXED_B, NCONF, NBCOLL, MIN_PART,
+! EV_START, KAPPA, AH, SIG, DETA, SIGETA, RHOCENTER; other reference flags
+! are ignored.  Driver flags: GUB_T0... (T0 of the Gubser initialiser in
+! fm^-1; the reference fixes 2.0 at common.f90:1468), SEED.....= (seed of
+! the Glauber-MC generator; the reference takes it from random_seed.dat or
+! the clock), GLAUB_DIR= (directory of the event files, default .),
+! NSTEPS...= (step limit), CHUNK....= (steps per device call), DUMP_FILE=
+! (final v then u, FP64 stream), DEVICE...=.
+! Output: frames (echo_write_frame) at the start, at every output time and
+! when TEMP_END stops the run (echo.f90:152, 243-247, 264-272).
 ! Freeze-out hypersurface: HYP_COMPU, FREEZKIND, FREEZEVAL, HYPSURFTI as in
 ! the reference (common.f90:1062-1076), written to HYP_FILE.= (default
 ! hypersurface.txt) at the cadence of echo.f90:176-177,252-258; with
@@ -33,6 +42,15 @@ program echo_amd_driver
   integer(c_int32_t) :: nrows
   real(8) :: t0gub, s(10), ttot, freeze_value, freeze_dt, thyp, hdr(2), tout_prev
   integer :: hyp_compu, freeze_kind, restart_type
+  ! Glauber Monte Carlo (INIT_TYPE 3), output frames, stop criterion
+  character(len=5) :: nucleus
+  character(len=256) :: glauber_dir, outdir
+  real(8) :: rads, sigma_in, bimpact, temp_end
+  integer :: fixed_b, nconf, nbcoll, min_part, ev_start, outp_prec
+  integer(c_int64_t) :: seed
+  integer(c_int32_t) :: sel(28), stopped
+  type(echo_glauber_params) :: gpar
+  real(8) :: maxv
   integer(c_int32_t) :: iout
   integer(c_int64_t) :: nfo
   real(8), allocatable :: v(:,:,:,:), u(:,:,:,:), rows(:,:)
@@ -42,7 +60,7 @@ program echo_amd_driver
   pfile = 'param.dat'
   if (command_argument_count() >= 1) call get_command_argument(1, pfile)
   ! defaults of common.f90
-  cfg = echo_config(0, 0, 0, 2, 0, 0, 0, 0, 1, 4, 3, 2, 5, 1, 4, 0, &
+  cfg = echo_config(0, 0, 0, 2, 0, 0, 0, 0, 1, 7, 2, 2, 5, 1, 4, 0, &
                     -1d0, 1d0, -1d0, 1d0, -1d0, 1d0, 1d0, 0.4d0, 0.1d0, 0.995d0, 0.08d0, &
                     5d0, 0.001d0, 10d0, 0.0058d0, 0.0015d0, 1d6, 0d0, 1d6)
   t0gub = 2.0d0
@@ -60,6 +78,23 @@ program echo_amd_driver
   freeze_kind = 1
   freeze_value = 0.5d0
   freeze_dt = 0.1d0
+  ! Glauber-MC and output defaults of common.f90:574-680
+  nucleus = 'Au'
+  rads = 200d0
+  sigma_in = 42d0
+  bimpact = 5d0
+  fixed_b = 0
+  nconf = 100
+  nbcoll = 1
+  min_part = 18
+  ev_start = 1
+  gpar = echo_glauber_params(19d0, 0.15d0, 0.8d0, 0d0, 2d0, 1d0, 1d0)
+  seed = 1
+  glauber_dir = '.'
+  temp_end = 0d0
+  outp_prec = 8
+  outdir = '.'
+  sel = 0
   open(29, file=trim(pfile), status='old', iostat=ios)
   if (ios /= 0) then
     write(*,*) 'cannot open ', trim(pfile)
@@ -108,8 +143,9 @@ program echo_amd_driver
       select case (trim(adjustl(val)))
       case ('RK2'); cfg%nrk = 2
       case ('RK3'); cfg%nrk = 3
+      case ('SSP'); cfg%nrk = 4
       case default
-        write(*,*) 'INT_ALGO ', trim(val), ' is not supported (RK2, RK3)'
+        write(*,*) 'INT_ALGO ', trim(val), ' is not supported (RK2, RK3, SSP)'
         stop 1
       end select
     case ('REC_ALGO.=')
@@ -125,6 +161,53 @@ program echo_amd_driver
         write(*,*) 'REC_ALGO ', trim(val), ' is not supported'
         stop 1
       end select
+    case ('TEMP_END.='); read(val, *) temp_end
+    case ('OUTP_PREC='); read(val, *) outp_prec
+    case ('outdir...='); outdir = adjustl(val)
+    case ('NUCLEUS..='); nucleus = adjustl(val)
+    case ('RADS.....='); read(val, *) rads
+    case ('SIGMA_IN.='); read(val, *) sigma_in
+    case ('B........='); read(val, *) bimpact
+    case ('FIXED_B..='); read(val, *) fixed_b
+    case ('NCONF....='); read(val, *) nconf
+    case ('NBCOLL...='); read(val, *) nbcoll
+    case ('MIN_PART.='); read(val, *) min_part
+    case ('EV_START.='); read(val, *) ev_start
+    case ('KAPPA....='); read(val, *) gpar%kappa
+    case ('AH.......='); read(val, *) gpar%ah
+    case ('SIG......='); read(val, *) gpar%sig_mc
+    case ('DETA.....='); read(val, *) gpar%deta
+    case ('SIGETA...='); read(val, *) gpar%sigeta
+    case ('RHOCENTER='); read(val, *) gpar%rhocenter
+    case ('SEED.....='); read(val, *) seed
+    case ('GLAUB_DIR='); glauber_dir = adjustl(val)
+    case ('density..='); read(val, *) sel(1)
+    case ('vx.......='); read(val, *) sel(2)
+    case ('vy.......='); read(val, *) sel(3)
+    case ('vz.......='); read(val, *) sel(4)
+    case ('pressure.='); read(val, *) sel(5)
+    case ('ene_dens.='); read(val, *) sel(6)
+    case ('temper...='); read(val, *) sel(7)
+    case ('entr_dens='); read(val, *) sel(8)
+    case ('bulk_visc='); read(val, *) sel(9)
+    case ('pi^tt....='); read(val, *) sel(10)
+    case ('pi^tx....='); read(val, *) sel(11)
+    case ('pi^ty....='); read(val, *) sel(12)
+    case ('pi^tz....='); read(val, *) sel(13)
+    case ('pi^xy....='); read(val, *) sel(14)
+    case ('pi^xz....='); read(val, *) sel(15)
+    case ('pi^yz....='); read(val, *) sel(16)
+    case ('pi^xx....='); read(val, *) sel(17)
+    case ('pi^yy....='); read(val, *) sel(18)
+    case ('pi^zz....='); read(val, *) sel(19)
+    case ('gamma....='); read(val, *) sel(20)
+    case ('bx.......='); read(val, *) sel(21)
+    case ('by.......='); read(val, *) sel(22)
+    case ('bz.......='); read(val, *) sel(23)
+    case ('ex.......='); read(val, *) sel(24)
+    case ('ey.......='); read(val, *) sel(25)
+    case ('ez.......='); read(val, *) sel(26)
+    case ('glm......='); read(val, *) sel(27)
     case ('GUB_T0...='); read(val, *) t0gub
     case ('NSTEPS...='); read(val, *) nsteps
     case ('CHUNK....='); read(val, *) chunk
@@ -141,8 +224,8 @@ program echo_amd_driver
     end select
   end do
   close(29)
-  if (cfg%init_type /= 4) then
-    write(*,*) 'this driver initialises INIT_TYPE=4 (ideal Gubser flow) only; ', &
+  if (cfg%init_type /= 4 .and. cfg%init_type /= 3) then
+    write(*,*) 'this driver initialises INIT_TYPE=3 (Glauber Monte Carlo) and 4 (ideal Gubser flow); ', &
                'other initial states are handed over with echo_upload_state'
     stop 2
   end if
@@ -179,10 +262,21 @@ program echo_amd_driver
     iout = iout + 1
     write(*,'(a,a)') ' restarted from ', trim(resume_file)
   else
-    call gubser_init(v)
-    ierr = echo_init_from_prim(grid, v)
-    if (ierr /= 0) call fail('echo_init_from_prim', ierr)
+    if (cfg%init_type == 3) then
+      call glauber_init()
+    else
+      call gubser_init(v)
+      ierr = echo_init_from_prim(grid, v)
+      if (ierr /= 0) call fail('echo_init_from_prim', ierr)
+    end if
     ierr = echo_set_tout(grid, cfg%tmin + cfg%dtout)
+    ! out_vars(0) of the initial state, then iout = 1 (echo.f90:152-159)
+    call frame(0)
+    iout = 1
+  end if
+  if (temp_end > 0) then
+    ierr = echo_set_stop(grid, temp_end, int(freeze_kind, c_int32_t))
+    if (ierr /= 0) call fail('echo_set_stop', ierr)
   end if
   ierr = echo_get_scalars(grid, s)
   if (s(1) == cfg%tmin) then
@@ -206,9 +300,25 @@ program echo_amd_driver
     iter = iter + n
     ierr = echo_get_scalars(grid, s)
     write(*,'(a,i8,a,es14.6,a,es12.4)') ' iteration', iter, '  t', s(1), '  dt', s(2)
+    if (temp_end > 0) then
+      ! check_temp at the top of the time loop (echo.f90:189-190): the grid
+      ! stops stepping; the frame of the last state (echo.f90:264-272)
+      ierr = echo_get_stop(grid, stopped, maxv)
+      if (ierr /= 0) call fail('echo_get_stop', ierr)
+      if (stopped /= 0) then
+        if (freeze_kind == 0) then
+          write(*,*) 'Temperature is under the value established for stopping the simulation'
+        else
+          write(*,*) 'Energy density is under the value established for stopping the simulation'
+        end if
+        call frame(iout)
+        exit
+      end if
+    end if
     if (s(7) /= tout_prev) then
       ! an output time was reached (echo.f90:243-248; the grid advanced tout
-      ! itself): the restart dump
+      ! itself): the frame and the restart dump
+      call frame(iout)
       if (restart_type /= 0) then
         ierr = echo_dump(grid, trim(dump_dir)//'/dump.dat'//c_null_char, iout, thyp)
         if (ierr /= 0) call fail('echo_dump', ierr)
@@ -237,9 +347,66 @@ program echo_amd_driver
     close(31)
     write(*,*) 'state written to ', trim(dump_file)
   end if
+  ierr = echo_wait_frames(grid)
+  if (ierr /= 0) call fail('echo_wait_frames', ierr)
   call echo_destroy(grid)
 
 contains
+
+  ! out_vars: a frame with the selected fields when any is selected
+  subroutine frame(io)
+    integer(c_int32_t), intent(in) :: io
+    if (all(sel == 0)) return
+    ierr = echo_write_frame(grid, sel, int(outp_prec, c_int32_t), io, trim(outdir)//c_null_char)
+    if (ierr /= 0) call fail('echo_write_frame', ierr)
+  end subroutine frame
+
+  ! INIT_TYPE 3: the events of generate_events (echo_glauber_generate into
+  ! GLAUB_DIR), event EV_START read back, its energy deposition on the device
+  ! (echo_init_glauber; ybeam = log(RADS / m_N), glaubermc.f90:741)
+  subroutine glauber_init()
+    type(echo_glauber_sampler) :: smp
+    type(echo_glauber_event) :: ev
+    integer(c_int32_t) :: nev
+    integer, parameter :: cap = 600
+    real(c_double), allocatable, target :: pxy(:), sxy(:), cxy(:)
+    integer(c_int32_t), allocatable, target :: pty(:), sty(:)
+    real(8), parameter :: am = 0.93891897d0   ! nucleon mass, common.f90:54
+    select case (trim(nucleus))
+    case ('Au'); smp = echo_glauber_sampler(197, 79, 0, 0, 0, 0, 6.38d0, 0.535d0, 0.1693d0, 0d0, 0d0, 20d0)
+    case ('Pb'); smp = echo_glauber_sampler(208, 82, 0, 0, 0, 0, 6.62d0, 0.546d0, 0.1604d0, 0d0, 0d0, 20d0)
+    case default
+      write(*,*) 'NUCLEUS ', trim(nucleus), ' is not supported (Au, Pb of nuclear_data.dat)'
+      stop 1
+    end select
+    smp%nconf = int(nconf, c_int32_t)
+    smp%nbcoll = int(nbcoll, c_int32_t)
+    smp%min_participants = int(min_part, c_int32_t)
+    smp%fixed_b = int(fixed_b, c_int32_t)
+    smp%sigma_nn = sigma_in
+    smp%bimpact = bimpact
+    ierr = echo_glauber_generate(smp, seed, trim(glauber_dir)//c_null_char, nev)
+    if (ierr /= 0) call fail('echo_glauber_generate', ierr)
+    if (ev_start < 1 .or. ev_start > nev) then
+      write(*,*) 'EV_START ', ev_start, ' but ', nev, ' events were generated'
+      stop 1
+    end if
+    allocate(pxy(2 * cap), sxy(2 * cap), cxy(2 * cap * cap / 4), pty(cap), sty(cap))
+    ev%part_xy = c_loc(pxy)
+    ev%part_type = c_loc(pty)
+    ev%spec_xy = c_loc(sxy)
+    ev%spec_type = c_loc(sty)
+    ev%coll_xy = c_loc(cxy)
+    ierr = echo_glauber_read_event(trim(glauber_dir)//c_null_char, int(ev_start, c_int32_t), ev, &
+                                   int(cap, c_int32_t), int(cap, c_int32_t), int(cap * cap / 8, c_int32_t))
+    if (ierr /= 0) call fail('echo_glauber_read_event', ierr)
+    write(*,'(a,i6,a,f8.3,a,i5,a,i6)') ' Glauber event', ev_start, '  b', ev%b, '  participants', &
+        ev%n_part_a + ev%n_part_b, '  collisions', ev%n_coll
+    gpar%ybeam = log(rads / am)
+    ierr = echo_init_glauber(grid, gpar, pxy, ev%n_part_a, pxy(2 * ev%n_part_a + 1:), ev%n_part_b, &
+                             cxy, ev%n_coll)
+    if (ierr /= 0) call fail('echo_init_glauber', ierr)
+  end subroutine glauber_init
 
   subroutine fail(what, code)
     character(len=*), intent(in) :: what

@@ -17,7 +17,8 @@ module echo_amd_f
             echo_write_frame, echo_wait_frames, echo_fo_setup, echo_fo_slice, echo_fo_width, &
             echo_fo_elements, echo_fo_write, echo_set_dump, echo_dump, echo_restart, &
             echo_glauber_params, echo_bfield_params, echo_nucleons, echo_init_glauber, &
-            echo_init_glauber_mhd
+            echo_init_glauber_mhd, echo_glauber_sampler, echo_glauber_event, echo_glauber_generate, &
+            echo_glauber_read_event, echo_set_bulk_peak
 
   ! struct echo_config: 16 int32 then 19 double (param.dat keys in brackets)
   type, bind(C) :: echo_config
@@ -55,6 +56,20 @@ module echo_amd_f
   type, bind(C) :: echo_glauber_params
     real(c_double) :: kappa, ah, sig_mc, ybeam, deta, sigeta, rhocenter
   end type echo_glauber_params
+
+  ! struct echo_glauber_sampler: generate_events (NUCLEUS data, SIGMA_IN in mb,
+  ! B, FIXED_B, NCONF, NBCOLL, MIN_PART)
+  type, bind(C) :: echo_glauber_sampler
+    integer(c_int32_t) :: A, Z, nconf, nbcoll, min_participants, fixed_b
+    real(c_double) :: radius, diffuse, rho0, sigma_nn, bimpact, bmax
+  end type echo_glauber_sampler
+
+  ! struct echo_glauber_event: one event as read back (caller's arrays)
+  type, bind(C) :: echo_glauber_event
+    real(c_double) :: b
+    integer(c_int32_t) :: n_part_a, n_part_b, n_spec_a, n_spec_b, n_coll
+    type(c_ptr) :: part_xy, part_type, spec_xy, spec_type, coll_xy
+  end type echo_glauber_event
 
   ! struct echo_bfield_params: generate_B_field_profile (gamma_col, B_amp_fac, PW, MAGFIELDT)
   type, bind(C) :: echo_bfield_params
@@ -219,6 +234,25 @@ module echo_amd_f
       type(c_ptr), value :: grid
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: header
+    end function
+    integer(c_int) function echo_glauber_generate(smp, seed, dir, n_events) bind(C, name="echo_glauber_generate")
+      import :: c_int, c_int32_t, c_int64_t, c_char, echo_glauber_sampler
+      type(echo_glauber_sampler), intent(in) :: smp
+      integer(c_int64_t), value :: seed
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int32_t), intent(out) :: n_events
+    end function
+    integer(c_int) function echo_glauber_read_event(dir, iev, ev, cap_part, cap_spec, cap_coll) &
+        bind(C, name="echo_glauber_read_event")
+      import :: c_int, c_int32_t, c_char, echo_glauber_event
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int32_t), value :: iev, cap_part, cap_spec, cap_coll
+      type(echo_glauber_event), intent(inout) :: ev
+    end function
+    integer(c_int) function echo_set_bulk_peak(grid, zs_max, t_c, width) bind(C, name="echo_set_bulk_peak")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: grid
+      real(c_double), value :: zs_max, t_c, width
     end function
     integer(c_int) function echo_init_glauber(grid, p, part_a, n_a, part_b, n_b, coll, n_coll) &
         bind(C, name="echo_init_glauber")

@@ -131,3 +131,69 @@ def test_fortran_driver_restart(oracle, tmp_path):
     vo, uo = oracle.state()
     assert rel_linf(v, vo) <= 1e-10
     assert rel_linf(u, uo) <= 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
+def test_fortran_driver_glauber(oracle, tmp_path):
+    """INIT_TYPE=3: the driver generates Au-Au events (echo_glauber_generate,
+    SEED 9, b = 6 fm fixed), reads event 1 back, deposits it on the device
+    and runs viscous hydro with the tabulated EoS; it writes frames of the
+    selected fields at the start and at the output times.  The final state
+    equals the oracle's run from the same event (read in Python with
+    echo_glauber_read_event) to 1e-10.  A second run with TEMP_END above
+    the largest energy density stops before its first step and writes its
+    last frame."""
+    from echoamd import glauber_read_event
+    from libs import make_config, oracle_glauber_prim
+    n, nz, steps = 40, 8, 4
+    gdir = tmp_path / "events"
+    gdir.mkdir()
+    out = tmp_path / "frames"
+    out.mkdir()
+    dump = tmp_path / "state.bin"
+    base = ["INIT_TYPE=3", "COORD....=2", "VISCOUS..=1", "EOS......=3", "EOS_FILE.=qcdIEOS0.dat",
+            f"NX.......={n}", f"NY.......={n}", f"NZ.......={nz}", "XMIN.....=-8.0", "XMAX.....=8.0",
+            "YMIN.....=-8.0", "YMAX.....=8.0", "ZMIN.....=-2.0", "ZMAX.....=2.0", "TSTART...=0.6",
+            "CFL......=0.4", "MAXDT....=0.05", "REC_ALGO.=WENO5", "INT_ALGO.=RK3", "bound_con=2",
+            "ETA_S....=0.1", "NUCLEUS..=Au", "SIGMA_IN.=42.", "B........=6.0", "FIXED_B..=1", "NCONF....=1",
+            "KAPPA....=19.", "AH.......=0.15", "SIG......=0.4", "DETA.....=2.0", "SIGETA...=0.8",
+            "RHOCENTER=1.", "RADS.....=200.", "SEED.....=9", f"GLAUB_DIR={gdir}", "DTOUT....=0.04",
+            f"outdir...={out}", "density..=1", "pressure.=1", "temper...=1", "OUTP_PREC=8"]
+    pf = tmp_path / "param.dat"
+    pf.write_text("\n".join(base + [f"NSTEPS...={steps}", "CHUNK....=1", f"DUMP_FILE={dump}"]) + "\n")
+    r = subprocess.run([DRIVER, str(pf)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-800:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Glauber event" in r.stdout
+    ev = glauber_read_event(gdir, 1)
+    assert abs(ev["b"] - 6.0) < 1e-12
+    ic, rc = make_config(nx=n, ny=n, nz=nz, coordinates=2, viscosity=1, eqstate=3, rec="WENO5", nrk=3, bc=2,
+                         init_type=3, xlim=(-8.0, 8.0), ylim=(-8.0, 8.0), zlim=(-2.0, 2.0), tmin=0.6,
+                         cfl=0.4, maxdt=0.05, eta_over_s=0.1, dtout=0.04)
+    par = dict(kappa=19.0, ah=0.15, sig_mc=0.4, ybeam=np.log(200.0 / 0.93891897), deta=2.0, sigeta=0.8,
+               rhocenter=1.0)
+    oracle.setup(ic, rc)
+    e, v0 = oracle_glauber_prim(oracle, par, ev["part_a"], ev["part_b"], ev["coll"])
+    assert e == 0 and oracle.init_from_prim(v0) == 0
+    assert oracle.steps(steps) == 0
+    vo, uo = oracle.state()
+    raw = np.fromfile(dump, dtype=np.float64)
+    m = 12 * n * n * nz
+    v = raw[:m].reshape(12, nz, n, n)
+    u = raw[m:].reshape(12, nz, n, n)
+    assert rel_linf(v, vo) <= 1e-10 and rel_linf(u, uo) <= 1e-10
+    frames = sorted(p.name for p in out.glob("out*.dat"))
+    assert frames[0] == "out0000.dat" and len(frames) >= 2
+    # t, then density, pressure, temperature: 3 fields of n*n*nz FP64
+    assert (out / "out0000.dat").stat().st_size == 8 * (1 + 3 * n * n * nz)
+    # TEMP_END above every cell's energy density (FREEZKIND 1): no step
+    out2 = tmp_path / "frames2"
+    out2.mkdir()
+    pf2 = tmp_path / "param2.dat"
+    pf2.write_text("\n".join([x for x in base if not x.startswith("outdir")]
+                             + [f"outdir...={out2}", "TEMP_END.=1.e6", "NSTEPS...=3"]) + "\n")
+    r = subprocess.run([DRIVER, str(pf2)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Energy density is under" in r.stdout
+    assert (out2 / "out0001.dat").exists()
