@@ -162,7 +162,7 @@ def design_bytes(nx, ny, nz, nv, resistive):
     return flux, upd
 
 
-def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=16, steps=1):
+def cpu_baseline(ic_full, rc_full, eos_rows, nz_sample=32, steps=1):
     """The C oracle (the FP64 restatement, oracle/liboracle.so), single
     thread, on a 192 x 192 x nz_sample slab of the same workload."""
     from libs import Oracle
