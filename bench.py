@@ -268,52 +268,60 @@ def c2p_leg(n_states, seed=99):
     return out
 
 
-def ensemble_leg(n_events, steps, eos_rows, seed=2026):
+def ensemble_leg(n_events, steps, eos_rows, seed=2026, resistive=True):
     """Event-by-event ensemble: n_events Au-Au events with impact parameter
     b ~ U[0, 12] fm, each on a grid sized to its fireball (transverse
     96^2..192^2 cells of 0.15 fm, 64..128 eta cells of 0.1), viscous
-    resistive MHD as the single-event run with RK3, all advanced together
-    (one captured graph per step, autotuned tiles).  Value: total cells x
-    steps / wall time."""
-    from echoamd import Ensemble
+    resistive MHD (or viscous hydro) as the single-event run with RK3, all
+    advanced together (one captured graph per step, autotuned tiles).
+    Value: total cells x steps / wall time.  The initial states are built
+    one event at a time, so that the host holds one at once."""
+    from echoamd import EchoGrid, Ensemble
     rng = np.random.default_rng(seed)
     bs = rng.uniform(0.0, 12.0, n_events)
-    cfgs, states = [], []
-    for k, b in enumerate(bs):
+    cfgs = []
+    for b in bs:
         f = 1.0 - b / 12.0
         nt = int(round((96 + 96 * f) / 16)) * 16
         ne = int(round((64 + 64 * f) / 16)) * 16
-        ic, rc = workload_config(nt, nt, ne, nrk=3)
-        v, _ = workload_state(ic, rc, eos_rows, seed=seed + k, b=b)
-        cfgs.append((ic, rc))
-        states.append(v)
+        cfgs.append(workload_config(nt, nt, ne, nrk=3, resistive=resistive))
+
+    def state(k):
+        ic, rc = cfgs[k]
+        return workload_state(ic, rc, eos_rows, seed=seed + k, b=bs[k])[0]
+
+    t0 = time.perf_counter()
     ens = Ensemble(cfgs, eos_rows=eos_rows)
-    for g, v in zip(ens.grids, states):
-        g.init_from_prim(v)
+    for k, g in enumerate(ens.grids):
+        g.init_from_prim(state(k))
+        if (k + 1) % 32 == 0:
+            print(f"ensemble: {k + 1} of {n_events} events set up", file=sys.stderr, flush=True)
+    setup_s = time.perf_counter() - t0
     ens.steps(1)
     ens.tune(reps=2)
     ens.steps(1)
     crashed = ens.steps(steps)
     ms = ens.last_ms()
+    print(f"ensemble: {steps} steps in {ms:.1f} ms", file=sys.stderr, flush=True)
     cells = sum(int(ic[0]) * int(ic[1]) * int(ic[2]) for ic, _ in cfgs)
     tiles = [g.tile().tolist() for g in ens.grids]
     ens.close()
     # the same events one at a time (one grid and one graph replay each)
-    from echoamd import EchoGrid
-    grids = [EchoGrid(ic, rc, eos_rows=eos_rows) for ic, rc in cfgs]
-    for g, v in zip(grids, states):
-        g.init_from_prim(v)
+    single_ms = 0.0
+    for k, (ic, rc) in enumerate(cfgs):
+        g = EchoGrid(ic, rc, eos_rows=eos_rows)
+        g.init_from_prim(state(k))
         g.steps(1)
         g.tune(reps=2)
         g.steps(1)
-    del states
-    t0 = time.perf_counter()
-    for g in grids:
+        t0 = time.perf_counter()
         g.steps(steps)
-    single_ms = (time.perf_counter() - t0) * 1e3
-    for g in grids:
+        single_ms += (time.perf_counter() - t0) * 1e3
         g.close()
-    return {"events": n_events, "cells": cells, "steps": steps, "ms": ms,
+        if (k + 1) % 32 == 0:
+            print(f"ensemble: {k + 1} of {n_events} events stepped one at a time", file=sys.stderr, flush=True)
+    return {"events": n_events, "system": "viscous resistive MHD (nv 19)" if resistive else "viscous hydro (nv 12)",
+            "cells": cells, "steps": steps, "ms": ms, "setup_s": setup_s,
             "zone_updates_per_s": cells * steps / (ms * 1e-3), "crashed": crashed,
             "one_at_a_time_zone_updates_per_s": cells * steps / (single_ms * 1e-3),
             "speedup_vs_one_at_a_time": single_ms / ms,
@@ -362,6 +370,9 @@ def main():
                     help="states of the learned-con2prim leg (0: skip)")
     ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--ensemble-viscous", action="store_true",
+                    help="ensemble of viscous-hydro events (nv 12) instead of viscous resistive MHD (nv 19): "
+                         "256 events of the large-ensemble workload fit in HBM only this way")
     ap.add_argument("--plain", action="store_true",
                     help="time with plain kernel launches instead of graph replays "
                          "(for counter collection with rocprofv3 --pmc)")
@@ -435,7 +446,8 @@ def main():
     if args.c2p_states > 0:
         out["c2p"] = c2p_leg(args.c2p_states)
     if args.events > 0:
-        out["ensemble"] = ensemble_leg(args.events, args.ensemble_steps, eos_rows)
+        out["ensemble"] = ensemble_leg(args.events, args.ensemble_steps, eos_rows,
+                                       resistive=not args.ensemble_viscous)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, rc, eos_rows)
     print(json.dumps(out), flush=True)
