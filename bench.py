@@ -96,7 +96,7 @@ AU197 = dict(nucleus="Au", sigma_nn=42.0)
 PB208 = dict(nucleus="Pb", sigma_nn=67.6)
 
 
-def workload_state(ic, rc, eos_rows, seed=12345, b=7.0, nucleus=AU197):
+def workload_state(ic, rc, eos_rows, seed=12345, b=7.0, nucleus=AU197, ns_shear=True):
     nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
     dx = (rc[1] - rc[0]) / nx
     tau0 = rc[6]
@@ -117,10 +117,12 @@ def workload_state(ic, rc, eos_rows, seed=12345, b=7.0, nucleus=AU197):
     v = np.zeros((nv, nz, ny, nx))
     v[0] = 1.0
     v[4] = p
-    # Navier-Stokes initial shear of viscous_initio (viscous.f90:101-136)
-    v[9] = 2. * eta_v / (3. * tau0)
-    v[10] = 2. * eta_v / (3. * tau0)
-    v[11] = -tau0 ** 2 * 4. * eta_v / (3. * tau0)
+    # Navier-Stokes initial shear of viscous_initio (viscous.f90:101-136),
+    # or none (the fluid starts in equilibrium)
+    if ns_shear:
+        v[9] = 2. * eta_v / (3. * tau0)
+        v[10] = 2. * eta_v / (3. * tau0)
+        v[11] = -tau0 ** 2 * 4. * eta_v / (3. * tau0)
     if nv == 19:
         x = (np.arange(nx) + 0.5) * dx - nx * dx / 2
         y = (np.arange(ny) + 0.5) * dx - ny * dx / 2
@@ -355,6 +357,56 @@ def side_leg(nx, ny, nz, eos_rows, steps, **kw):
             "crashed": crashed}
 
 
+def large_event_leg(steps, eos_rows, nx=384, ny=384, nz=256, chunk=50, maxdt=0.02):
+    """The large single event: Pb-Pb (Woods-Saxon R = 6.62 fm, a = 0.546 fm,
+    sigma_NN = 67.6 mb, b = 5 fm) on 384 x 384 x 256 cells (dx = 0.08 fm,
+    deta = 0.05), viscous resistive MHD with eta/s = 0.12 and zeta/s peaked
+    at T_c (0.08 at 155 MeV, width 20 MeV), the dipole field, IMEX, starting
+    without viscous stresses (with the Navier-Stokes shear of the floor, the
+    near-vacuum edges of this finer grid fail a con2prim repair within 30
+    steps), MAXDT 0.02 fm (with 0.1 the light-speed step reaches 0.032 fm and
+    the expanding near-vacuum corners fail a repair at tau = 13.2 fm, step
+    400); graph-replayed steps in chunks (a progress line per chunk).  A run
+    that stops reports where."""
+    from libs import make_config
+    from echoamd import EchoError, EchoGrid
+    dx, deta = 0.08, 0.05
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, viscosity=1, bulk=1, eqstate=3, rec="WENO5",
+                         nrk=4, bc=2, obtained_zz=1, init_type=3, mhd=2, divclean=1, algo_solver=7,
+                         xlim=(-nx * dx / 2, nx * dx / 2), ylim=(-ny * dx / 2, ny * dx / 2),
+                         zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4, cfl=0.4, maxdt=maxdt,
+                         eta_over_s=0.12, tau_pi_coeff=5.0, enezero=0.001, sigma_el=0.0058, sigma_chi=0.0)
+    v0, npart = workload_state(ic, rc, eos_rows, seed=424242, b=5.0, nucleus=PB208, ns_shear=False)
+    g = EchoGrid(ic, rc, device=0, eos_rows=eos_rows)
+    g.set_bulk_peak(0.08, 0.155, 0.02)
+    g.init_from_prim(v0)
+    del v0
+    g.steps(1)
+    g.tune(reps=2)
+    wall, done, stop = 0.0, 0, None
+    while done < steps:
+        n = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        try:
+            g.steps(n)
+        except EchoError as err:
+            stop = str(err)
+            break
+        wall += time.perf_counter() - t0
+        done += n
+        print(f"large event: {done} of {steps} steps, t = {g.scalars()['t']:.4f} fm", file=sys.stderr, flush=True)
+    sc = g.scalars()
+    g.close()
+    N = nx * ny * nz
+    out = {"workload": f"Pb-Pb b=5 fm, {nx}x{ny}x{nz}, viscous resistive MHD (nv 19), eta/s 0.12, zeta/s "
+                       f"peaked at T_c, dipole B, IMEX, MAXDT {maxdt}", "participants": npart, "steps": done,
+           "ms_per_step": 1e3 * wall / done if done else None,
+           "zone_updates_per_s": N * done / wall if done else None, "t_end": sc["t"], "crashed": sc["crashed"]}
+    if stop:
+        out["stopped"] = stop
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -370,6 +422,8 @@ def main():
                     help="states of the learned-con2prim leg (0: skip)")
     ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--large-steps", type=int, default=0,
+                    help="steps of the large Pb-Pb event on 384x384x256 (0: skip)")
     ap.add_argument("--ensemble-viscous", action="store_true",
                     help="ensemble of viscous-hydro events (nv 12) instead of viscous resistive MHD (nv 19): "
                          "256 events of the large-ensemble workload fit in HBM only this way")
@@ -443,6 +497,8 @@ def main():
     if not args.no_legs:
         out["rk3_resistive"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3)
         out["viscous_hydro_rk3"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3, resistive=False)
+    if args.large_steps > 0:
+        out["large_event"] = large_event_leg(args.large_steps, eos_rows)
     if args.c2p_states > 0:
         out["c2p"] = c2p_leg(args.c2p_states)
     if args.events > 0:
