@@ -257,9 +257,12 @@ def c2p_leg(n_states, seed=99):
             "ms": best[0] + best[1], "ms_network_newton": best[0], "ms_fallback": best[1],
             "newton_iters": iters, "gate_tol": 1e-11, "fallback": int(nfb), "max_err": mx,
             "frac_err_above_1e-10": frac}
-    # headline: two Newton iterations; the states it sends to the plain solver
-    # carry the plain solver's error
-    out["surrogate"] = out["surrogate_2it"]
+    # headline: the setting that meets 1e-10 on every state (the metric is
+    # the speedup at 1e-10 error): three Newton iterations, which also leave
+    # no state to the plain solver; with two, the fallback states carry the
+    # plain solver's error
+    ok3 = out["surrogate_3it"]["frac_err_above_1e-10"] == 0.0
+    out["surrogate"] = out["surrogate_3it"] if ok3 else out["surrogate_2it"]
     ms = out["surrogate"]["ms"]
     out["speedup_vs_plain_cold"] = out["plain_cold"]["ms"] / ms
     out["speedup_vs_plain_warm"] = out["plain_warm"]["ms"] / ms
