@@ -171,3 +171,34 @@ def test_unsupported_options_rejected():
     with pytest.raises(EchoError, match="restart dump"):
         g.set_dump(True)
     g.close()
+
+
+def test_resistive_events_in_ensemble(oracle):
+    """Resistive grids (nv 12 and 19, IMEX and RK3, their flux buffers with
+    the extra div E slot) in one ensemble with a viscous-hydro grid, on the
+    shared face pool: each equals its own single grid bit for bit, and the
+    viscous resistive one the oracle."""
+    from echoamd import EchoGrid, Ensemble
+    from scenarios import fireball3d
+    events = [rmhd_fireball(viscosity=1, nrk=4), rmhd_fireball(12, 10, 8, viscosity=0, nrk=3),
+              fireball3d(20, 16, 12, viscosity=1, eqstate=3)]
+    ens = Ensemble([(ic, rc) for ic, rc, _ in events])
+    for g, (ic, rc, v0) in zip(ens.grids, events):
+        assert g.init_from_prim(v0) == 0
+    assert ens.steps(3) == 0
+    for i, (g, (ic, rc, v0)) in enumerate(zip(ens.grids, events)):
+        ve, ue = g.state()
+        s = EchoGrid(ic, rc)
+        s.init_from_prim(v0)
+        s.steps(3)
+        vs, us = s.state()
+        s.close()
+        assert np.array_equal(ve, vs) and np.array_equal(ue, us), i
+    ic, rc, v0 = events[0]
+    oracle.setup(ic, rc)
+    assert oracle.init_from_prim(v0) == 0
+    assert oracle.steps(3) == 0
+    vo, uo = oracle.state()
+    ve, ue = ens.grids[0].state()
+    assert rel_linf(ve, vo) <= TOL and rel_linf(ue, uo) <= TOL
+    ens.close()
