@@ -360,6 +360,83 @@ def side_leg(nx, ny, nz, eos_rows, steps, **kw):
             "crashed": crashed}
 
 
+def learned_step_leg(steps, eos_rows, nx=192, ny=192, nz=128, iters=3, tol=1e-12):
+    """The learned inversion inside the step.  Ideal MHD with p = e/3 (the
+    system the network covers: GLM, algorithm 5), RK3, WENO5 + HLL, on the
+    workload's grid with its initial energy density (p = e/3, no viscous
+    stresses) and dipole field, stepped from the same state with the plain
+    inversion and with the network (guess, `iters` Newton iterations, gate
+    `tol`): ms per step of both (graph replays), the cell update's time
+    (HIP events; with the network it includes the network, refinement and
+    fallback kernels), the fraction of inversions left to the plain solver,
+    and the largest relative difference of the two final states."""
+    from libs import make_config
+    from echoamd import C2pNet, EchoGrid
+    from echoamd.surrogate import load_weights
+
+    def rel_linf(a, b):
+        # max |a - b| over each variable against max |b| over its group (the
+        # components of v and of B share their vector's scale; a group that
+        # is zero is measured in absolute terms)
+        out = 0.0
+        for grp in ([0], [1, 2, 3], [4], [5, 6, 7], [8]):
+            scale = max(float(np.abs(b[k]).max()) for k in grp)
+            for k in grp:
+                d = float(np.abs(a[k] - b[k]).max())
+                out = max(out, d / scale if scale > 0 else d)
+        return out
+    dx, deta = 0.15, 0.1
+    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, mhd=1, divclean=1, eqstate=1, rec="WENO5",
+                         nrk=3, bc=2, algo_solver=5, xlim=(-nx * dx / 2, nx * dx / 2),
+                         ylim=(-ny * dx / 2, ny * dx / 2), zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4,
+                         cfl=0.4, maxdt=0.1)
+    cfg_v, _ = workload_config(nx, ny, nz)
+    v19, _ = workload_state(cfg_v, rc, eos_rows, ns_shear=False)
+    # the same energy density with p = e/3
+    hc3 = .00768351
+    T = eos_rows[:, 0]
+    e = np.interp(v19[4], eos_rows[:, 2] / hc3 * T ** 4, eos_rows[:, 1] / hc3 * T ** 4)
+    v0 = np.zeros((9, nz, ny, nx))
+    v0[0] = 1.0
+    v0[4] = e / 3.0
+    v0[5] = v19[12]
+    v0[6] = v19[13]
+    del v19
+    N = nx * ny * nz
+    net = C2pNet(0, load_weights())
+    res = {}
+    states = {}
+    for name in ("plain", "learned"):
+        g = EchoGrid(ic, rc, device=0)
+        g.init_from_prim(v0)
+        if name == "learned":
+            g.set_c2p_net(net, iters=iters, tol=tol)
+        g.steps(1)
+        g.tune(reps=2)
+        wall = time_steps(g, steps, 2)
+        g.set_timing(True)
+        g.steps(2)
+        upd = g.timing()["update"]
+        g.set_timing(False)
+        res[name] = {"ms_per_step": 1e3 * wall / steps, "zone_updates_per_s": N * steps / wall,
+                     "update_ms": upd, "crashed": g.scalars()["crashed"]}
+        if name == "learned":
+            # inversions: three stages of each step (1 + 2 + steps + 2; the
+            # tuner times sweeps only)
+            res[name]["fallbacks"] = g.c2p_fallbacks()
+            res[name]["fallback_fraction"] = res[name]["fallbacks"] / (3.0 * N * (steps + 5))
+        states[name] = g.state()
+        g.close()
+        print(f"learned-step leg: {name} {res[name]['ms_per_step']:.3f} ms per step", file=sys.stderr, flush=True)
+    net.close()
+    (vp, up), (vl, ul) = states["plain"], states["learned"]
+    return {"workload": f"ideal MHD p=e/3 (GLM, algorithm 5), {nx}x{ny}x{nz}, RK3, WENO5+HLL, the headline's "
+                        "initial energy density and dipole field",
+            "iters": iters, "gate_tol": tol, "plain": res["plain"], "learned": res["learned"],
+            "step_speedup": res["plain"]["ms_per_step"] / res["learned"]["ms_per_step"],
+            "rel_diff_v": rel_linf(vl, vp), "rel_diff_u": rel_linf(ul, up)}
+
+
 def large_event_leg(steps, eos_rows, nx=384, ny=384, nz=256, chunk=50, maxdt=0.02):
     """The large single event: Pb-Pb (Woods-Saxon R = 6.62 fm, a = 0.546 fm,
     sigma_NN = 67.6 mb, b = 5 fm) on 384 x 384 x 256 cells (dx = 0.08 fm,
@@ -425,6 +502,8 @@ def main():
                     help="states of the learned-con2prim leg (0: skip)")
     ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--learned-steps", type=int, default=20,
+                    help="timed steps of the learned-inversion-in-the-step leg (ideal MHD p = e/3; 0: skip)")
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of the large Pb-Pb event on 384x384x256 (0: skip)")
     ap.add_argument("--ensemble-viscous", action="store_true",
@@ -500,6 +579,8 @@ def main():
     if not args.no_legs:
         out["rk3_resistive"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3)
         out["viscous_hydro_rk3"] = side_leg(nx, ny, nz, eos_rows, 20, nrk=3, resistive=False)
+    if args.learned_steps > 0:
+        out["learned_step"] = learned_step_leg(args.learned_steps, eos_rows, nx, ny, nz)
     if args.large_steps > 0:
         out["large_event"] = large_event_leg(args.large_steps, eos_rows)
     if args.c2p_states > 0:

@@ -121,6 +121,31 @@ bool surrogate_config_ok(const echo_config& c) {
 
 }  // namespace
 
+// The learned inversion inside a grid's step (ideal MHD, after k_update
+// wrote the stage's u and left V at the stage start): the network's guess,
+// Newton and the gate for every cell, the plain solver from V for the cells
+// the gate rejects, failures marked for the repair that follows.  The
+// counter is cleared by a kernel, so the launches can be captured.
+void enqueue_net_c2p(echo_grid* g, const double* U) {
+  echo_mlp* m = g->net;
+  int* cnt = g->net_scratch;
+  const long long N = g->N;
+  mlp::k_zero_int<<<1, 64, 0, g->stream>>>(cnt, 1);
+  mlp::C2pArgs ca{g->cfg.coordinates, g->net_iters, g->net_tol, g->S, cnt, cnt + 1, 1};
+  const int nb = mlp_grid_blocks(g->dev, N);
+  int ncu = 256;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev);
+  if (g->cfg.divclean) {
+    mlp::k_mlp<mlp::MODE_C2P, 9><<<nb, mlp::BLOCK, 0, g->stream>>>(net_args(m), ca, U, N, 0, N, g->V, nullptr,
+                                                                   nullptr, nullptr, nullptr, nullptr);
+    mlp::k_c2p_fallback_step<SYS_MHDGLM><<<ncu, 256, 0, g->stream>>>(g->ga, U, g->V, g->fix, cnt + 1, cnt);
+  } else {
+    mlp::k_mlp<mlp::MODE_C2P, 8><<<nb, mlp::BLOCK, 0, g->stream>>>(net_args(m), ca, U, N, 0, N, g->V, nullptr,
+                                                                   nullptr, nullptr, nullptr, nullptr);
+    mlp::k_c2p_fallback_step<SYS_MHD><<<ncu, 256, 0, g->stream>>>(g->ga, U, g->V, g->fix, cnt + 1, cnt);
+  }
+}
+
 extern "C" {
 
 int echo_mlp_create(int32_t device, echo_mlp** out) {
@@ -278,7 +303,7 @@ static int surrogate_run(echo_grid* g, echo_mlp* m, const double* d_u, double* d
                          int64_t n, int32_t iters, double tol, int64_t* n_fallback, int* fail_count,
                          int* fail_list) {
   mlp::k_zero_int<<<1, 64, 0, g->stream>>>(fail_count, 1);
-  mlp::C2pArgs ca{g->cfg.coordinates, iters, tol, g->S, fail_count, fail_list};
+  mlp::C2pArgs ca{g->cfg.coordinates, iters, tol, g->S, fail_count, fail_list, 0};
   const int nb = mlp_grid_blocks(m->dev, n);
   if (g->timing) HIPCHECK(hipEventRecord(g->ev[0], g->stream));
   if (g->cfg.divclean)
@@ -322,6 +347,45 @@ static int surrogate_check(echo_grid* g, echo_mlp* m, const double* d_u, double*
     return fail(ECHO_E_ARG, "the learned inversion covers ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1)");
   if (iters < 0 || iters > 8 || !(tol > 0.0)) return fail(ECHO_E_ARG, "need 0 <= iters <= 8 and tol > 0");
   if (n > 0x7fffffffLL) return fail(ECHO_E_ARG, "at most 2^31-1 states per call");
+  return 0;
+}
+
+int echo_set_c2p_net(echo_grid* g, echo_mlp* m, int32_t iters, double tol) {
+  if (!g) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_c2p_net: the grid is inside a step");
+  if (!m) {
+    g->net = nullptr;
+    g->ga.ext_c2p = 0;
+    hipFree(g->net_scratch);
+    g->net_scratch = nullptr;
+  } else {
+    if (g->dev != m->dev) return fail(ECHO_E_ARG, "network and grid on different devices");
+    if (!surrogate_config_ok(g->cfg))
+      return fail(ECHO_E_ARG, "the learned inversion covers ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1)");
+    if (iters < 0 || iters > 8 || !(tol > 0.0)) return fail(ECHO_E_ARG, "need 0 <= iters <= 8 and tol > 0");
+    if (g->N > 0x7fffffffLL) return fail(ECHO_E_ARG, "at most 2^31-1 cells");
+    if (!g->net_scratch && hipMalloc(&g->net_scratch, sizeof(int) * (size_t)(g->N + 1)) != hipSuccess) {
+      g->net_scratch = nullptr;
+      return fail(ECHO_E_NOMEM, "echo_set_c2p_net: device allocation failed");
+    }
+    g->net = m;
+    g->net_iters = iters;
+    g->net_tol = tol;
+    g->ga.ext_c2p = 1;
+  }
+  drop_graph(g);
+  if (g->graph_dirty) *g->graph_dirty = true;
+  return 0;
+}
+
+int echo_get_c2p_fallbacks(echo_grid* g, int64_t* n) {
+  if (!g || !n) return fail(ECHO_E_ARG, "null argument");
+  HIPCHECK(hipSetDevice(g->dev));
+  echo::DevScalars S;
+  HIPCHECK(hipMemcpyAsync(&S, g->S, sizeof S, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  *n = S.nfallback;
   return 0;
 }
 

@@ -96,6 +96,7 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   k_update<SYS, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
       g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
       g->VS[0], g->VS[1], g->fix, c1, c2, irk == 1 ? 1 : 0, last ? 1 : 0);
+  if (g->net) enqueue_net_c2p(g, Uout);   // ideal MHD with the learned inversion
   if (g->timing) hipEventRecord(g->ev[5], g->stream);
   // con2prim repair of the cells k_update could not invert (rare; returns at
   // once when there are none); it writes the next step's vnewstep too,
@@ -175,13 +176,6 @@ static void enqueue_flux(echo_grid* g, int d) {
     else if (d == 1) launch_flux_dir<2, S, Z>(g, part[1]);
     else launch_flux_dir<3, S, Z>(g, part[2]);
   });
-}
-
-static void drop_graph(echo_grid* g) {
-  if (g->gexec) hipGraphExecDestroy(g->gexec);
-  if (g->graph) hipGraphDestroy(g->graph);
-  g->gexec = nullptr;
-  g->graph = nullptr;
 }
 
 static int set_tile(echo_grid* g, int d, int ty) {
@@ -282,6 +276,7 @@ static void release(echo_grid* g) {
   hipFree(g->fo_f[0]); hipFree(g->fo_f[1]); hipFree(g->fo_f[2]); hipFree(g->fo_vb); hipFree(g->fo_mask);
   hipFree(g->fo_bsum); hipFree(g->fo_status); hipFree(g->fo_rec);
   hipFree(g->dump_buf[0]); hipFree(g->dump_buf[1]);
+  hipFree(g->net_scratch);
   for (auto& e : g->ev) if (e) hipEventDestroy(e);
   if (g->stream && g->owns_stream) hipStreamDestroy(g->stream);
   delete g;
@@ -511,6 +506,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   a.res = c.mhd == 2;
   a.sig = c.sigma_el / 0.197326;
   a.sch = c.sigma_chi / 0.197326;
+  a.ext_c2p = 0;
   a.S = g->S;
   if (!share_scratch && hipMemset(g->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(g->N)) != hipSuccess) {
     release(g);

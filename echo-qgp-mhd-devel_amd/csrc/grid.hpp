@@ -103,7 +103,26 @@ struct echo_grid {
   int* fo_status = nullptr;
   double* fo_rec = nullptr;
   long long fo_cap = 0, fo_n = 0;
+  // learned con2prim inside the step (echo_set_c2p_net): the network (owned
+  // by the caller), its Newton iterations and gate tolerance, and the
+  // fallback count and list ([0] count, [1..N] cells)
+  echo_mlp* net = nullptr;
+  int net_iters = 3;
+  double net_tol = 1e-12;
+  int* net_scratch = nullptr;
 };
+
+// the learned inversion of a stage's result U into g->V after k_update
+// (defined with the network kernels, c2p_mlp.hip)
+void enqueue_net_c2p(echo_grid* g, const double* U);
+
+// forget the grid's captured step (rebuilt at the next echo_steps)
+inline void drop_graph(echo_grid* g) {
+  if (g->gexec) hipGraphExecDestroy(g->gexec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+}
 
 
 namespace echo {

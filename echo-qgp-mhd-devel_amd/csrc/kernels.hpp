@@ -24,6 +24,7 @@ struct DevScalars {
   double aflux[3];         // max(ap, am) at the last interface of each sweep
   double check_max;        // max temperature (or energy density) of the last check
   long long nrepair;       // cells repaired by k_fixup since set-up
+  long long nfallback;     // states the learned inversion in the step left to the plain solver
   int iter, run_crashed, errcode, nfail;
   int vs_parity;
   int freeze_type;         // 0: temperature, 1: energy density
@@ -63,6 +64,7 @@ struct GridArgs {
   double glm_alpha;     // GLM damping parameter
   int res;              // resistive MHD (E evolved): light-speed CFL, glm_ch = 1
   double sig, sch;      // electric and chiral conductivity, fm^-1
+  int ext_c2p;          // ideal MHD: k_update leaves con2prim to the learned inversion
   DevScalars* S;
 };
 
@@ -615,6 +617,15 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     if (last_stage) {   // evolve.f90:201
       const double ch = S->glm_ch;
       un[KG] = un[KG] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
+    }
+  }
+  if constexpr (SYS == SYS_MHD || SYS == SYS_MHDGLM) {
+    if (a.ext_c2p) {
+      // the learned inversion inverts u (k_mlp, then k_c2p_fallback_step
+      // from the primitives V still holds)
+#pragma unroll
+      for (int k = 0; k < NV; k++) __builtin_nontemporal_store(un[k], &Uout[k * N + c]);
+      return;
     }
   }
   int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);

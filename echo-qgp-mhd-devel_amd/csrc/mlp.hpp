@@ -220,6 +220,7 @@ struct C2pArgs {
   const DevScalars* S;    // time (metric)
   int* fail_count;
   int* fail_list;
+  int in_step;            // inside a grid's step: nothing to do once the run stopped
 };
 
 // Launch bounds (BLOCK, 3): three waves per SIMD (at most 168 VGPRs; the
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const
   __shared__ h16x8 sX[4][64];
   __shared__ float2 sY[4][64];
 
+  if (MODE == MODE_C2P && ca.in_step && (ca.S->run_crashed || ca.S->temp_check)) return;   // block-uniform
   {  // weights to LDS
     const uint4* src = reinterpret_cast<const uint4*>(net.packed);
     uint4* d1 = reinterpret_cast<uint4*>(sA1);
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const
 #pragma unroll
       for (int k = 0; k < 8; k++) Vb[k * stride + off + s] = v[k];
       if (NVU == 9) Vb[8 * stride + off + s] = Ub[8 * stride + off + s];
-      status[off + s] = 0;
+      if (status) status[off + s] = 0;
     }
     const bool bad = valid && !ok;
     const unsigned long long mask = __ballot(bad);
@@ -444,6 +446,37 @@ __global__ __launch_bounds__(256) void k_c2p_fallback(GridArgs a, const double* 
     status[s] = cons2prim<SYS, false>(a.eos, M, u, v, a.algo_solver);
 #pragma unroll
     for (int k = 0; k < NV; k++) Vb[k * stride + s] = v[k];
+  }
+}
+
+// The plain solver inside a grid's step, on the states the gate rejected:
+// con2prim from the primitives V holds (the stage start, as in k_update),
+// failures marked for k_fixup as k_update marks them.  Block 0 adds the
+// number of listed states to the grid's fallback count.
+template <int SYS>
+__global__ __launch_bounds__(256) void k_c2p_fallback_step(GridArgs a, const double* __restrict__ U,
+                                                           double* __restrict__ V, FixBufs fb,
+                                                           const int* __restrict__ list,
+                                                           const int* __restrict__ count) {
+  constexpr int NV = SysT<SYS>::NV;
+  DevScalars* S = a.S;
+  if (S->run_crashed || S->temp_check) return;
+  const int m = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S->nfallback += m;
+  const Metric M = make_metric(a.coordinates, S->t);
+  const long long N = a.N;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
+    const long long c = list[i];
+    double u[NV], v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) { u[k] = U[k * N + c]; v[k] = V[k * N + c]; }
+    const int e = cons2prim<SYS, false>(a.eos, M, u, v, a.algo_solver);
+    if (e != 0) {
+      atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+      atomicAdd(&S->nfail, 1);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) V[k * N + c] = v[k];
   }
 }
 

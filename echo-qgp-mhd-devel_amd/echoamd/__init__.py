@@ -37,6 +37,7 @@ EXPORTS = (
     "echo_init_glauber", "echo_set_stop", "echo_get_stop", "echo_write_frame", "echo_wait_frames",
     "echo_fo_setup", "echo_fo_slice", "echo_fo_width", "echo_fo_elements", "echo_fo_write",
     "echo_set_dump", "echo_dump", "echo_restart", "echo_init_glauber_mhd", "echo_get_repairs",
+    "echo_set_c2p_net", "echo_get_c2p_fallbacks",
 )
 
 # out_vars field flags (type out_type of the reference), in file order
@@ -198,6 +199,8 @@ def load_library(path=LIB_PATH):
         "echo_c2p_surrogate_noalloc": (C.c_int, [P, P, P, P, P, C.c_int64, C.c_int32, C.c_double,
                                                  C.POINTER(C.c_int64)]),
         "echo_set_bulk_peak": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
+        "echo_set_c2p_net": (C.c_int, [P, P, C.c_int32, C.c_double]),
+        "echo_get_c2p_fallbacks": (C.c_int, [P, C.POINTER(C.c_int64)]),
         "echo_glauber_generate": (C.c_int, [P, C.c_uint64, C.c_char_p, C.POINTER(C.c_int32)]),
         "echo_glauber_read_event": (C.c_int, [C.c_char_p, C.c_int32, P, C.c_int32, C.c_int32, C.c_int32]),
         "echo_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
@@ -509,6 +512,20 @@ class EchoGrid:
         """Cells repaired by the con2prim repair (evolve_u2v) since set-up."""
         n = C.c_int64()
         self._check(self.lib.echo_get_repairs(self.h, C.byref(n)))
+        return n.value
+
+    def set_c2p_net(self, net, iters=3, tol=1e-12):
+        """Invert each stage with the learned con2prim (C2pNet; None
+        detaches it).  Ideal MHD with p = e/3.  The grid keeps a reference
+        to the network until it is detached."""
+        self._check(self.lib.echo_set_c2p_net(self.h, None if net is None else net.h, int(iters), float(tol)))
+        self._net = net
+
+    def c2p_fallbacks(self):
+        """States of the steps that the learned inversion left to the plain
+        solver, since set-up."""
+        n = C.c_int64()
+        self._check(self.lib.echo_get_c2p_fallbacks(self.h, C.byref(n)))
         return n.value
 
     def set_tout(self, tout):

@@ -400,6 +400,21 @@ int echo_c2p_surrogate_noalloc(echo_grid *g, echo_mlp *m, const double *d_u, dou
                                int32_t *d_status, int64_t n, int32_t iters, double tol,
                                int64_t *n_fallback);
 
+/* The learned inversion inside the grid's own steps: replaces the con2prim
+ * of each RK stage (evolve_u2v, evolve.f90:769-868, algorithm 5 of
+ * system_cons2prim, system.f90:945-998) with the network's guess, iters
+ * FP64 Newton iterations and the gate of tolerance tol; the states the gate
+ * rejects go to algorithm 5 from the stage-start primitives, as without the
+ * network, and cells that fail there are repaired as usual.  Ideal MHD with
+ * p = e/3 only (mhd 1, algo_solver 5, eqstate 1).  m = NULL detaches the
+ * network.  The grid keeps the pointer: detach it (or destroy the grid)
+ * before echo_mlp_destroy.  Allocates a fallback list of N + 1 int32 on the
+ * first attach.  Not between echo_rk_stage calls of one step. */
+int echo_set_c2p_net(echo_grid *g, echo_mlp *m, int32_t iters, double tol);
+/* States of the grid's steps that the gate left to the plain solver, since
+ * set-up. */
+int echo_get_c2p_fallbacks(echo_grid *g, int64_t *n);
+
 /* Kernel timing in milliseconds, measured with HIP events on the grid's
  * stream when enabled with echo_set_timing(g, 1).  After echo_steps (per
  * launch): t[0] flux x, t[1] flux y, t[2] flux z, t[3] cell update, means
