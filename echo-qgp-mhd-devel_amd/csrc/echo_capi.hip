@@ -102,7 +102,7 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   // once when there are none); it writes the next step's vnewstep too,
   // before k_end_step flips the parity
   k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, Uout, g->fix, g->VS[0],
-                                                    g->VS[1], last ? 1 : 0, 0, 1);
+                                                    g->VS[1], last ? 1 : 0, ImexView{nullptr, nullptr, 0}, 1);
 }
 
 // one stage of IMEX-SSP3(4,3,3) for the resistive systems (kernels.hpp:
@@ -118,9 +118,11 @@ static void enqueue_imex_stage_t(echo_grid* g, int irk) {
     const unsigned nbu = nblocks_xcd(g->N, UPDATE_BLOCK);
     double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
     double* Ust = g->U[1];
-    auto fixup = [&](double* U, int last, int imex) {
-      k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, U, g->fix, g->VS[0], g->VS[1], last, imex,
-                                                        last);
+    // the repair after the implicit solve of K_phase (phase 0: the step's
+    // result), reading U* through ImexView
+    auto fixup = [&](double* U, int last, int phase) {
+      k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, U, g->fix, g->VS[0], g->VS[1], last,
+                                                        ImexView{g->U[0], g->imex.P4, phase}, last);
     };
     if (g->timing) hipEventRecord(g->ev[0], g->stream);
     if (irk == 1) {
@@ -131,7 +133,7 @@ static void enqueue_imex_stage_t(echo_grid* g, int irk) {
       fixup(Ust, 0, 1);
       k_imex_k2<SYS, ZZ><<<nbu, UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0], Ust, g->imex, g->fix);
       if (g->timing) hipEventRecord(g->ev[5], g->stream);
-      fixup(Ust, 0, 1);
+      fixup(Ust, 0, 2);
       return;
     }
     if (g->active[0]) launch_flux_dir<1, SYS, ZZ>(g, part[0]);
@@ -153,7 +155,7 @@ static void enqueue_imex_stage_t(echo_grid* g, int irk) {
 #undef ECHO_IMEX_UPDATE
     if (g->timing) hipEventRecord(g->ev[5], g->stream);
     if (irk == 4) fixup(g->U[0], 1, 0);
-    else fixup(Ust, 0, 1);
+    else fixup(Ust, 0, irk + 1);
   }
 }
 

@@ -68,12 +68,30 @@ struct GridArgs {
   DevScalars* S;
 };
 
-// IMEX-SSP3(4,3,3) scratch (resistive systems): the explicit start of the
-// fourth stage as it accumulates (P4) and the final combination as it
-// accumulates (ACC), nv components each, in units of a rate (u = u0 - dt *)
+// IMEX-SSP3(4,3,3) scratch (resistive systems), nv components each, in
+// units of a rate (u = u0 - dt * rate).  Only the three E components carry
+// a stiff part, so the other components need one slot between stages:
+//   P4   E: the explicit start of K4 as it accumulates (b ds1, + du2/4 + h ds2);
+//        other components: du2 / 4 after stage 2, then R3 = du2/4 + du3/4
+//        (the rate of K4's start) after stage 3;
+//   ACC  E: the final combination as it accumulates (ds2/6 + du2/6, + du3/6
+//        + ds3/6); the other components take it from P4 (2/3 R3).
 struct ImexBufs {
   double* P4;
   double* ACC;
+};
+
+// What the con2prim repair inside an implicit IMEX stage reads as a
+// neighbour's explicit start U*: the solve kernels store U* only for the
+// three E components (and the repair stores a repaired cell's whole u); the
+// other components follow from u0 and the stored rate with the arithmetic
+// the update used:
+//   phase 1 (K1) U* = u0 (E too), phase 2 (K2) u0,
+//   phase 3 (K3) fma(-dt, 4 P4, u0), phase 4 (K4) fma(-dt, P4, u0).
+struct ImexView {
+  const double* U0;
+  const double* P4;
+  int phase;              // 0: not inside an implicit stage
 };
 
 // 128 threads (two waves) per flux block: with 24 KB of LDS per block, six
@@ -680,19 +698,22 @@ constexpr double IMEX_A = 0.24169426078821, IMEX_B = 0.06042356519705, IMEX_H = 
 constexpr double IMEX_D = 0.5 - IMEX_B - IMEX_H - IMEX_A;
 
 // implicit solve of one cell from its stage start us; marks a failure for
-// k_fixup (which repairs with U = U*, V for the cells before it)
-template <int SYS, bool ZZ>
+// k_fixup (which repairs with U* and, for the cells before it, V).  U* of
+// the E components goes to Ust when STORE_E (the other components, and E in
+// the first stage, the repair recomputes: ImexView).
+template <int SYS, bool ZZ, bool STORE_E>
 __device__ __forceinline__ void imex_solve(const GridArgs& a, const Metric& M, long long c, double* __restrict__ V,
                                            double* __restrict__ Ust, const FixBufs& fb, const double* us,
                                            double* v, double adt) {
   constexpr int NV = SysT<SYS>::NV;
+  constexpr int KE = SysT<SYS>::KE;
   const long long N = a.N;
   __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
   double uu[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     uu[k] = us[k];
-    __builtin_nontemporal_store(us[k], &Ust[(long long)k * N + c]);
+    if (STORE_E && k >= KE && k < KE + 3) __builtin_nontemporal_store(us[k], &Ust[(long long)k * N + c]);
   }
   const int e = res_cons2prim_implicit<SYS, ZZ>(a.eos, M, uu, v, adt, a.sig, a.sch);
   if (e != 0) {
@@ -731,7 +752,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_k1(GridArgs a, double* __
     us[k] = U0[(long long)k * N + c];
     v[k] = V[(long long)k * N + c];
   }
-  imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, us, v, IMEX_A * a.S->dt);
+  imex_solve<SYS, ZZ, false>(a, M, c, V, Ust, fb, us, v, IMEX_A * a.S->dt);
 }
 
 // the stiff rate of K1 (cell-local), P4 = b ds1, and K2 from
@@ -756,13 +777,11 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_k2(GridArgs a, double* __
   double ds[3];
   imex_stiff_rate<SYS>(a, M, v, ds);
 #pragma unroll
-  for (int k = 0; k < NV; k++) {
-    const int ke = k - T::KE;
-    const bool isE = ke >= 0 && ke < 3;
-    ib.P4[(long long)k * N + c] = isE ? IMEX_B * ds[isE ? ke : 0] : 0.;
-    if (isE) us[k] = us[k] + dt * IMEX_A * ds[ke];
+  for (int k = 0; k < 3; k++) {
+    ib.P4[(long long)(T::KE + k) * N + c] = IMEX_B * ds[k];
+    us[T::KE + k] = us[T::KE + k] + dt * IMEX_A * ds[k];
   }
-  imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, us, v, IMEX_A * dt);
+  imex_solve<SYS, ZZ, true>(a, M, c, V, Ust, fb, us, v, IMEX_A * dt);
 }
 
 // stages 2-4 after their sweeps: the rates of stage STAGE at K_STAGE
@@ -821,25 +840,40 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
   double u0[NV], un[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) u0[k] = __builtin_nontemporal_load(&U0[k * N + c]);
+  // u = fma(-dt, rate, u0); the repair recomputes U* of the components other
+  // than E from P4 with the same operation (ImexView)
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     const int ke = k - T::KE;
     const bool isE = ke >= 0 && ke < 3;
-    const double s = isE ? ds[isE ? ke : 0] : 0.;
     const long long o = (long long)k * N + c;
-    if constexpr (STAGE == 2) {
-      ib.P4[o] = ib.P4[o] + (du[k] / 4. + IMEX_H * s);
-      ib.ACC[o] = du[k] / 6. + s / 6.;
-      un[k] = u0[k] - dt * (du[k] + (1. - IMEX_A) * s);
-    } else if constexpr (STAGE == 3) {
-      un[k] = u0[k] - dt * (ib.P4[o] + du[k] / 4. + IMEX_D * s);
-      ib.ACC[o] = ib.ACC[o] + (du[k] / 6. + s / 6.);
+    if (isE) {
+      const double s = ds[ke];
+      if constexpr (STAGE == 2) {
+        ib.P4[o] = ib.P4[o] + (du[k] / 4. + IMEX_H * s);
+        ib.ACC[o] = du[k] / 6. + s / 6.;
+        un[k] = __builtin_fma(-dt, du[k] + (1. - IMEX_A) * s, u0[k]);
+      } else if constexpr (STAGE == 3) {
+        un[k] = __builtin_fma(-dt, ib.P4[o] + du[k] / 4. + IMEX_D * s, u0[k]);
+        ib.ACC[o] = ib.ACC[o] + (du[k] / 6. + s / 6.);
+      } else {
+        un[k] = __builtin_fma(-dt, ib.ACC[o] + du[k] * 2. / 3. + s * 2. / 3., u0[k]);
+      }
     } else {
-      un[k] = u0[k] - dt * (ib.ACC[o] + du[k] * 2. / 3. + s * 2. / 3.);
+      if constexpr (STAGE == 2) {
+        ib.P4[o] = du[k] * 0.25;
+        un[k] = __builtin_fma(-dt, du[k], u0[k]);
+      } else if constexpr (STAGE == 3) {
+        const double r3 = ib.P4[o] + du[k] * 0.25;
+        ib.P4[o] = r3;
+        un[k] = __builtin_fma(-dt, r3, u0[k]);
+      } else {
+        un[k] = __builtin_fma(-dt, ib.P4[o] * (2. / 3.) + du[k] * 2. / 3., u0[k]);
+      }
     }
   }
   if constexpr (STAGE < 4) {
-    imex_solve<SYS, ZZ>(a, M, c, V, Ust, fb, un, v, IMEX_A * dt);
+    imex_solve<SYS, ZZ, true>(a, M, c, V, Ust, fb, un, v, IMEX_A * dt);
   } else {
     {
       const double ch = S->glm_ch;
@@ -898,10 +932,26 @@ __device__ __forceinline__ bool fix_failed(const unsigned* __restrict__ bits, lo
   return (bits[q >> 5] >> (q & 31)) & 1u;
 }
 
+// U* of component k of cell q (ImexView) as the repair of cell c reads it
+template <int SYS>
+__device__ __forceinline__ double imex_ustar(const ImexView& iv, const double* __restrict__ U,
+                                             const unsigned* __restrict__ bits, long long c, long long q, int k,
+                                             long long N, double dt) {
+  constexpr int KE = SysT<SYS>::KE;
+  const long long o = (long long)k * N + q;
+  if (q < c && fix_failed(bits, q)) return U[o];   // repaired before cell c: its whole u is in U
+  const bool isE = k >= KE && k < KE + 3;
+  if (iv.phase == 1) return iv.U0[o];
+  if (isE) return U[o];
+  if (iv.phase == 2) return iv.U0[o];
+  const double r = iv.phase == 3 ? 4.0 * iv.P4[o] : iv.P4[o];
+  return __builtin_fma(-dt, r, iv.U0[o]);
+}
+
 template <int SYS, bool ZZ>
 __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, double* __restrict__ V,
                             double* __restrict__ U, const FixBufs& fb, double* __restrict__ VSnext,
-                            int last_stage, int imex, int smooth_stage) {
+                            int last_stage, const ImexView& iv, int smooth_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
   DevScalars* S = a.S;
@@ -917,10 +967,12 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
   // u of a neighbour as the reference's loop sees it: a smoothed cell's u
   // before its smoothing (fb.Upre), else U (the RK result of later cells,
   // failed ones included, and the repaired u of earlier failed cells).
-  // Inside an implicit IMEX stage U holds the explicit start U* of every
-  // cell, and V the E its implicit solve found: earlier cells are read with
-  // that E (u_E = sqrt(-g) E), later ones (not yet solved in the
-  // reference's loop) with E*.
+  // Inside an implicit IMEX stage a neighbour's u is its explicit start U*
+  // (imex_ustar; the repaired u for earlier failed cells), and V holds the
+  // E its implicit solve found: earlier cells are read with that E (u_E =
+  // sqrt(-g) E), later ones (not yet solved in the reference's loop) with E*.
+  const int imex = iv.phase;
+  const double dt = S->dt;
   const double* us[6];
   const bool smooth = VISC && a.enable_smooth && smooth_stage;
   const double ps = S->smooth_pressure;
@@ -938,6 +990,10 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
 #pragma unroll
     for (int j = 0; j < 6; j++) w[j] = us[j][o + nb[j]];
     if constexpr (SysT<SYS>::RES) {
+      if (imex) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) w[j] = imex_ustar<SYS>(iv, U, fb.failbits, c, nb[j], k, N, dt);
+      }
       constexpr int KE = SysT<SYS>::KE;
       if (imex && k >= KE && k < KE + 3) {
 #pragma unroll
@@ -984,7 +1040,7 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
 template <int SYS, bool ZZ>
 __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restrict__ V, double* __restrict__ U,
                                                      FixBufs fb, double* __restrict__ VS0,
-                                                     double* __restrict__ VS1, int last_stage, int imex,
+                                                     double* __restrict__ VS1, int last_stage, ImexView iv,
                                                      int smooth_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
@@ -1066,7 +1122,7 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
   for (int lev = 0; lev <= sh_maxlev; lev++) {
     for (int i = t; i < n; i += FIX_BLOCK) {
       const long long c = fb.list[i];
-      if (fb.level[c] == lev) repair_cell<SYS, ZZ>(a, M, c, V, U, fb, VSnext, last_stage, imex, smooth_stage);
+      if (fb.level[c] == lev) repair_cell<SYS, ZZ>(a, M, c, V, U, fb, VSnext, last_stage, iv, smooth_stage);
     }
     __syncthreads();
   }
