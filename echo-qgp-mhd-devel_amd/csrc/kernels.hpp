@@ -932,18 +932,16 @@ __device__ __forceinline__ bool fix_failed(const unsigned* __restrict__ bits, lo
   return (bits[q >> 5] >> (q & 31)) & 1u;
 }
 
-// U* of component k of cell q (ImexView) as the repair of cell c reads it
+// U* of component k of cell q (ImexView) as the repair reads it; repaired:
+// q was repaired before the cell being repaired (its whole u is in U)
 template <int SYS>
-__device__ __forceinline__ double imex_ustar(const ImexView& iv, const double* __restrict__ U,
-                                             const unsigned* __restrict__ bits, long long c, long long q, int k,
-                                             long long N, double dt) {
+__device__ __forceinline__ double imex_ustar(const ImexView& iv, const double* __restrict__ U, bool repaired,
+                                             long long q, int k, long long N, double dt) {
   constexpr int KE = SysT<SYS>::KE;
   const long long o = (long long)k * N + q;
-  if (q < c && fix_failed(bits, q)) return U[o];   // repaired before cell c: its whole u is in U
   const bool isE = k >= KE && k < KE + 3;
-  if (iv.phase == 1) return iv.U0[o];
-  if (isE) return U[o];
-  if (iv.phase == 2) return iv.U0[o];
+  if (repaired || (isE && iv.phase > 1)) return U[o];
+  if (iv.phase <= 2) return iv.U0[o];
   const double r = iv.phase == 3 ? 4.0 * iv.P4[o] : iv.P4[o];
   return __builtin_fma(-dt, r, iv.U0[o]);
 }
@@ -973,6 +971,9 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
   // sqrt(-g) E), later ones (not yet solved in the reference's loop) with E*.
   const int imex = iv.phase;
   const double dt = S->dt;
+  bool repaired[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) repaired[j] = nb[j] < c && fix_failed(fb.failbits, nb[j]);
   const double* us[6];
   const bool smooth = VISC && a.enable_smooth && smooth_stage;
   const double ps = S->smooth_pressure;
@@ -992,7 +993,7 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
     if constexpr (SysT<SYS>::RES) {
       if (imex) {
 #pragma unroll
-        for (int j = 0; j < 6; j++) w[j] = imex_ustar<SYS>(iv, U, fb.failbits, c, nb[j], k, N, dt);
+        for (int j = 0; j < 6; j++) w[j] = imex_ustar<SYS>(iv, U, repaired[j], nb[j], k, N, dt);
       }
       constexpr int KE = SysT<SYS>::KE;
       if (imex && k >= KE && k < KE + 3) {
