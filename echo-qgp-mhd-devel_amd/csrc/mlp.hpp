@@ -66,6 +66,18 @@ __device__ __forceinline__ float act_d(float x) {
 
 __device__ __forceinline__ float h16round(float x) { return (float)(_Float16)x; }
 
+// The per-tile LDS exchanges of k_mlp (sX[w], sY[w]) stay inside one wave.
+// LDS operations of a wave complete in the order it issues them, so only the
+// compiler must keep each store before the loads that follow: wave-scope
+// fences, no barrier across the block.  The block's waves then run out of
+// step and hide one another's latencies (the FP64 refinement of one wave
+// beside the loads and MFMA chains of another).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // two activations in half precision (see above)
 __device__ __forceinline__ h16x2 act2h(f32x2 z) {
   const h16x2 x = __builtin_convertvector(z, h16x2);
@@ -295,7 +307,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const
         for (int k = 0; k < 8; k++) xbuf[k * n + s] = (float)xb[k];
       }
     }
-    __syncthreads();
+    wave_sync();
 
     // layer 1 (K = 8, padded to 16)
     h16x4 H[4][4];
@@ -394,14 +406,14 @@ __global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const
           }
         }
       }
-      __syncthreads();
+      wave_sync();
       continue;
     }
 
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (g == 0) sY[w][16 * q + c] = make_float2(y4[q][0], y4[q][1]);
-    __syncthreads();
+    wave_sync();
     const float y1 = sY[w][lane].y;
     double v[8];
     const bool ok = valid && refine(u, y1, g33, ca.iters, ca.tol, v);
@@ -425,7 +437,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_mlp(NetArgs net, C2pArgs ca, const
         ca.fail_list[base + rank] = (int)(off + s);
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
