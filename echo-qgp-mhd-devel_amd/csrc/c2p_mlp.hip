@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 
 #include "grid.hpp"
@@ -36,13 +37,26 @@ namespace {
 
 constexpr int C2P_BLOCK = 256;
 
+// Persistent grid of k_mlp<MODE, NVU>: as many blocks as the device holds at
+// once (the instantiation's occupancy, taken once), fewer for small n.  A
+// larger grid would leave a second, partial round of blocks that each carry
+// a full share of the tiles (1024 blocks against 768 resident at three waves
+// per SIMD: the last 256 start when the first 768 end).
+template <int MODE, int NVU>
 int mlp_grid_blocks(int dev, long long n) {
-  // persistent: enough blocks to fill the chip a few times over
+  static std::atomic<int> per_cu{0};
+  int per = per_cu.load(std::memory_order_relaxed);
+  if (per < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&mlp::k_mlp<MODE, NVU>),
+                                                     mlp::BLOCK, 0) != hipSuccess || per < 1)
+      per = 1;
+    per_cu.store(per, std::memory_order_relaxed);
+  }
   int ncu = 256;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const long long tiles = (n + 63) / 64;
   long long nb = (tiles + 3) / 4;
-  const long long cap = 4LL * ncu;
+  const long long cap = (long long)per * ncu;
   if (nb > cap) nb = cap;
   if (nb < 1) nb = 1;
   return (int)nb;
@@ -106,7 +120,7 @@ mlp::NetArgs net_args(const echo_mlp* m) { return mlp::NetArgs{m->packed, m->bia
 void enqueue_grad(echo_mlp* m, hipStream_t st, const double* du, const double* dv, long long stride,
                   long long off, long long n, double* loss_slot) {
   mlp::C2pArgs ca{};
-  const int nb = mlp_grid_blocks(m->dev, n);
+  const int nb = mlp_grid_blocks<mlp::MODE_TRAIN, 8>(m->dev, n);
   mlp::k_mlp<mlp::MODE_TRAIN, 8><<<nb, mlp::BLOCK, 0, st>>>(net_args(m), ca, du, stride, off, n,
                                                             nullptr, nullptr, m->ybuf, m->xbuf,
                                                             m->zbuf, m->hbuf);
@@ -132,7 +146,8 @@ void enqueue_net_c2p(echo_grid* g, const double* U) {
   const long long N = g->N;
   mlp::k_zero_int<<<1, 64, 0, g->stream>>>(cnt, 1);
   mlp::C2pArgs ca{g->cfg.coordinates, g->net_iters, g->net_tol, g->S, cnt, cnt + 1, 1};
-  const int nb = mlp_grid_blocks(g->dev, N);
+  const int nb = g->cfg.divclean ? mlp_grid_blocks<mlp::MODE_C2P, 9>(g->dev, N)
+                                  : mlp_grid_blocks<mlp::MODE_C2P, 8>(g->dev, N);
   int ncu = 256;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev);
   if (g->cfg.divclean) {
@@ -212,7 +227,7 @@ int echo_mlp_forward(echo_mlp* m, echo_grid* g, const double* d_u, int64_t n, fl
   if (n == 0) return 0;
   HIPCHECK(hipSetDevice(m->dev));
   mlp::C2pArgs ca{};
-  mlp::k_mlp<mlp::MODE_FWD, 8><<<mlp_grid_blocks(m->dev, n), mlp::BLOCK, 0, g->stream>>>(
+  mlp::k_mlp<mlp::MODE_FWD, 8><<<mlp_grid_blocks<mlp::MODE_FWD, 8>(m->dev, n), mlp::BLOCK, 0, g->stream>>>(
       net_args(m), ca, d_u, n, 0, n, nullptr, nullptr, d_y, nullptr, nullptr, nullptr);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(g->stream));
@@ -304,7 +319,8 @@ static int surrogate_run(echo_grid* g, echo_mlp* m, const double* d_u, double* d
                          int* fail_list) {
   mlp::k_zero_int<<<1, 64, 0, g->stream>>>(fail_count, 1);
   mlp::C2pArgs ca{g->cfg.coordinates, iters, tol, g->S, fail_count, fail_list, 0};
-  const int nb = mlp_grid_blocks(m->dev, n);
+  const int nb = g->cfg.divclean ? mlp_grid_blocks<mlp::MODE_C2P, 9>(m->dev, n)
+                                  : mlp_grid_blocks<mlp::MODE_C2P, 8>(m->dev, n);
   if (g->timing) HIPCHECK(hipEventRecord(g->ev[0], g->stream));
   if (g->cfg.divclean)
     mlp::k_mlp<mlp::MODE_C2P, 9><<<nb, mlp::BLOCK, 0, g->stream>>>(net_args(m), ca, d_u, n, 0, n, d_v,
@@ -373,6 +389,9 @@ int echo_set_c2p_net(echo_grid* g, echo_mlp* m, int32_t iters, double tol) {
     g->net_iters = iters;
     g->net_tol = tol;
     g->ga.ext_c2p = 1;
+    // take the network kernel's occupancy now, outside any graph capture
+    if (g->cfg.divclean) mlp_grid_blocks<mlp::MODE_C2P, 9>(g->dev, 1);
+    else mlp_grid_blocks<mlp::MODE_C2P, 8>(g->dev, 1);
   }
   drop_graph(g);
   if (g->graph_dirty) *g->graph_dirty = true;
