@@ -70,6 +70,13 @@ static unsigned flux_blocks(const echo_config& c, int d, int ty) {
 
 static bool tiles_supported(const echo_config& c) { return c.recal == REC_WENO5 || c.recal == REC_PPM4; }
 
+// launch blocks of the cell updates (update_cell), padded for xcd_block
+static unsigned update_blocks(const echo_grid* g) {
+  if (g->ga.upd_tz == 1) return nblocks_xcd(g->N, UPDATE_BLOCK);
+  const long long t = (long long)((g->cfg.nx + UPD_TX - 1) / UPD_TX) * g->cfg.ny * ((g->cfg.nz + UPD_TZ - 1) / UPD_TZ);
+  return (unsigned)((t + NXCD - 1) / NXCD * NXCD);
+}
+
 template <int SYS, bool ZZ>
 static void enqueue_stage_t(echo_grid* g, int irk) {
   const int nrk = g->cfg.nrk;
@@ -93,7 +100,7 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
   const bool last = (irk == nrk);
   double* Ucur = (irk == 1) ? g->U[0] : g->U[1];
   double* Uout = last ? g->U[0] : g->U[1];
-  k_update<SYS, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
+  k_update<SYS, ZZ><<<update_blocks(g), UPDATE_BLOCK, 0, g->stream>>>(
       g->ga, g->V, g->U[0], Ucur, Uout, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2],
       g->VS[0], g->VS[1], g->fix, c1, c2, irk == 1 ? 1 : 0, last ? 1 : 0);
   if (g->net) enqueue_net_c2p(g, Uout);   // ideal MHD with the learned inversion
@@ -146,7 +153,7 @@ static void enqueue_imex_stage_t(echo_grid* g, int irk) {
       hipEventRecord(g->ev[4], g->stream);
     }
 #define ECHO_IMEX_UPDATE(ST)                                                                        \
-  k_update_imex<SYS, ZZ, ST><<<nbu, UPDATE_BLOCK, 0, g->stream>>>(                                \
+  k_update_imex<SYS, ZZ, ST><<<update_blocks(g), UPDATE_BLOCK, 0, g->stream>>>(                    \
       g->ga, g->V, g->U[0], Ust, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2], g->VS[0], \
       g->VS[1], g->imex, g->fix)
     if (irk == 2) ECHO_IMEX_UPDATE(2);
@@ -509,6 +516,7 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   a.sig = c.sigma_el / 0.197326;
   a.sch = c.sigma_chi / 0.197326;
   a.ext_c2p = 0;
+  a.upd_tz = (c.nz >= UPD_TZ && c.nx >= UPD_TX) ? UPD_TZ : 1;
   a.S = g->S;
   if (!share_scratch && hipMemset(g->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(g->N)) != hipSuccess) {
     release(g);

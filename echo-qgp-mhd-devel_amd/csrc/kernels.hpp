@@ -65,6 +65,7 @@ struct GridArgs {
   int res;              // resistive MHD (E evolved): light-speed CFL, glm_ch = 1
   double sig, sch;      // electric and chiral conductivity, fm^-1
   int ext_c2p;          // ideal MHD: k_update leaves con2prim to the learned inversion
+  int upd_tz;           // z rows per block of the cell updates (update_cell): 4 or 1
   DevScalars* S;
 };
 
@@ -111,6 +112,39 @@ constexpr int NXCD = 8;   // MI355X: 8 XCDs, each with its own L2
 __device__ __forceinline__ long long xcd_block(unsigned b, unsigned nb) {
   const unsigned per = nb / NXCD;
   return (long long)(b % NXCD) * per + b / NXCD;
+}
+
+// Cell of a cell-update thread (k_update, k_update_imex).  With upd_tz = 4 a
+// block is a tile of 64 x-consecutive cells by 4 z-consecutive rows (one
+// wave per row), tiles ordered x chunk fastest, then y, then z.  A cell
+// reads the z faces below and above it, and the face between two rows of a
+// tile is read by both rows within the block: the face arrays of the z
+// sweep are fetched about once, not once per side (with the cells in
+// storage order the row above comes nx * ny cells later, long after the
+// face has left the XCD's L2).  With upd_tz = 1 (grids under 4 cells in z
+// or 64 in x) the cells go in storage order, UPDATE_BLOCK per block.
+// Returns false for the padding threads of the launch.
+constexpr int UPD_TX = 64, UPD_TZ = 4;
+__device__ __forceinline__ bool update_cell(const GridArgs& a, int& x, int& y, int& z, long long& c) {
+  const long long b = xcd_block(blockIdx.x, gridDim.x);
+  if (a.upd_tz == 1) {
+    c = b * UPDATE_BLOCK + threadIdx.x;
+    if (c >= a.N) return false;
+    x = (int)(c % a.nx);
+    const long long r = c / a.nx;
+    y = (int)(r % a.ny);
+    z = (int)(r / a.ny);
+    return true;
+  }
+  const int nxc = (a.nx + UPD_TX - 1) / UPD_TX;
+  const long long r = b / nxc;
+  x = (int)(b % nxc) * UPD_TX + (int)(threadIdx.x & (UPD_TX - 1));
+  y = (int)(r % a.ny);
+  const long long zz = (r / a.ny) * UPD_TZ + (threadIdx.x >> 6);
+  if (x >= a.nx || zz >= a.nz) return false;
+  z = (int)zz;
+  c = x + (long long)a.nx * (y + (long long)a.ny * z);
+  return true;
 }
 
 // cell index along a direction of length m for a stencil position c0
@@ -562,14 +596,11 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
     double c2, int first_stage, int last_stage) {
   constexpr bool VISC = SysT<SYS>::VISC;
   constexpr int NV = SysT<SYS>::NV;
-  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
-  if (c >= a.N || a.S->temp_check) return;
+  int x, y, z;
+  long long c;
+  if (!update_cell(a, x, y, z, c) || a.S->temp_check) return;
   DevScalars* S = a.S;
   if (S->run_crashed) return;
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int x = (int)(c % nx);
-  const long long r = c / nx;
-  const int y = (int)(r % ny), z = (int)(r / ny);
   const Metric M = make_metric(a.coordinates, S->t);
   const long long N = a.N;
   double du[NV];
@@ -798,14 +829,11 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
   using T = SysT<SYS>;
   constexpr bool VISC = T::VISC;
   constexpr int NV = T::NV;
-  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
-  if (c >= a.N || a.S->temp_check) return;
+  int x, y, z;
+  long long c;
+  if (!update_cell(a, x, y, z, c) || a.S->temp_check) return;
   DevScalars* S = a.S;
   if (S->run_crashed) return;
-  const int nx = a.nx, ny = a.ny;
-  const int x = (int)(c % nx);
-  const long long r = c / nx;
-  const int y = (int)(r % ny), z = (int)(r / ny);
   const Metric M = make_metric(a.coordinates, S->t);
   const long long N = a.N;
   double du[NV];
