@@ -516,7 +516,10 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   a.sig = c.sigma_el / 0.197326;
   a.sch = c.sigma_chi / 0.197326;
   a.ext_c2p = 0;
-  a.upd_tz = (c.nz >= UPD_TZ && c.nx >= UPD_TX) ? UPD_TZ : 1;
+  // z-row tiles for the MHD systems, whose updates read the most face data
+  // (measured: viscous resistive step 2.1% faster, ideal MHD 1% faster,
+  // viscous hydro 0.6% slower than cells in storage order)
+  a.upd_tz = (c.mhd && c.nz >= UPD_TZ && c.nx >= UPD_TX) ? UPD_TZ : 1;
   a.S = g->S;
   if (!share_scratch && hipMemset(g->fix.failbits, 0, sizeof(unsigned) * (size_t)fix_words(g->N)) != hipSuccess) {
     release(g);

@@ -121,8 +121,9 @@ __device__ __forceinline__ long long xcd_block(unsigned b, unsigned nb) {
 // tile is read by both rows within the block: the face arrays of the z
 // sweep are fetched about once, not once per side (with the cells in
 // storage order the row above comes nx * ny cells later, long after the
-// face has left the XCD's L2).  With upd_tz = 1 (grids under 4 cells in z
-// or 64 in x) the cells go in storage order, UPDATE_BLOCK per block.
+// face has left the XCD's L2).  MHD grids with at least 64 cells in x and
+// 4 in z use the tiles; with upd_tz = 1 the cells go in storage order,
+// UPDATE_BLOCK per block.
 // Returns false for the padding threads of the launch.
 constexpr int UPD_TX = 64, UPD_TZ = 4;
 __device__ __forceinline__ bool update_cell(const GridArgs& a, int& x, int& y, int& z, long long& c) {

@@ -1,10 +1,11 @@
 """GPU parity of the cell updates on tiled blocks (64 x-consecutive cells by 4
-z rows per block, used when a grid has at least 64 cells in x and 4 in z),
-against the oracle on grids small enough for it: viscous hydro with the
-tabulated EoS (RK3), ideal MHD with GLM (RK3; algorithm 1 with the
+z rows per block, used for MHD grids with at least 64 cells in x and 4 in
+z), against the oracle on grids small enough for it: ideal MHD with GLM
+(RK3; algorithm 1 with the
 tabulated EoS, algorithm 5 with p = e/3), resistive MHD (RK3) and viscous
 resistive MHD (IMEX), including grids whose x and z sizes leave partial
-tiles (65 and 100 in x, 5 and 9 in z).  Each cell's update is independent
+tiles (65 in x, 5 in z); viscous hydro at the same widths (storage-order
+blocks) beside them.  Each cell's update is independent
 of how cells are grouped in blocks, so the bar is the trajectory bar of the
 other tests: relative L-infinity <= 1e-10 after the steps.  The states are
 ones the oracle evolves without repairs (a run from inputs perturbed by
@@ -30,7 +31,7 @@ CASES = {
 def test_tiled_update_vs_oracle(oracle, name):
     from echoamd import EchoGrid
     ic, rc, v0 = CASES[name]()
-    assert ic[0] >= 64 and ic[2] >= 4   # the tiled path
+    assert ic[0] >= 64 and ic[2] >= 4   # the tiled path for MHD grids
     print(f"\n{name}: {ic[:3]}", flush=True)
     oracle.setup(ic, rc)
     assert oracle.init_from_prim(v0) == 0
