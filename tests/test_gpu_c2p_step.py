@@ -30,6 +30,7 @@ SCEN = {
     "bjorken-glm": lambda: mhd_fireball3d(),
     "minkowski": lambda: mhd2d(divclean=0),
     "minkowski-glm": lambda: mhd2d(divclean=1),
+    "minkowski-glm-rk2": lambda: mhd2d(divclean=1, nrk=2),
 }
 
 
