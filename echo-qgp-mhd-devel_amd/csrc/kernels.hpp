@@ -126,6 +126,7 @@ __device__ __forceinline__ long long xcd_block(unsigned b, unsigned nb) {
 // UPDATE_BLOCK per block.
 // Returns false for the padding threads of the launch.
 constexpr int UPD_TX = 64, UPD_TZ = 4;
+static_assert(UPDATE_BLOCK == UPD_TX * UPD_TZ && UPD_TX == 64, "a tile row is one wave of 64 lanes");
 __device__ __forceinline__ bool update_cell(const GridArgs& a, int& x, int& y, int& z, long long& c) {
   const long long b = xcd_block(blockIdx.x, gridDim.x);
   if (a.upd_tz == 1) {
