@@ -334,6 +334,69 @@ def ensemble_leg(n_events, steps, eos_rows, seed=2026, resistive=True):
             "tiles_chosen": {str(t): tiles.count(t) for t in map(list, {tuple(t) for t in tiles})}}
 
 
+def mhd_net_ensemble_leg(n_events, steps, eos_rows, seed=2026, iters=3, tol=1e-12):
+    """The ensemble with the learned inversion inside every event's step:
+    n_events Au-Au events sized as in ensemble_leg, ideal MHD with p = e/3
+    (the system the network covers) and the dipole field, RK3, advanced
+    together once with the plain inversion and once with the network
+    attached to every event (guess, `iters` Newton iterations, gate `tol`).
+    Value of each run: total cells x steps / wall time of the timed steps."""
+    from echoamd import C2pNet, Ensemble
+    from echoamd.surrogate import load_weights
+    rng = np.random.default_rng(seed)
+    bs = rng.uniform(0.0, 12.0, n_events)
+    cfgs = []
+    for b in bs:
+        f = 1.0 - b / 12.0
+        nt = int(round((96 + 96 * f) / 16)) * 16
+        ne = int(round((64 + 64 * f) / 16)) * 16
+        cfgs.append(mhd_e3p_config(nt, nt, ne))
+    cells = sum(int(ic[0]) * int(ic[1]) * int(ic[2]) for ic, _ in cfgs)
+    out = {}
+    for name in ("plain", "learned"):
+        t0 = time.perf_counter()
+        ens = Ensemble(cfgs)
+        net = C2pNet(0, load_weights()) if name == "learned" else None
+        for k, ((ic, rc), g) in enumerate(zip(cfgs, ens.grids)):
+            g.init_from_prim(mhd_e3p_state(ic, rc, eos_rows, seed=seed + k, b=bs[k]))
+            if net is not None:
+                g.set_c2p_net(net, iters=iters, tol=tol)
+            if (k + 1) % 32 == 0:
+                print(f"mhd ensemble ({name}): {k + 1} of {n_events} events set up", file=sys.stderr, flush=True)
+        setup_s = time.perf_counter() - t0
+
+        def phase(label, fn):
+            t1 = time.perf_counter()
+            r = fn()
+            print(f"mhd ensemble ({name}): {label} in {time.perf_counter() - t1:.2f} s", file=sys.stderr, flush=True)
+            return r
+        phase("first step (graph capture)", lambda: ens.steps(1))
+        phase("tile tuner", lambda: ens.tune(reps=2))
+        phase("one step (graph capture)", lambda: ens.steps(1))
+        # timed steps in chunks, a progress line each
+        ms, crashed, done = 0.0, 0, 0
+        while done < steps:
+            n = min(10, steps - done)
+            crashed = ens.steps(n)
+            ms += ens.last_ms()
+            done += n
+            print(f"mhd ensemble ({name}): {done} of {steps} steps, {ms:.1f} ms", file=sys.stderr, flush=True)
+        res = {"ms": ms, "zone_updates_per_s": cells * steps / (ms * 1e-3), "crashed": crashed,
+               "setup_s": setup_s}
+        if net is not None:
+            nfb = sum(g.c2p_fallbacks() for g in ens.grids)
+            res["fallbacks"] = nfb
+            res["fallback_fraction"] = nfb / (3.0 * cells * (steps + 2))
+        ens.close()
+        if net is not None:
+            net.close()
+        out[name] = res
+    return {"events": n_events, "system": "ideal MHD p=e/3 (GLM, algorithm 5), RK3", "cells": cells,
+            "steps": steps, "iters": iters, "gate_tol": tol, "plain": out["plain"], "learned": out["learned"],
+            "learned_vs_plain": out["plain"]["ms"] / out["learned"]["ms"],
+            "grid_sizes": sorted({f"{int(ic[0])}x{int(ic[1])}x{int(ic[2])}" for ic, _ in cfgs})}
+
+
 def time_steps(g, steps, warmup):
     g.steps(warmup)
     t0 = time.perf_counter()
@@ -360,6 +423,39 @@ def side_leg(nx, ny, nz, eos_rows, steps, **kw):
             "crashed": crashed}
 
 
+def mhd_e3p_config(nx, ny, nz, maxdt=0.04):
+    """Ideal MHD with p = e/3 (GLM, algorithm 5: the system the learned
+    inversion covers), RK3, WENO5 + HLL, on the workload's cell sizes.
+    MAXDT 0.04 fm keeps 50 steps below tau = 2.4 fm: these inviscid events
+    with the dipole field develop superluminal states at the fireball edge
+    from tau ~ 2.9 fm on (hundreds, then thousands of con2prim repairs per
+    step), which would measure the repair instead of the inversion."""
+    from libs import make_config
+    dx, deta = 0.15, 0.1
+    return make_config(nx=nx, ny=ny, nz=nz, coordinates=2, mhd=1, divclean=1, eqstate=1, rec="WENO5",
+                       nrk=3, bc=2, algo_solver=5, xlim=(-nx * dx / 2, nx * dx / 2),
+                       ylim=(-ny * dx / 2, ny * dx / 2), zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4,
+                       cfl=0.4, maxdt=maxdt)
+
+
+def mhd_e3p_state(ic, rc, eos_rows, seed=12345, b=7.0):
+    """The workload's initial energy density (Glauber event, eta profile)
+    with p = e/3, no viscous stresses, and its dipole field (nv 9)."""
+    nx, ny, nz = int(ic[0]), int(ic[1]), int(ic[2])
+    cfg_v, _ = workload_config(nx, ny, nz)
+    v19, _ = workload_state(cfg_v, rc, eos_rows, seed=seed, b=b, ns_shear=False)
+    # the same energy density with p = e/3
+    hc3 = .00768351
+    T = eos_rows[:, 0]
+    e = np.interp(v19[4], eos_rows[:, 2] / hc3 * T ** 4, eos_rows[:, 1] / hc3 * T ** 4)
+    v0 = np.zeros((9, nz, ny, nx))
+    v0[0] = 1.0
+    v0[4] = e / 3.0
+    v0[5] = v19[12]
+    v0[6] = v19[13]
+    return v0
+
+
 def learned_step_leg(steps, eos_rows, nx=192, ny=192, nz=128, iters=3, tol=1e-12):
     """The learned inversion inside the step.  Ideal MHD with p = e/3 (the
     system the network covers: GLM, algorithm 5), RK3, WENO5 + HLL, on the
@@ -370,7 +466,6 @@ def learned_step_leg(steps, eos_rows, nx=192, ny=192, nz=128, iters=3, tol=1e-12
     (HIP events; with the network it includes the network, refinement and
     fallback kernels), the fraction of inversions left to the plain solver,
     and the largest relative difference of the two final states."""
-    from libs import make_config
     from echoamd import C2pNet, EchoGrid
     from echoamd.surrogate import load_weights
 
@@ -385,23 +480,8 @@ def learned_step_leg(steps, eos_rows, nx=192, ny=192, nz=128, iters=3, tol=1e-12
                 d = float(np.abs(a[k] - b[k]).max())
                 out = max(out, d / scale if scale > 0 else d)
         return out
-    dx, deta = 0.15, 0.1
-    ic, rc = make_config(nx=nx, ny=ny, nz=nz, coordinates=2, mhd=1, divclean=1, eqstate=1, rec="WENO5",
-                         nrk=3, bc=2, algo_solver=5, xlim=(-nx * dx / 2, nx * dx / 2),
-                         ylim=(-ny * dx / 2, ny * dx / 2), zlim=(-nz * deta / 2, nz * deta / 2), tmin=0.4,
-                         cfl=0.4, maxdt=0.1)
-    cfg_v, _ = workload_config(nx, ny, nz)
-    v19, _ = workload_state(cfg_v, rc, eos_rows, ns_shear=False)
-    # the same energy density with p = e/3
-    hc3 = .00768351
-    T = eos_rows[:, 0]
-    e = np.interp(v19[4], eos_rows[:, 2] / hc3 * T ** 4, eos_rows[:, 1] / hc3 * T ** 4)
-    v0 = np.zeros((9, nz, ny, nx))
-    v0[0] = 1.0
-    v0[4] = e / 3.0
-    v0[5] = v19[12]
-    v0[6] = v19[13]
-    del v19
+    ic, rc = mhd_e3p_config(nx, ny, nz)
+    v0 = mhd_e3p_state(ic, rc, eos_rows)
     N = nx * ny * nz
     net = C2pNet(0, load_weights())
     res = {}
@@ -502,6 +582,9 @@ def main():
                     help="states of the learned-con2prim leg (0: skip)")
     ap.add_argument("--events", type=int, default=16, help="events of the ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--mhd-net-events", type=int, default=0,
+                    help="events of the ideal-MHD ensemble run plain and with the learned inversion in every "
+                         "event's step (0: skip); steps: --ensemble-steps")
     ap.add_argument("--learned-steps", type=int, default=20,
                     help="timed steps of the learned-inversion-in-the-step leg (ideal MHD p = e/3; 0: skip)")
     ap.add_argument("--large-steps", type=int, default=0,
@@ -588,6 +671,8 @@ def main():
     if args.events > 0:
         out["ensemble"] = ensemble_leg(args.events, args.ensemble_steps, eos_rows,
                                        resistive=not args.ensemble_viscous)
+    if args.mhd_net_events > 0:
+        out["mhd_net_ensemble"] = mhd_net_ensemble_leg(args.mhd_net_events, args.ensemble_steps, eos_rows)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, rc, eos_rows)
     print(json.dumps(out), flush=True)
