@@ -237,7 +237,11 @@ static void enqueue_check(echo_grid* g) {
   k_check_decide<<<1, 1024, 0, g->stream>>>(g->S, g->partial, (int)nb);
 }
 
-static int dump_kend(const echo_grid* g) { return g->cfg.mhd ? 8 : 5; }   // common.f90:379-386
+// krh..kvold_end of vold / uold (common.f90:379-386): up to B^eta with MHD
+// (after the viscous variables on the viscous resistive grid), else up to p
+static int dump_kend(const echo_grid* g) {
+  return g->cfg.mhd ? (g->cfg.mhd == 2 && g->cfg.viscosity ? 15 : 8) : 5;
+}
 
 // vnewstep / unewstep of the restart dump, at the start of every step
 // (all three ways of stepping: captured steps, timed steps, single stages)
@@ -622,9 +626,6 @@ static int glauber_primitives(echo_grid* g, const echo_glauber_params* p, const 
   const int ns = n_a + n_b + n_coll;
   if (ns < 1 || ns > GLAUBER_MAX_SRC) return fail(ECHO_E_ARG, "1..%d sources supported, got %d", GLAUBER_MAX_SRC, ns);
   if (!(p->sig_mc > 0.) || !(p->ybeam > 0.)) return fail(ECHO_E_ARG, "need sig_mc > 0 and ybeam > 0");
-  if (g->cfg.mhd == 2)
-    return fail(ECHO_E_ARG, "the Glauber initialisation is built for hydro, viscous hydro and ideal MHD "
-                            "(resistive grids start from echo_init_from_prim)");
   HIPCHECK(hipSetDevice(g->dev));
   // centre of mass of the sources, in the reference's summation order
   // (glaubermc.f90:806-821)
@@ -686,7 +687,6 @@ int echo_init_glauber_mhd(echo_grid* g, const echo_glauber_params* p, const doub
                           const echo_bfield_params* bp, const echo_nucleons* lists) {
   if (!g || !p || !bp || !lists) return fail(ECHO_E_ARG, "null argument");
   if (!g->cfg.mhd) return fail(ECHO_E_ARG, "the grid has no magnetic field (mhd = 0)");
-  if (g->cfg.mhd == 2) return fail(ECHO_E_ARG, "the Glauber initialisation is built for ideal MHD (mhd = 1)");
   if (!(bp->gamma_col > 1.)) return fail(ECHO_E_ARG, "need gamma_col > 1");
   if (bp->magfield_type < 1 || bp->magfield_type > 3) return fail(ECHO_E_ARG, "magfield_type must be 1, 2 or 3");
   // each list up to its first neutron, as glaubermc.f90:1026-1058 reads it
@@ -724,7 +724,9 @@ int echo_init_glauber_mhd(echo_grid* g, const echo_glauber_params* p, const doub
     const double dx = (c.x2 - c.x1) / c.nx, dy = (c.y2 - c.y1) / c.ny, dz = (c.z2 - c.z1) / c.nz;
     dispatch_sys(c, [&](auto sys, auto) {
       constexpr int SYS = decltype(sys)::value;
-      if constexpr (SysT<SYS>::MHD)
+      // resistive grids: B at the layout's slots, E and psi start at zero
+      // (init.f90:2732-2832 fills kbx..kbz only)
+      if constexpr (SysT<SYS>::MHD || SysT<SYS>::RES)
         k_glauber_bfield<SYS><<<nblocks(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(
             g->ga, b, c.x1, dx, c.y1, dy, c.z1, dz, dxy, g->V);
     });
@@ -939,8 +941,6 @@ int echo_set_stop(echo_grid* g, double temp_end, int32_t freeze_type) {
 int echo_set_dump(echo_grid* g, int32_t on) {
   if (!g) return fail(ECHO_E_ARG, "null argument");
   if (g->next_stage != 1) return fail(ECHO_E_ARG, "echo_set_dump inside a step");
-  if (on && g->cfg.mhd == 2)
-    return fail(ECHO_E_ARG, "the restart dump is built for hydro, viscous hydro and ideal MHD");
   HIPCHECK(hipSetDevice(g->dev));
   if (on && !g->dump_buf[0]) {
     const size_t sz = sizeof(double) * 2 * dump_kend(g) * (size_t)g->N;
@@ -1275,7 +1275,6 @@ int echo_write_frame(echo_grid* g, const int32_t* sel, int32_t precision, int32_
   if (precision != 4 && precision != 8) return fail(ECHO_E_ARG, "precision must be 4 or 8");
   if (iout < 0 || iout > 9999) return fail(ECHO_E_ARG, "iout must be 0..9999");
   if (sel[27]) return fail(ECHO_E_ARG, "the comoving density (rc) is not written");
-  if (g->cfg.mhd == 2) return fail(ECHO_E_ARG, "output frames are built for hydro, viscous hydro and ideal MHD");
   const echo_config& c = g->cfg;
   const bool visc = c.viscosity != 0, mhd = c.mhd != 0;
   const bool anypit = sel[9] || sel[10] || sel[11] || sel[12];
@@ -1298,11 +1297,15 @@ int echo_write_frame(echo_grid* g, const int32_t* sel, int32_t precision, int32_
   }
   if (sel[19]) order.push_back({1, 8});
   if (mhd) {
-    if (sel[20]) order.push_back({0, KBX});
-    if (sel[21]) order.push_back({0, KBY});
-    if (sel[22]) order.push_back({0, KBZ});
-    for (int q = 0; q < 3; q++) if (sel[23 + q]) order.push_back({1, 9 + q});
-    if (c.divclean && sel[26]) order.push_back({0, KGLM});
+    // field slots of the layout (system.f90:92-124): resistive systems hold
+    // B, E and psi after the fluid variables, and their E is written as
+    // evolved (out.f90:404-413); ideal MHD derives E = -v x B (414-429)
+    const int nf = visc ? 12 : 5;
+    const int kb = c.mhd == 2 ? nf : KBX, kg = c.mhd == 2 ? nf + 6 : KGLM;
+    for (int q = 0; q < 3; q++) if (sel[20 + q]) order.push_back({0, kb + q});
+    for (int q = 0; q < 3; q++)
+      if (sel[23 + q]) order.push_back(c.mhd == 2 ? std::make_pair(0, nf + 3 + q) : std::make_pair(1, 9 + q));
+    if (c.divclean && sel[26]) order.push_back({0, kg});
   }
   const long long N = g->N;
   const size_t need = 1 + order.size() * (size_t)N;

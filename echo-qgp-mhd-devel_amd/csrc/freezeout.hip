@@ -42,7 +42,7 @@ constexpr int FO_TILE = 32;
 
 template <int SYS>
 struct FoWidth {
-  static constexpr int W = SysT<SYS>::MHD ? 16 : (SysT<SYS>::VISC ? 25 : 13);
+  static constexpr int W = (SysT<SYS>::MHD || SysT<SYS>::RES) ? 16 : (SysT<SYS>::VISC ? 25 : 13);
 };
 
 struct FoArgs {
@@ -205,10 +205,13 @@ __device__ void fo_element(const GridArgs& a, const FoArgs& f, int d, int ix, in
   r[10] = si * ((double)(jx - ix) * dt * f.dy * f.dz * tau);
   r[11] = si * ((double)(jy - iy) * f.dx * dt * f.dz * tau);
   r[12] = si * ((double)(jz - iz) * f.dx * f.dy * dt * tau);
-  if constexpr (SysT<SYS>::MHD) {
-    r[13] = interp(KBX);
-    r[14] = interp(KBY);
-    r[15] = interp(KBZ);
+  if constexpr (SysT<SYS>::MHD || SysT<SYS>::RES) {
+    // find_hypersurface_mhd on every MHD grid, resistive and viscous
+    // resistive included (hypersurface.f90:152-166): B of the layout
+    constexpr int KB = SysT<SYS>::KB;
+    r[13] = interp(KB);
+    r[14] = interp(KB + 1);
+    r[15] = interp(KB + 2);
   } else if constexpr (SysT<SYS>::VISC) {
     double p[12];
 #pragma unroll
@@ -330,8 +333,6 @@ int echo_fo_width(const echo_grid* g) {
 
 int echo_fo_setup(echo_grid* g, int32_t freeze_type, double freeze_value, double* hdr) {
   if (!g) return fail(ECHO_E_ARG, "echo_fo_setup: null grid");
-  if (g->cfg.mhd == 2)
-    return fail(ECHO_E_ARG, "echo_fo_setup: the hypersurface is built for hydro, viscous hydro and ideal MHD");
   if (freeze_type != 0 && freeze_type != 1)
     return fail(ECHO_E_ARG, "freeze_type must be 0 (temperature) or 1 (energy density), got %d", freeze_type);
   if (!isfinite(freeze_value)) return fail(ECHO_E_ARG, "freeze_value must be finite");

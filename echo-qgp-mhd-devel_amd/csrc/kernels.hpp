@@ -1519,16 +1519,17 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_glauber_bfield(GridArgs a, BFi
   } else {
     tt = b.tmin * cosh(z); zcoord = b.tmin * sinh(z); tom = cosh(z); tomz = b.tmin;
   }
-  double bx = V[KBX * N + c], by = V[KBY * N + c], bz = V[KBZ * N + c];
+  constexpr int KB = SysT<SYS>::KB;   // B^x of the layout
+  double bx = V[KB * N + c], by = V[(KB + 1) * N + c], bz = V[(KB + 2) * N + c];
   int i = 0;
   for (int l = 0; l < 4; l++) {
     const double zpos = l < 2 ? tt * b.vel + zcoord : tt * b.vel - zcoord;
     const int end = i + b.n[l];
     for (; i < end; i++) proton_field(l < 2, b.w[l], x - s[i].x, y - s[i].y, zpos, b, tom, tomz, bx, by, bz);
   }
-  V[KBX * N + c] = bx;
-  V[KBY * N + c] = by;
-  V[KBZ * N + c] = bz;
+  V[KB * N + c] = bx;
+  V[(KB + 1) * N + c] = by;
+  V[(KB + 2) * N + c] = bz;
 }
 
 // batched con2prim over n independent states (system_cons2prim)

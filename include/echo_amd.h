@@ -163,12 +163,13 @@ typedef struct echo_glauber_params {
 int echo_init_glauber(echo_grid *g, const echo_glauber_params *p, const double *part_a, int32_t n_a,
                       const double *part_b, int32_t n_b, const double *coll, int32_t n_coll);
 
-/* The same for an ideal-MHD grid, with the initial magnetic field of
+/* The same for an MHD grid (ideal or resistive), with the initial magnetic field of
  * generate_B_field_profile (glaubermc.f90:877-1174, called from init.f90:
  * 2796-2811 when no B-field file is given; one event, avg_events = 0): the field of the protons among
  * the spectators and participants of nuclei A and B (participants weighted
  * by pw) in a medium of the grid's sigma_el and sigma_chi, added to B^x,
- * B^y, B^eta before prim2cons.  lists[4]: spectators A, participants A,
+ * B^y, B^eta before prim2cons (on resistive grids E and psi start at zero,
+ * as init.f90:2732-2832 fills only kbx..kbz).  lists[4]: spectators A, participants A,
  * spectators B, participants B; each is read up to its first neutron, as
  * the reference does (its files list protons first); at most 2048 protons. */
 typedef struct echo_bfield_params {
@@ -237,6 +238,8 @@ int echo_set_tout(echo_grid *g, double tout);
  * is made; the device-to-host copy and the file write then run in the
  * background while the grid keeps stepping.  echo_wait_frames waits for the
  * frame in flight (a new echo_write_frame waits for the previous one).
+ * On resistive grids ex, ey, ez are the evolved E as held (out.f90:404-413),
+ * on ideal-MHD grids -v x B (414-429).
  * rc is not supported; pizz needs TRACE_IMP 'zz' and a pi^{t mu} field. */
 int echo_write_frame(echo_grid *g, const int32_t *sel, int32_t precision, int32_t iout,
                      const char *dir);
@@ -255,7 +258,9 @@ int echo_get_stop(echo_grid *g, int32_t *stopped, double *max_value);
 /* Restart dump: out_dump / out_restart (out.f90:1313-1483) in the
  * reference's dump.dat format (a Fortran stream file: 4-byte iout, t,
  * timeold, timeinterval, v, u, vold, uold -- components krh..kvold_end,
- * 5 for hydro, 8 for MHD --, g_cov_3_old, and with echo_fo_setup active:
+ * up to p for hydro (5), up to B^eta for MHD (8; 15 on the viscous
+ * resistive grid, whose B follows the viscous variables) --, g_cov_3_old,
+ * and with echo_fo_setup active:
  * thyp, time_arr, a primitive slice, field(2,:,:,:)).
  * echo_set_dump(g, 1) keeps vold / uold on the device (one copy of
  * 2 * kvold_end fields per step, at the step's start, as echo.f90:192-201);

@@ -114,7 +114,7 @@ int orc_setup(const int *ic, const double *rc, const char *table_path) {
   G.du_stiff = G.dus_all;
   G.deriv = calloc(n * NDERIV, sizeof(double));
   /* common.f90:379-388: vold, uold, vnewstep, unewstep hold krh..kvold_end */
-  G.kvold_end = G.mhd ? 8 : 5;
+  G.kvold_end = G.mhd ? G.kbz + 1 : 5;   /* up to B^eta, after any viscous variables */
   G.vnewstep = calloc(n * G.kvold_end, sizeof(double));
   G.vold = calloc(n * G.kvold_end, sizeof(double));
   G.unewstep = calloc(n * G.kvold_end, sizeof(double));

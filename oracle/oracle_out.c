@@ -8,7 +8,8 @@
  * energy density, temperature, entropy density from get_derived_data;
  * pi^{t mu} (and pi^{eta eta} with TRACE_IMP 'zz') from get_derived_pi(_zz);
  * v0 = gamma; the ideal-MHD electric field of system_compute_E_in_ideal_MHD
- * (system.f90:1885-1897) with E_z made contravariant.  The reference prints
+ * (system.f90:1885-1897) with E_z made contravariant, and on resistive grids
+ * the evolved E as held (out.f90:404-413).  The reference prints
  * pi^{eta eta} from the 'zz' work array, filled only when a pi^{t mu} field
  * is selected too; the same selection is required here.  The comoving
  * density (rc), the derivatives and the flows are not written.
@@ -97,7 +98,12 @@ int orc_out_frame(const int *sel, int prec, int iout, const char *path, const ch
     if (sel[O_BX]) err |= put(f, G.v + (size_t)G.kbx * n, n, prec);
     if (sel[O_BY]) err |= put(f, G.v + (size_t)G.kby * n, n, prec);
     if (sel[O_BZ]) err |= put(f, G.v + (size_t)G.kbz * n, n, prec);
-    if (sel[O_EX] || sel[O_EY] || sel[O_EZ]) {
+    if (G.rmhd) {
+      /* out.f90:404-413: the evolved E as it is held */
+      const int ke[3] = {G.kex, G.key, G.kez};
+      for (int q = 0; q < 3; q++)
+        if (sel[O_EX + q]) err |= put(f, G.v + (size_t)ke[q] * n, n, prec);
+    } else if (sel[O_EX] || sel[O_EY] || sel[O_EZ]) {
       for (size_t c = 0; c < n; c++) {
         double V3[3] = {G.v[KVX * n + c], G.v[KVY * n + c], G.v[KVZ * n + c]};
         double B3[3] = {G.v[G.kbx * n + c], G.v[G.kby * n + c], G.v[G.kbz * n + c]};

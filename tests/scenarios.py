@@ -330,6 +330,17 @@ def rmhd_fireball(nx=16, ny=14, nz=10, viscosity=1, eqstate=1, nrk=4, B0=0.3, E0
     return ic, rc, v
 
 
+def reference_resistive_config(ic):
+    """The reference's set-up of a resistive grid for its writers: it has no
+    inversion for the resistive system (no algorithm 7, system.f90:1160-1162;
+    its set-up stops on ALG_SOLV 7), so it is set up with algorithm 5, which
+    out_vars, out_dump and the hypersurface finder never call, and is then
+    given the oracle's states."""
+    ic = ic.copy()
+    ic[12] = 5
+    return ic
+
+
 def uniform_resistive(E0=(0.3, -0.2, 0.1), B0=(0.0, 0.0, 0.0), p=1.0, sigma_el=0.0058, nrk=4, n=4,
                       maxdt=0.01, coordinates=1, viscosity=0):
     """A uniform fluid at rest with uniform E and B on a small periodic grid:

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from libs import OracleFO, oracle_dump, oracle_restart
-from scenarios import fireball3d, mhd3d, rel_linf
+from scenarios import fireball3d, mhd3d, rel_linf, rmhd_fireball
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -40,7 +40,10 @@ def _grid(ic, rc):
 
 
 @pytest.mark.parametrize("make,kend", [(lambda: fireball3d(24, 20, 16, viscosity=1, eqstate=3), 5),
-                                       (lambda: mhd3d(16, 12, 10, algo=1), 8)], ids=["visc-tab", "mhd-tab"])
+                                       (lambda: mhd3d(16, 12, 10, algo=1), 8),
+                                       (lambda: rmhd_fireball(16, 12, 10, viscosity=0, nrk=3), 8),
+                                       (lambda: rmhd_fireball(16, 12, 10, viscosity=1), 15)],
+                         ids=["visc-tab", "mhd-tab", "rmhd-rk3", "vrmhd-imex"])
 def test_dump_matches_oracle(oracle, tmp_path, make, kend):
     print("\ndump vs oracle", flush=True)
     ic, rc, v0 = make()
@@ -69,9 +72,11 @@ def test_dump_matches_oracle(oracle, tmp_path, make, kend):
     g.close()
 
 
-def test_restart_continues_bit_for_bit(tmp_path):
+@pytest.mark.parametrize("make", [lambda: fireball3d(24, 20, 16, viscosity=1, eqstate=3),
+                                  lambda: rmhd_fireball(16, 12, 10, viscosity=1)], ids=["visc-tab", "vrmhd-imex"])
+def test_restart_continues_bit_for_bit(tmp_path, make):
     print("\nrestart round trip", flush=True)
-    ic, rc, v0 = fireball3d(24, 20, 16, viscosity=1, eqstate=3)
+    ic, rc, v0 = make()
     g = _grid(ic, rc)
     g.init_from_prim(v0)
     g.set_dump(True)

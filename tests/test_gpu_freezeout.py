@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from libs import OracleFO
-from scenarios import fireball3d, mhd3d
+from scenarios import fireball3d, mhd3d, rmhd_fireball
 from test_oracle_freezeout import SCENARIOS
 
 pytestmark = pytest.mark.gpu
@@ -43,7 +43,14 @@ def _rows(path):
     return np.array(out)
 
 
-@pytest.mark.parametrize("name,make,ftype,fval", SCENARIOS, ids=[s[0] for s in SCENARIOS])
+# resistive grids (find_hypersurface_mhd with B at the resistive slots)
+RESISTIVE = [
+    ("rmhd-rk3-T", lambda: rmhd_fireball(12, 10, 8, viscosity=0, nrk=3), 0, 0.2),
+    ("vrmhd-imex-e", lambda: rmhd_fireball(12, 10, 8, viscosity=1), 1, 1.5),
+]
+
+
+@pytest.mark.parametrize("name,make,ftype,fval", SCENARIOS + RESISTIVE, ids=[s[0] for s in SCENARIOS + RESISTIVE])
 def test_gpu_elements_match_oracle(oracle, tmp_path, name, make, ftype, fval):
     from echoamd import EchoGrid
     print(f"\nfreeze-out {name}", flush=True)

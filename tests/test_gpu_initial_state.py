@@ -158,3 +158,39 @@ def test_glauber_from_generated_event(oracle, tmp_path):
     vo, uo = oracle.state()
     assert rel_linf(vg, vo) <= 1e-10 and rel_linf(ug, uo) <= 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("viscosity", [0, 1])
+def test_glauber_resistive_initial_state(oracle, viscosity):
+    """echo_init_glauber_mhd on a resistive grid (B at the resistive slots,
+    E and psi zero), IMEX-SSP3(4,3,3): against the oracle, then three
+    steps."""
+    from echoamd import EchoGrid
+    rng = np.random.default_rng(12)
+    ic, rc = make_config(nx=32, ny=28, nz=12, coordinates=2, mhd=2, divclean=1, algo_solver=7, nrk=4,
+                         viscosity=viscosity, eqstate=1, xlim=(-6.0, 6.0), ylim=(-5.25, 5.25),
+                         zlim=(-3.0, 3.0), tmin=0.6, init_type=3, maxdt=0.05, eta_over_s=0.1,
+                         sigma_el=0.0058, sigma_chi=0.0015)
+    pa = rng.normal(0, 2.0, (100, 2)) + [1.0, 0.0]
+    pb = rng.normal(0, 2.0, (90, 2)) - [1.0, 0.0]
+    pc = rng.normal(0, 1.6, (250, 2))
+    nuc = _nucleons(rng)
+    bp = dict(gamma_col=106.6, b_amplify=0.1, pw=0.5, magfield_type=1)
+    print(f"\nglauber resistive: viscosity={viscosity}", flush=True)
+    oracle.setup(ic, rc)
+    e, v0 = oracle_glauber_prim(oracle, PAR, pa, pb, pc)
+    assert e == 0
+    assert oracle_glauber_bfield(oracle, bp, nuc, v0) == 0
+    assert oracle.init_from_prim(v0) == 0
+    vo, uo = oracle.state()
+    g = EchoGrid(ic, rc)
+    assert g.init_glauber_mhd(PAR, pa, pb, pc, bp, nuc) == 0
+    vg, ug = g.state()
+    assert rel_linf(vg, vo) <= 1e-12
+    assert rel_linf(ug, uo) <= 1e-12
+    assert oracle.steps(3) == 0
+    g.steps(3)
+    vg, ug = g.state()
+    vo, uo = oracle.state()
+    assert rel_linf(vg, vo) <= 1e-10 and rel_linf(ug, uo) <= 1e-10
+    g.close()

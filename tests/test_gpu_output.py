@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from scenarios import fireball3d, mhd3d
+from scenarios import fireball3d, mhd3d, rmhd_fireball
 from test_oracle_output import oracle_frame
 
 pytestmark = pytest.mark.gpu
@@ -26,7 +26,8 @@ def _fields(ic):
 
 def _rel(a, b):
     """max |a - b| relative to max |b| of one field"""
-    return np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-300)
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
 def _read(path, n, prec):
@@ -37,7 +38,10 @@ def _read(path, n, prec):
 
 @pytest.mark.parametrize("prec", [8, 4])
 @pytest.mark.parametrize("make", [lambda: fireball3d(16, 12, 10, viscosity=1, eqstate=3),
-                                  lambda: mhd3d(16, 12, 10, algo=1)], ids=["visc-tab", "mhd-glm"])
+                                  lambda: mhd3d(16, 12, 10, algo=1),
+                                  lambda: rmhd_fireball(16, 12, 10, viscosity=0, nrk=3),
+                                  lambda: rmhd_fireball(16, 12, 10, viscosity=1)],
+                         ids=["visc-tab", "mhd-glm", "rmhd-rk3", "vrmhd-imex"])
 def test_gpu_frame(oracle, tmp_path, make, prec):
     from echoamd import OUT_FIELDS, EchoGrid
     ic, rc, v0 = make()

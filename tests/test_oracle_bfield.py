@@ -140,3 +140,49 @@ def test_bjorken_plane_eta_zero_is_the_lab_field(oracle):
     Bm = _field(oracle, L, coordinates=2, **kw)
     Bl = _field(oracle, L, coordinates=1, **kw)
     assert np.allclose(Bm[:2], Bl[:2], rtol=1e-14, atol=0)
+
+
+def test_resistive_glauber_state_has_the_ideal_fields():
+    """The Glauber state of a resistive grid (init.f90:2732-2832 fills
+    kbx..kbz; the layout of system.f90:92-124 puts B, E, psi after the fluid
+    variables): the fluid variables and B equal those of the ideal-MHD (or,
+    with viscosity, the viscous hydro) grid of the same event, E and psi are
+    zero, and the state initialises (prim2cons, init.f90:2937-2958)."""
+    from libs import Oracle, oracle_glauber_prim
+    from scenarios import resistive_layout
+    rng = np.random.default_rng(3)
+    pa = rng.normal(0, 2.0, (40, 2)) + [1.0, 0.0]
+    pb = rng.normal(0, 2.0, (36, 2)) - [1.0, 0.0]
+    pc = rng.normal(0, 1.6, (90, 2))
+    lists = [(rng.normal(0, 2.2, (20, 2)) + [s, 0.0], np.r_[np.zeros(9, np.int32), np.ones(11, np.int32)])
+             for s in (4.0, 1.0, -4.0, -1.0)]
+    par = dict(kappa=19.0, ah=0.15, sig_mc=0.4, ybeam=5.36, deta=3.0, sigeta=0.8, rhocenter=1.0)
+    bp = dict(gamma_col=106.6, b_amplify=0.1, pw=0.5, magfield_type=1)
+    box = dict(nx=16, ny=14, nz=6, coordinates=2, eqstate=1, xlim=(-6.0, 6.0), ylim=(-5.25, 5.25),
+               zlim=(-3.0, 3.0), tmin=0.6, init_type=3, eta_over_s=0.1, sigma_el=0.0058, sigma_chi=0.0015)
+    o = Oracle()
+
+    def state(**kw):
+        ic, rc = make_config(**box, **kw)
+        o.setup(ic, rc)
+        e, v = oracle_glauber_prim(o, par, pa, pb, pc)
+        assert e == 0
+        if kw.get("mhd"):
+            assert oracle_glauber_bfield(o, bp, lists, v) == 0
+        return v, o.init_from_prim(v)
+
+    v_ideal, e = state(mhd=1, divclean=1, algo_solver=5)
+    assert e == 0
+    v_visc, e = state(viscosity=1)
+    assert e == 0
+    for viscosity in (0, 1):
+        v, e = state(mhd=2, divclean=1, algo_solver=7, nrk=4, viscosity=viscosity)
+        assert e == 0
+        kb, ke, kg, nv = resistive_layout(viscosity)
+        assert v.shape[0] == nv
+        assert np.array_equal(v[:5], v_ideal[:5])
+        assert np.array_equal(v[kb:kb + 3], v_ideal[5:8])
+        assert np.abs(v[kb:kb + 3]).max() > 0.0
+        assert not v[ke:].any()
+        if viscosity:
+            assert np.array_equal(v[:12], v_visc)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from libs import OracleFO, RefFO
-from scenarios import fireball3d, mhd3d
+from scenarios import fireball3d, mhd3d, reference_resistive_config, rmhd_fireball
 
 SCENARIOS = [
     ("visc-tab-T", lambda: fireball3d(12, 10, 8, viscosity=1, eqstate=3), 0, 0.2),
@@ -70,6 +70,50 @@ def test_elements_and_file_match_reference(oracle, ref, tmp_path, name, make, ft
     b = open(opath, "rb").read().split(b"\n", 1)
     assert a[1] == b[1]
     _same_header(a[0], b[0], ftype)
+
+
+@pytest.mark.parametrize("ftype,fval", [(0, 0.2), (1, 1.5)])
+def test_resistive_elements_match_reference(oracle, ref, tmp_path, ftype, fval):
+    """A resistive grid takes find_hypersurface_mhd (hypersurface.f90:152-166),
+    with B read at the resistive layout's slots.  The oracle evolves the grid
+    (the reference does not evolve the resistive system) and the reference's
+    finder gets the oracle's slices."""
+    ic, rc, v0 = rmhd_fireball(12, 10, 8, viscosity=0)
+    oracle.setup(ic, rc)
+    ref.setup(reference_resistive_config(ic), rc)
+    assert oracle.init_from_prim(v0) == 0
+    assert ref.init_from_prim(v0) == 0
+    ofo = OracleFO(oracle)
+    rfo = RefFO(ref, str(tmp_path / "ref"))
+    assert ofo.setup(ftype, fval)[0] == 0
+    rfo.setup(ftype, fval)
+    assert ofo.width == 16
+    opath = str(tmp_path / "orc.txt")
+    assert ofo.write(opath, 1) == 0
+    total = 0
+    for k in range(3):
+        if k:
+            assert oracle.steps(2) == 0
+        v, u = oracle.state()
+        ref.set_state(v, u)
+        ref.set_time(oracle.scalars()["t"])
+        e, n = ofo.slice()
+        assert e == 0 and rfo.slice() == 0
+        if k == 0:
+            continue
+        assert ofo.write(opath, 0) == 0
+        assert rfo.last_count() == n
+        a = ofo.elements()
+        b = rfo.elements(n, 16)
+        assert np.array_equal(a, b)
+        # B of the elements: interpolated between the B slots 5..7
+        assert np.abs(a[:, 13:16]).max() > 0
+        total += n
+    assert total > 20, total
+    ta = open(rfo.path, "rb").read().split(b"\n", 1)
+    tb = open(opath, "rb").read().split(b"\n", 1)
+    assert ta[1] == tb[1]
+    _same_header(ta[0], tb[0], ftype)
 
 
 def _same_header(ha, hb, ftype):

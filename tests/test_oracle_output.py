@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from libs import EOS_DIR
-from scenarios import fireball3d, mhd3d
+from scenarios import fireball3d, mhd3d, reference_resistive_config, rmhd_fireball
 
 ALL = np.ones(28, dtype=np.int32)
 ALL[27] = 0          # rc (comoving density) is not written
@@ -69,3 +69,32 @@ def test_frame_selection_subset(oracle, ref, tmp_path):
     path_o = str(tmp_path / "orc.dat")
     assert oracle_frame(oracle, sel, 8, 2, path_o, str(tmp_path / "time.dat")) == 0
     assert open(path_r, "rb").read() == open(path_o, "rb").read()
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_resistive_frame_bytes_match_reference(oracle, ref, tmp_path, prec):
+    """A resistive grid (B, E, psi after the fluid variables): the evolved E
+    is written as held (out.f90:404-413), not the ideal -v x B.  The oracle
+    evolves the grid; the reference's writer gets the oracle's state."""
+    ic, rc, v0 = rmhd_fireball(12, 10, 8, viscosity=0)
+    oracle.setup(ic, rc)
+    assert oracle.init_from_prim(v0) == 0
+    assert oracle.steps(2) == 0
+    ref.setup(reference_resistive_config(ic), rc)
+    assert ref.init_from_prim(v0) == 0
+    v, u = oracle.state()
+    ref.set_state(v, u)
+    ref.set_time(oracle.scalars()["t"])
+    path_r = ref_frame(ref, ALL, prec, 3, str(tmp_path / "ref"))
+    path_o = str(tmp_path / "orc.dat")
+    assert oracle_frame(oracle, ALL, prec, 3, path_o, str(tmp_path / "time.dat")) == 0
+    a = open(path_r, "rb").read()
+    b = open(path_o, "rb").read()
+    assert len(a) == len(b) and a == b
+    # the E fields of the frame are the state's E, which differs from -v x B
+    n = v[0].size
+    dt = np.float64 if prec == 8 else np.float32
+    f = np.fromfile(path_o, dtype=dt)[1:].reshape(-1, n)
+    assert np.array_equal(f[12:15], v[8:11].reshape(3, n).astype(dt))
+    ideal_ex = -(v[2] * v[7] - v[3] * v[6]).ravel()
+    assert np.abs(f[12] - ideal_ex).max() > 1e-3
