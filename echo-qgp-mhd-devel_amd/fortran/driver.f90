@@ -11,8 +11,9 @@
 ! EL_COND, CHIR_COND, bound_con, the output selection of out_vars (density..=
 ! .. glm......=, common.f90:1110-1190) with outdir...=, and for INIT_TYPE 3
 ! NUCLEUS (Au, Pb), RADS, SIGMA_IN, B, FIXED_B, NCONF, NBCOLL, MIN_PART,
-! EV_START, KAPPA, AH, SIG, DETA, SIGETA, RHOCENTER; other reference flags
-! are ignored.  Driver flags: GUB_T0... (T0 of the Gubser initialiser in
+! EV_START, KAPPA, AH, SIG, DETA, SIGETA, RHOCENTER, and on MHD grids the
+! protons' initial field MAGFIELDT, B_amp_fac, PW (common.f90:826-833);
+! other reference flags are ignored.  Driver flags: GUB_T0... (T0 of the Gubser initialiser in
 ! fm^-1; the reference fixes 2.0 at common.f90:1468), SEED.....= (seed of
 ! the Glauber-MC generator; the reference takes it from random_seed.dat or
 ! the clock), GLAUB_DIR= (directory of the event files, default .),
@@ -50,6 +51,7 @@ program echo_amd_driver
   integer(c_int64_t) :: seed
   integer(c_int32_t) :: sel(28), stopped
   type(echo_glauber_params) :: gpar
+  type(echo_bfield_params) :: bpar
   real(8) :: maxv
   integer(c_int32_t) :: iout
   integer(c_int64_t) :: nfo
@@ -89,6 +91,7 @@ program echo_amd_driver
   min_part = 18
   ev_start = 1
   gpar = echo_glauber_params(19d0, 0.15d0, 0.8d0, 0d0, 2d0, 1d0, 1d0)
+  bpar = echo_bfield_params(0d0, 1d0, 0.5d0, 1_c_int32_t)
   seed = 1
   glauber_dir = '.'
   temp_end = 0d0
@@ -180,6 +183,9 @@ program echo_amd_driver
     case ('SIGETA...='); read(val, *) gpar%sigeta
     case ('RHOCENTER='); read(val, *) gpar%rhocenter
     case ('SEED.....='); read(val, *) seed
+    case ('MAGFIELDT='); read(val, *) bpar%magfield_type
+    case ('B_amp_fac='); read(val, *) bpar%b_amplify
+    case ('PW.......='); read(val, *) bpar%pw
     case ('GLAUB_DIR='); glauber_dir = adjustl(val)
     case ('density..='); read(val, *) sel(1)
     case ('vx.......='); read(val, *) sel(2)
@@ -363,10 +369,14 @@ contains
 
   ! INIT_TYPE 3: the events of generate_events (echo_glauber_generate into
   ! GLAUB_DIR), event EV_START read back, its energy deposition on the device
-  ! (echo_init_glauber; ybeam = log(RADS / m_N), glaubermc.f90:741)
+  ! (echo_init_glauber; ybeam = log(RADS / m_N), glaubermc.f90:741); on MHD
+  ! grids with the field of the event's protons (echo_init_glauber_mhd,
+  ! init.f90:2796-2811 without a B-field file; gamma_col = RADS / (2 m_N),
+  ! common.f90:2261)
   subroutine glauber_init()
     type(echo_glauber_sampler) :: smp
     type(echo_glauber_event) :: ev
+    type(echo_nucleons) :: lists(4)
     integer(c_int32_t) :: nev
     integer, parameter :: cap = 600
     real(c_double), allocatable, target :: pxy(:), sxy(:), cxy(:)
@@ -403,9 +413,21 @@ contains
     write(*,'(a,i6,a,f8.3,a,i5,a,i6)') ' Glauber event', ev_start, '  b', ev%b, '  participants', &
         ev%n_part_a + ev%n_part_b, '  collisions', ev%n_coll
     gpar%ybeam = log(rads / am)
-    ierr = echo_init_glauber(grid, gpar, pxy, ev%n_part_a, pxy(2 * ev%n_part_a + 1:), ev%n_part_b, &
-                             cxy, ev%n_coll)
-    if (ierr /= 0) call fail('echo_init_glauber', ierr)
+    if (cfg%mhd == 0) then
+      ierr = echo_init_glauber(grid, gpar, pxy, ev%n_part_a, pxy(2 * ev%n_part_a + 1:), ev%n_part_b, &
+                               cxy, ev%n_coll)
+      if (ierr /= 0) call fail('echo_init_glauber', ierr)
+      return
+    end if
+    ! spectators A, participants A, spectators B, participants B
+    lists(1) = echo_nucleons(c_loc(sxy), c_loc(sty), ev%n_spec_a)
+    lists(2) = echo_nucleons(c_loc(pxy), c_loc(pty), ev%n_part_a)
+    lists(3) = echo_nucleons(c_loc(sxy(2 * ev%n_spec_a + 1)), c_loc(sty(ev%n_spec_a + 1)), ev%n_spec_b)
+    lists(4) = echo_nucleons(c_loc(pxy(2 * ev%n_part_a + 1)), c_loc(pty(ev%n_part_a + 1)), ev%n_part_b)
+    bpar%gamma_col = rads / (2d0 * am)
+    ierr = echo_init_glauber_mhd(grid, gpar, pxy, ev%n_part_a, pxy(2 * ev%n_part_a + 1:), ev%n_part_b, &
+                                 cxy, ev%n_coll, bpar, lists)
+    if (ierr /= 0) call fail('echo_init_glauber_mhd', ierr)
   end subroutine glauber_init
 
   subroutine fail(what, code)

@@ -197,3 +197,66 @@ def test_fortran_driver_glauber(oracle, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Energy density is under" in r.stdout
     assert (out2 / "out0001.dat").exists()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
+def test_fortran_driver_glauber_resistive(oracle, tmp_path):
+    """The driver on the viscous resistive system (MHD=2, VISCOUS=1,
+    INT_ALGO=SSP, tabulated EoS) from a generated Au-Au event with the field
+    of its protons (MAGFIELDT 1, B_amp_fac 1, PW 0.5: the reference's
+    defaults): frames with B and E, a restart dump at the output time, and a
+    final state equal to the oracle's run from the same event to 1e-10."""
+    from echoamd import glauber_read_event
+    from libs import make_config, oracle_glauber_bfield, oracle_glauber_prim
+    n, nz, steps = 40, 8, 4
+    gdir = tmp_path / "events"
+    gdir.mkdir()
+    out = tmp_path / "frames"
+    out.mkdir()
+    dump = tmp_path / "state.bin"
+    lines = ["INIT_TYPE=3", "COORD....=2", "VISCOUS..=1", "MHD......=2", "DIVCLEAN.=1", "ALG_SOLV.=7",
+             "EOS......=3", "EOS_FILE.=qcdIEOS0.dat", f"NX.......={n}", f"NY.......={n}", f"NZ.......={nz}",
+             "XMIN.....=-8.0", "XMAX.....=8.0", "YMIN.....=-8.0", "YMAX.....=8.0", "ZMIN.....=-2.0",
+             "ZMAX.....=2.0", "TSTART...=0.6", "CFL......=0.4", "MAXDT....=0.05", "REC_ALGO.=WENO5",
+             "INT_ALGO.=SSP", "bound_con=2", "ETA_S....=0.1", "NUCLEUS..=Au", "SIGMA_IN.=42.", "B........=6.0",
+             "FIXED_B..=1", "NCONF....=1", "KAPPA....=19.", "AH.......=0.15", "SIG......=0.4", "DETA.....=2.0",
+             "SIGETA...=0.8", "RHOCENTER=1.", "RADS.....=200.", "SEED.....=9", f"GLAUB_DIR={gdir}",
+             "MAGFIELDT=1", "B_amp_fac=1.", "PW.......=0.5", "DTOUT....=0.1", "RESTART..=1",
+             f"DUMP_DIR.={tmp_path}", f"outdir...={out}", "density..=1", "bx.......=1", "ex.......=1",
+             "OUTP_PREC=8", f"NSTEPS...={steps}", "CHUNK....=1", f"DUMP_FILE={dump}"]
+    pf = tmp_path / "param.dat"
+    pf.write_text("\n".join(lines) + "\n")
+    print("\nfortran driver: viscous resistive IMEX from a Glauber event", flush=True)
+    r = subprocess.run([DRIVER, str(pf)], cwd=PKG, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-800:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    ev = glauber_read_event(gdir, 1)
+    ic, rc = make_config(nx=n, ny=n, nz=nz, coordinates=2, viscosity=1, eqstate=3, rec="WENO5", nrk=4, bc=2,
+                         mhd=2, divclean=1, algo_solver=7, init_type=3, xlim=(-8.0, 8.0), ylim=(-8.0, 8.0),
+                         zlim=(-2.0, 2.0), tmin=0.6, cfl=0.4, maxdt=0.05, eta_over_s=0.1, dtout=0.1)
+    par = dict(kappa=19.0, ah=0.15, sig_mc=0.4, ybeam=np.log(200.0 / 0.93891897), deta=2.0, sigeta=0.8,
+               rhocenter=1.0)
+    bp = dict(gamma_col=200.0 / (2 * 0.93891897), b_amplify=1.0, pw=0.5, magfield_type=1)
+    lists = [(ev["spec_a"], ev["type_spec_a"]), (ev["part_a"], ev["type_part_a"]),
+             (ev["spec_b"], ev["type_spec_b"]), (ev["part_b"], ev["type_part_b"])]
+    oracle.setup(ic, rc)
+    e, v0 = oracle_glauber_prim(oracle, par, ev["part_a"], ev["part_b"], ev["coll"])
+    assert e == 0
+    assert oracle_glauber_bfield(oracle, bp, lists, v0) == 0
+    assert np.abs(v0[12:15]).max() > 0.0
+    assert oracle.init_from_prim(v0) == 0
+    assert oracle.steps(steps) == 0
+    vo, uo = oracle.state()
+    raw = np.fromfile(dump, dtype=np.float64)
+    m = 19 * n * n * nz
+    v = raw[:m].reshape(19, nz, n, n)
+    u = raw[m:].reshape(19, nz, n, n)
+    assert rel_linf(v, vo) <= 1e-10 and rel_linf(u, uo) <= 1e-10
+    # frames: t, density, B^x, E^x
+    f0 = np.fromfile(out / "out0000.dat", dtype=np.float64)
+    assert f0.size == 1 + 3 * n * n * nz
+    # device and glibc atan2, sin, cos, exp, pow differ in the last bits
+    bx = f0[1 + n * n * nz:1 + 2 * n * n * nz]
+    assert np.abs(bx - v0[12].ravel()).max() <= 1e-12 * np.abs(v0[12]).max()
+    assert not f0[1 + 2 * n * n * nz:].any()          # E starts at zero
+    assert (tmp_path / "dump.dat").exists()
