@@ -379,6 +379,8 @@ int echo_set_c2p_net(echo_grid* g, echo_mlp* m, int32_t iters, double tol) {
     if (g->dev != m->dev) return fail(ECHO_E_ARG, "network and grid on different devices");
     if (!surrogate_config_ok(g->cfg))
       return fail(ECHO_E_ARG, "the learned inversion covers ideal MHD with p = e/3 (mhd 1, algo_solver 5, eqstate 1)");
+    if (g->cfg.nrk == 4)
+      return fail(ECHO_E_ARG, "the learned inversion runs in the RK2 / RK3 steps (not IMEX-SSP3(4,3,3))");
     if (iters < 0 || iters > 8 || !(tol > 0.0)) return fail(ECHO_E_ARG, "need 0 <= iters <= 8 and tol > 0");
     if (g->N > 0x7fffffffLL) return fail(ECHO_E_ARG, "at most 2^31-1 cells");
     if (!g->net_scratch && hipMalloc(&g->net_scratch, sizeof(int) * (size_t)(g->N + 1)) != hipSuccess) {

@@ -112,16 +112,55 @@ static void enqueue_stage_t(echo_grid* g, int irk) {
                                                     g->VS[1], last ? 1 : 0, ImexView{nullptr, nullptr, 0}, 1);
 }
 
-// one stage of IMEX-SSP3(4,3,3) for the resistive systems (kernels.hpp:
-// k_imex_k1, k_imex_k2, k_update_imex).  The launches are shifted against
-// the scheme's stages: stage 1 sets dt and solves the implicit parts of K1
-// and K2 (no sweep: the explicit rate of K1 has weight zero); stages 2 and 3
-// sweep K2 (K3), form their rates and solve K3 (K4); stage 4 sweeps K4 and
-// forms the step's result.  Each launch that inverts is followed by the
-// repair of the cells it could not invert.
+// one stage of IMEX-SSP3(4,3,3) (kernels.hpp: k_imex_k1, k_imex_k2,
+// k_imex_u2v, k_update_imex).  The launches are shifted against the scheme's
+// stages: stage 1 sets dt and solves the implicit parts of K1 and K2 (the
+// explicit rate of K1 has weight zero: on the resistive systems, which clean
+// at the speed of light, it needs no sweep; on the others the sweep of K1
+// gives maxval(aflux), the cleaning speed of stage 2, and K1 = K2 = u0 is
+// inverted once); stages 2 and 3 sweep K2 (K3), form their rates and solve
+// K3 (K4); stage 4 sweeps K4 and forms the step's result.  Each launch that
+// inverts is followed by the repair of the cells it could not invert.
 template <int SYS, bool ZZ>
 static void enqueue_imex_stage_t(echo_grid* g, int irk) {
-  if constexpr (SysT<SYS>::RES) {
+  if constexpr (!SysT<SYS>::RES) {
+    double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
+    double* Ust = g->U[1];
+    auto fixup = [&](double* U, int last, int phase) {
+      k_fixup<SYS, ZZ><<<1, FIX_BLOCK, 0, g->stream>>>(g->ga, g->V, U, g->fix, g->VS[0], g->VS[1], last,
+                                                        ImexView{g->U[0], g->imex.P4, phase}, last);
+    };
+    if (g->timing) hipEventRecord(g->ev[0], g->stream);
+    if (g->active[0]) launch_flux_dir<1, SYS, ZZ>(g, part[0]);
+    if (g->timing) hipEventRecord(g->ev[1], g->stream);
+    if (g->active[1]) launch_flux_dir<2, SYS, ZZ>(g, part[1]);
+    if (g->timing) hipEventRecord(g->ev[2], g->stream);
+    if (g->active[2]) launch_flux_dir<3, SYS, ZZ>(g, part[2]);
+    if (g->timing) hipEventRecord(g->ev[3], g->stream);
+    // dt from the speeds of the step's first sweeps, as in the RK steps
+    if (irk == 1) k_cfl<<<1, 1024, 0, g->stream>>>(g->ga, g->partial, g->nb[0], g->nb[1], g->nb[2]);
+    if (g->timing) hipEventRecord(g->ev[4], g->stream);
+    if (irk == 1) {
+      // K2 = u0 (evolve.f90:232-233): its inversion; the repair also sets
+      // the cleaning speed of stage 2 from this stage's sweeps
+      k_imex_u2v<SYS, ZZ><<<nblocks_xcd(g->N, UPDATE_BLOCK), UPDATE_BLOCK, 0, g->stream>>>(g->ga, g->V, g->U[0],
+                                                                                          g->fix);
+      if (g->timing) hipEventRecord(g->ev[5], g->stream);
+      fixup(Ust, 0, 2);
+      return;
+    }
+#define ECHO_IMEX_UPDATE(ST)                                                                        \
+  k_update_imex<SYS, ZZ, ST><<<update_blocks(g), UPDATE_BLOCK, 0, g->stream>>>(                    \
+      g->ga, g->V, g->U[0], Ust, g->F[0], g->F[1], g->F[2], g->L[0], g->L[1], g->L[2], g->VS[0], \
+      g->VS[1], g->imex, g->fix)
+    if (irk == 2) ECHO_IMEX_UPDATE(2);
+    else if (irk == 3) ECHO_IMEX_UPDATE(3);
+    else ECHO_IMEX_UPDATE(4);
+#undef ECHO_IMEX_UPDATE
+    if (g->timing) hipEventRecord(g->ev[5], g->stream);
+    if (irk == 4) fixup(g->U[0], 1, 0);
+    else fixup(Ust, 0, irk + 1);
+  } else {
     const unsigned nbu = nblocks_xcd(g->N, UPDATE_BLOCK);
     double* part[3] = {g->partial, g->partial + g->nb[0], g->partial + g->nb[0] + g->nb[1]};
     double* Ust = g->U[1];
@@ -337,9 +376,8 @@ static int grid_create(const echo_config* cfg, const double* eos_rows, int32_t n
   if (c.derivatives_out)
     return fail(ECHO_E_ARG, "derivativ = 1: the derivative output of out_derivatives (out.f90:860-1250) "
                             "is not built");
-  if (c.nrk == 4 && c.mhd != 2)
-    return fail(ECHO_E_ARG, "INT_ALGO 4 (IMEX-SSP3(4,3,3)) is built for resistive MHD (MHD=2); "
-                            "use RK2 (2) or RK3 (3)");
+  if (c.nrk == 4 && !c.mhd)
+    return fail(ECHO_E_ARG, "INT_ALGO 4 (IMEX-SSP3(4,3,3)) is for MHD grids (common.f90:2076-2083)");
   if (c.nrk != 2 && c.nrk != 3 && c.nrk != 4)
     return fail(ECHO_E_ARG, "INT_ALGO must be RK2 (2), RK3 (3) or IMEX-SSP3(4,3,3) (4)");
   if (c.boundary_cond != 0 && c.boundary_cond != 2)

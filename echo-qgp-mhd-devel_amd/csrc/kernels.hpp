@@ -713,7 +713,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 
 // ---------------------------------------------------------------------------
 // IMEX-SSP3(4,3,3) (Pareschi & Russo; coefficients of evolve.f90:127-130,
-// stages of evolve.f90:204-264) for the resistive systems.  With du the
+// stages of evolve.f90:204-264).  With du the
 // explicit rate (flux divergence minus the non-stiff sources) and ds the
 // stiff one (minus Ohm's current, only E), u = u0 - dt * (...):
 //   K1 = u0                                  + a dt R(K1)
@@ -726,6 +726,10 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update(
 // start U* for the primitives and E; U keeps U*, V gets the stage's
 // primitives.  The kernels below are shifted against the stages: each
 // finishes the rates of one stage and solves the next stage's implicit part.
+// On the other systems (hydro, viscous hydro, ideal MHD: no stiff term,
+// ds = 0, as the reference runs INT_ALGO SSP there) every "implicit solve" is
+// the plain inversion of the stage start, and the first stage sweeps only for
+// maxval(aflux), the cleaning speed of the second (evolve.f90:225-231).
 // ---------------------------------------------------------------------------
 constexpr double IMEX_A = 0.24169426078821, IMEX_B = 0.06042356519705, IMEX_H = 0.12915286960590;
 constexpr double IMEX_D = 0.5 - IMEX_B - IMEX_H - IMEX_A;
@@ -817,6 +821,32 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_k2(GridArgs a, double* __
   imex_solve<SYS, ZZ, true>(a, M, c, V, Ust, fb, us, v, IMEX_A * dt);
 }
 
+// systems without a stiff term, second stage: V from u0 (evolve.f90:232-233,
+// u = u0 + a dt ds1 with ds1 = 0); U keeps u0
+template <int SYS, bool ZZ>
+__global__ __launch_bounds__(UPDATE_BLOCK) void k_imex_u2v(GridArgs a, double* __restrict__ V,
+                                                           const double* __restrict__ U0, FixBufs fb) {
+  constexpr int NV = SysT<SYS>::NV;
+  const long long c = xcd_block(blockIdx.x, gridDim.x) * UPDATE_BLOCK + threadIdx.x;
+  if (c >= a.N || a.S->temp_check || a.S->run_crashed) return;
+  const long long N = a.N;
+  const Metric M = make_metric(a.coordinates, a.S->t);
+  double u[NV], v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    u[k] = U0[(long long)k * N + c];
+    v[k] = V[(long long)k * N + c];
+  }
+  __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
+  const int e = cons2prim<SYS, ZZ>(a.eos, M, u, v, a.algo_solver);
+  if (e != 0) {
+    atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+    atomicAdd(&a.S->nfail, 1);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) __builtin_nontemporal_store(v[k], &V[(long long)k * N + c]);
+}
+
 // stages 2-4 after their sweeps: the rates of stage STAGE at K_STAGE
 // (flux divergence, non-stiff sources with the charge density div E, stiff
 // rate), their accumulation, and K_{STAGE+1} (STAGE 2, 3) or the step's
@@ -848,13 +878,18 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
   double* VSn = par ? VS1 : VS0;
   double* VSo = par ? VS0 : VS1;
   if constexpr (VISC) D = cell_deriv(a, M, c, x, y, z, LX, LY, LZ, VSn, VSo);
-  double ds[3];
+  double ds[3] = {0., 0., 0.};
   {
-    double vv[NV], src[NV], sst[3];
+    double vv[NV], src[NV], sst[3] = {0., 0., 0.};
 #pragma unroll
     for (int k = 0; k < NV; k++) vv[k] = v[k];
-    const double rhoc = res_div_e(a, x, y, z, FX, FY, FZ, NV);
-    const int e = res_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, rhoc, a.sig, a.sch, src, sst);
+    int e;
+    if constexpr (T::RES) {
+      const double rhoc = res_div_e(a, x, y, z, FX, FY, FZ, NV);
+      e = res_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, rhoc, a.sig, a.sch, src, sst);
+    } else {
+      e = sys_source<SYS, ZZ>(a.eos, M, a.coordinates, a.vp, vv, D, src);
+    }
     if (e > 0) {
       atomicMax(&S->errcode, e);
       atomicExch(&S->run_crashed, 1);
@@ -875,7 +910,7 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     const int ke = k - T::KE;
-    const bool isE = ke >= 0 && ke < 3;
+    const bool isE = T::RES && ke >= 0 && ke < 3;
     const long long o = (long long)k * N + c;
     if (isE) {
       const double s = ds[ke];
@@ -902,10 +937,20 @@ __global__ __launch_bounds__(UPDATE_BLOCK) void k_update_imex(
       }
     }
   }
-  if constexpr (STAGE < 4) {
+  if constexpr (STAGE < 4 && T::RES) {
     imex_solve<SYS, ZZ, true>(a, M, c, V, Ust, fb, un, v, IMEX_A * dt);
+  } else if constexpr (STAGE < 4) {
+    // no stiff term: the plain inversion of K_{STAGE+1} (evolve.f90:241,250)
+    __builtin_nontemporal_store(v[KPR], &fb.Pold[c]);
+    const int e = cons2prim<SYS, ZZ>(a.eos, M, un, v, a.algo_solver);
+    if (e != 0) {
+      atomicOr(&fb.failbits[c >> 5], 1u << (c & 31));
+      atomicAdd(&S->nfail, 1);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) __builtin_nontemporal_store(v[k], &V[(long long)k * N + c]);
   } else {
-    {
+    if constexpr (T::GLM) {
       const double ch = S->glm_ch;
       un[T::KG] = un[T::KG] * exp(-(ch * ch) * a.glm_alpha * (dt / 2.));
     }
@@ -969,7 +1014,7 @@ __device__ __forceinline__ double imex_ustar(const ImexView& iv, const double* _
                                              long long q, int k, long long N, double dt) {
   constexpr int KE = SysT<SYS>::KE;
   const long long o = (long long)k * N + q;
-  const bool isE = k >= KE && k < KE + 3;
+  const bool isE = SysT<SYS>::RES && k >= KE && k < KE + 3;
   if (repaired || (isE && iv.phase > 1)) return U[o];
   if (iv.phase <= 2) return iv.U0[o];
   const double r = iv.phase == 3 ? 4.0 * iv.P4[o] : iv.P4[o];
@@ -1020,11 +1065,11 @@ __device__ void repair_cell(const GridArgs& a, const Metric& M, long long c, dou
     double w[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) w[j] = us[j][o + nb[j]];
-    if constexpr (SysT<SYS>::RES) {
-      if (imex) {
+    if (imex) {
 #pragma unroll
-        for (int j = 0; j < 6; j++) w[j] = imex_ustar<SYS>(iv, U, repaired[j], nb[j], k, N, dt);
-      }
+      for (int j = 0; j < 6; j++) w[j] = imex_ustar<SYS>(iv, U, repaired[j], nb[j], k, N, dt);
+    }
+    if constexpr (SysT<SYS>::RES) {
       constexpr int KE = SysT<SYS>::KE;
       if (imex && k >= KE && k < KE + 3) {
 #pragma unroll

@@ -54,7 +54,7 @@ typedef struct echo_config {
   int32_t recal;               /* [REC_ALGO] 1 TVD2 2 CENO3 3 WENO3 4 WENO5
                                   5 PPM4 6 MPE3 7 MPE5 */
   int32_t nrk;                 /* [INT_ALGO] 2 RK2, 3 RK3, 4 IMEX-SSP3(4,3,3)
-                                  (MHD = 2 only) */
+                                  (MHD grids, as common.f90:2076-2083) */
   int32_t boundary_cond;       /* [BOUND_CON] 0 periodic, 2 outflow */
   int32_t algo_solver;         /* [ALG_SOLV] MHD con2prim: 4 ideal gas
                                   (EQ_STATE 2), 5 p=e/3 (EQ_STATE 1), 6 p=e/3

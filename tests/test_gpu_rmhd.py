@@ -167,10 +167,6 @@ def test_unsupported_options_rejected():
     bad[7] = 0   # no divergence cleaning
     with pytest.raises(EchoError, match="DIVCLEAN"):
         _grid(bad, rc)
-    g = _grid(ic, rc)
-    with pytest.raises(EchoError, match="restart dump"):
-        g.set_dump(True)
-    g.close()
 
 
 def test_resistive_events_in_ensemble(oracle):
