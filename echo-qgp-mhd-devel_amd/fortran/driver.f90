@@ -6,7 +6,7 @@
 ! Supported reference flags: INIT_TYPE (3 Glauber Monte Carlo, 4 ideal
 ! Gubser flow), COORD, VISCOUS, BULK, MHD, DIVCLEAN, GLM_PARAM, CUT_TEMP, NX,
 ! NY, NZ, XMIN..ZMAX, TSTART, TSTOP, TEMP_END, DTOUT, OUTP_PREC, MAXDT, CFL,
-! REC_ALGO (default MPE5), INT_ALGO (RK2 default, RK3, SSP for MHD=2),
+! REC_ALGO (default MPE5), INT_ALGO (RK2 default, RK3, SSP on MHD grids),
 ! MAXSPEED, ENEZERO, ETA_S, TAU_PI_C, TRACE_IMP, EOS, EOS_FILE, ALG_SOLV,
 ! EL_COND, CHIR_COND, bound_con, the output selection of out_vars (density..=
 ! .. glm......=, common.f90:1110-1190) with outdir...=, and for INIT_TYPE 3
