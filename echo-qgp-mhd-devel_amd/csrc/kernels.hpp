@@ -1166,13 +1166,21 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
     }
   }
   __syncthreads();
-  // 2. dependency levels, in cell order: one more than the highest level of
-  //    the failed face neighbours that come earlier (x-1, y-1, z-1)
-  if (t == 0) {
-    int maxlev = 0;
-    const int nx = a.nx, ny = a.ny;
-    const long long sxy = (long long)nx * ny;
-    for (int i = 0; i < n; i++) {
+  // 2. dependency levels: one more than the highest level of the failed face
+  //    neighbours that come earlier (x-1, y-1, z-1), i.e. the longest chain
+  //    of earlier failed neighbours ending at the cell.  All threads relax
+  //    the levels from zero until a pass changes none; the values only grow
+  //    and never pass the fixed point, which the pass that changes nothing
+  //    has reached (at most one pass per level, plus one).
+  const int nx = a.nx, ny = a.ny;
+  const long long sxy = (long long)nx * ny;
+  for (int i = t; i < n; i += FIX_BLOCK) fb.level[fb.list[i]] = 0;
+  if (t == 0) { sh_maxlev = 0; sh_stop = 0; }
+  __syncthreads();
+  int mylev = 0;
+  for (;;) {
+    int changed = 0;
+    for (int i = t; i < n; i += FIX_BLOCK) {
       const long long c = fb.list[i];
       const int x = (int)(c % nx);
       const long long r = c / nx;
@@ -1181,12 +1189,15 @@ __global__ __launch_bounds__(FIX_BLOCK) void k_fixup(GridArgs a, double* __restr
       if (x > 0 && fix_failed(fb.failbits, c - 1)) lev = max(lev, fb.level[c - 1] + 1);
       if (y > 0 && fix_failed(fb.failbits, c - nx)) lev = max(lev, fb.level[c - nx] + 1);
       if (z > 0 && fix_failed(fb.failbits, c - sxy)) lev = max(lev, fb.level[c - sxy] + 1);
-      fb.level[c] = lev;
-      maxlev = max(maxlev, lev);
+      if (lev > fb.level[c]) {
+        fb.level[c] = lev;
+        changed = 1;
+      }
+      mylev = max(mylev, lev);
     }
-    sh_maxlev = maxlev;
-    sh_stop = 0;
+    if (!__syncthreads_or(changed)) break;
   }
+  atomicMax(&sh_maxlev, mylev);
   __syncthreads();
   // 3. the repairs, one level after the other.  A failed repair crashes the
   //    run; the reference returns from evolve_u2v there, leaving the later
