@@ -150,3 +150,33 @@ def test_repairs_in_ensemble(oracle):
     assert g2.repairs() == oracle.repair_counts()["u2v"] > 0
     assert rel_linf(v2, vo) <= TOL and rel_linf(u2, uo) <= TOL
     ens.close()
+
+
+# a 4 x 4 x 4 block of superluminal cells: chains of up to ten dependent
+# repairs per stage (levels 0..9), where each cell waits for its failed
+# neighbours at x-1, y-1, z-1 (the oracle's spread under 1e-16 perturbations
+# of these inputs is 2e-13)
+BLOCK = tuple((z, y, x) for z in range(2, 6) for y in range(3, 7) for x in range(3, 7))
+
+
+@pytest.mark.parametrize("name", ["forced-idealgas-tvd2", "forced-viscous-weno5"])
+def test_forced_block_chains(oracle, name):
+    ic, rc, v0 = forced_background(**FORCED[name])
+    oracle.setup(ic, rc)
+    assert oracle.init_from_prim(v0) == 0
+    v, u = oracle.state()
+    u = force_superluminal(u, cells=BLOCK)
+    oracle.set_state(v, u)
+    assert oracle.steps(2) == 0
+    cnt = oracle.repair_counts()
+    assert cnt["u2v"] >= len(BLOCK)
+    print(f"\n{name}: {cnt['u2v']} repairs", flush=True)
+    g = _grid(ic, rc)
+    assert g.init_from_prim(v0) == 0
+    g.upload(v, u)
+    g.steps(2)
+    vg, ug = g.state()
+    vo, uo = oracle.state()
+    assert g.repairs() == cnt["u2v"]
+    assert rel_linf(vg, vo) <= TOL and rel_linf(ug, uo) <= TOL, (rel_linf(vg, vo), rel_linf(ug, uo))
+    g.close()
